@@ -1,0 +1,119 @@
+"""Loader for ``libcostal_hip.so``, the gfx950 library behind ``include/costal_hip.h``.
+
+There is deliberately no CPU fallback: if the library is not built, or no GPU is present, calls
+fail loudly with the library's own error.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcostal_hip.so")
+
+# error codes (include/costal_hip.h)
+OK = 0
+E_SHAPE, E_BAD_K, E_TOO_MANY_ROWS, E_NON_FINITE, E_UNAVAILABLE, E_DEVICE, E_UNSUPPORTED, E_INVALID = range(1, 9)
+
+SELECT = {"cosine": 0, "tanimoto": 1}
+PRECISION = {"fp32": 0, "fp16": 1}
+WEIGHT = {"tanimoto": 0, "cosine": 1, "jaccard": 2}
+
+# every symbol the header declares
+EXPORTS = (
+    "costal_knn_exact",
+    "costal_knn_exact_device",
+    "costal_knn_bruteforce",
+    "costal_reweight",
+    "costal_reweight_device",
+    "costal_fuzzy_union",
+    "costal_last_error",
+    "costal_version",
+    "costal_device_info",
+)
+
+
+class KnnParams(C.Structure):
+    _fields_ = [
+        ("k", C.c_uint64),
+        ("margin", C.c_int64),
+        ("select", C.c_uint32),
+        ("precision", C.c_uint32),
+        ("row_block", C.c_uint64),
+        ("col_block", C.c_uint64),
+        ("col_splits", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("row_threshold", C.c_void_p),
+    ]
+
+
+class KnnStats(C.Structure):
+    _fields_ = [
+        ("fallback_rows", C.c_uint64),
+        ("row_block", C.c_uint64),
+        ("col_block", C.c_uint64),
+        ("blocks_skipped", C.c_uint64),
+        ("blocks_total", C.c_uint64),
+        ("candidates", C.c_uint64),
+        ("col_splits", C.c_uint64),
+        ("short_rows", C.c_uint64),
+        ("path", C.c_uint64),
+        ("device_ms", C.c_double),
+    ]
+
+
+class CostalError(ValueError):
+    """An error returned by the library; ``code`` is one of the E_* values."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load the library (once). Raises ImportError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(hipcc --offload-arch=gfx950)"
+        )
+    L = C.CDLL(LIB_PATH)
+    P, U64, U32 = C.c_void_p, C.c_uint64, C.c_uint32
+    L.costal_last_error.restype = C.c_char_p
+    L.costal_version.restype = C.c_char_p
+    L.costal_device_info.argtypes = [C.c_char_p, C.c_size_t]
+    L.costal_knn_exact.argtypes = [P, U64, U64, C.POINTER(KnnParams), P, P, P, P, C.POINTER(KnnStats)]
+    L.costal_knn_exact_device.argtypes = [P, U64, U64, C.POINTER(KnnParams), P, P, P, P,
+                                          C.POINTER(KnnStats), P]
+    L.costal_knn_bruteforce.argtypes = [P, U64, U64, U64, U32, P, P, P]
+    L.costal_reweight.argtypes = [U64, U64, P, P, P, U32, P]
+    L.costal_reweight_device.argtypes = [U64, U64, P, P, P, U32, P, P]
+    L.costal_fuzzy_union.argtypes = [U64, U64, P, P, P, P, P, C.POINTER(C.c_uint64)]
+    for name in EXPORTS:
+        getattr(L, name).restype = getattr(L, name).restype or C.c_int
+    L.costal_last_error.restype = C.c_char_p
+    L.costal_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != OK:
+        raise CostalError(rc, load().costal_last_error().decode())
+
+
+def version() -> str:
+    return load().costal_version().decode()
+
+
+def device_info() -> str:
+    buf = C.create_string_buffer(256)
+    check(load().costal_device_info(buf, 256))
+    return buf.value.decode()

@@ -1,0 +1,1133 @@
+// costal_hip.hip -- exact cosine kNN + Tanimoto edge weights for MI355X (gfx950).
+//
+// The path (reference = ike12th/CosTaL 2.1.0, crates/costal-core/src/knn.rs):
+//
+//   prep_kernel      row norms in f64 with a sequential sum (knn.rs:289-296, 315-323), the
+//                    L2-normalized f32 working copy (knn.rs:420-436) and its FP16 copy, and the
+//                    first non-finite row (knn.rs:325-342).
+//   coarse_kernel    the fused blocked coarse selection (knn.rs:603-660, 759-875): MFMA similarity
+//                    tiles of 32 candidates x 32 queries, staged through LDS, never written to HBM;
+//                    each query keeps its best K' = min(k+margin, n-1) candidates in LDS.
+//                    FP32 form: v_mfma_f32_32x32x2f32 on the f32 working copy.
+//                    FP16 form: v_mfma_f32_32x32x16_f16 on the FP16 copy; a candidate that clears
+//                    the list threshold minus the FP16 error bound is re-scored in f32 before it
+//                    is admitted, so the lists hold f32-accurate scores either way.
+//   rescore_kernel   finalize_row (knn.rs:688-720): f64 rescoring over the original rows,
+//                    ranking by (score desc, index asc), the epsilon certificate; rows that fail it
+//                    are queued for the full-row path.  Fuses the Tanimoto weight (weight.rs:28-37).
+//   rowscan_kernel   gather_row (knn.rs:723-743) and the brute force (knn.rs:884-924): one workgroup
+//                    streams every candidate of a row, keeps an exact running top-k in LDS.
+//   reweight_kernel  weight::reweight (weight.rs:41-60).
+//
+// Exactness: every admitted candidate is re-scored in f64 exactly as the reference does; a
+// candidate that was never admitted has coarse score <= the row's floor, hence exact score
+// <= floor + eps; a row whose k-th f64 score is not strictly above floor + eps is recomputed by the
+// full-row path.  So idx/cos equal the f64 brute force bit for bit for any tile shape, precision or
+// admission threshold.
+//
+// Built with -ffp-contract=off: the f64 expressions (cos64, tani_from_cos) must round as written.
+// f32 inner products that only feed the coarse stage use explicit fmaf.
+
+#include <hip/hip_runtime.h>
+
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/costal_hip.h"
+
+#define COSTAL_VERSION "costal-mi355x 0.1.0 (gfx950)"
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+namespace {
+
+constexpr int PREC_F32 = COSTAL_PREC_FP32;
+constexpr int PREC_F16 = COSTAL_PREC_FP16;
+constexpr uint32_t NONE = 0xffffffffu;
+
+// ------------------------------------------------------------------------------------------------
+// Scalar f64 primitives (bit-exact with the reference)
+
+// knn.rs:289-296: f64 sum in index order.  f32*f32 is exact in f64, so mul+add == fma here.
+__device__ __forceinline__ double dot64(const float* __restrict__ a, const float* __restrict__ b,
+                                        uint32_t d) {
+  double s = 0.0;
+  for (uint32_t t = 0; t < d; ++t) s += (double)a[t] * (double)b[t];
+  return s;
+}
+
+// knn.rs:300-312
+__device__ __forceinline__ double cos64(double dot, double ni, double nj) {
+  if (ni == 0.0 || nj == 0.0) return 0.0;
+  double c = dot / (ni * nj);
+  return c == 0.0 ? 0.0 : c;
+}
+
+// weight.rs:28-37 (evaluation order kept: (a*b)*cos, then ((a*a)+(b*b))-d)
+__device__ __forceinline__ double tani_from_cos(double a, double b, double c) {
+  double d = a * b * c;
+  if (d == 0.0) return 0.0;
+  return d / (a * a + b * b - d);
+}
+
+// f32 inner product of two zero-padded normalized rows, sequential fma: |err| <= gamma_d.
+__device__ __forceinline__ float dot32(const float* __restrict__ a, const float* __restrict__ b,
+                                       uint32_t dp) {
+  float s = 0.0f;
+  const f32x4* a4 = reinterpret_cast<const f32x4*>(a);
+  const f32x4* b4 = reinterpret_cast<const f32x4*>(b);
+  for (uint32_t t = 0; t < dp / 4; ++t) {
+    f32x4 u = a4[t], v = b4[t];
+    s = __builtin_fmaf(u[0], v[0], s);
+    s = __builtin_fmaf(u[1], v[1], s);
+    s = __builtin_fmaf(u[2], v[2], s);
+    s = __builtin_fmaf(u[3], v[3], s);
+  }
+  return s;
+}
+
+// (score desc, index asc): the order of sort_cands (knn.rs:589-591).  Scores are never NaN and
+// -0.0 never occurs (cos64 normalizes it), so '>' agrees with total_cmp.
+__device__ __forceinline__ bool ranks_before(double sa, uint32_t ia, double sb, uint32_t ib) {
+  return sa > sb || (sa == sb && ia < ib);
+}
+
+// ------------------------------------------------------------------------------------------------
+// prep: norms, normalized copies, non-finite check.  One thread per row; rows in [n, n_pad) are
+// written as zeros so that full tiles can be read without bounds checks.
+
+__global__ void prep_kernel(const float* __restrict__ x, uint32_t n, uint32_t d, uint32_t n_pad,
+                            uint32_t dp32, uint32_t dp16, double* __restrict__ norms,
+                            float* __restrict__ xs32, _Float16* __restrict__ xs16,
+                            uint32_t* __restrict__ bad_row) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_pad) return;
+  const uint32_t dmax = dp32 > dp16 ? dp32 : dp16;
+  if (i >= n) {
+    for (uint32_t t = 0; t < dp32; ++t) xs32[(size_t)i * dp32 + t] = 0.0f;
+    if (xs16)
+      for (uint32_t t = 0; t < dp16; ++t) xs16[(size_t)i * dp16 + t] = (_Float16)0.0f;
+    return;
+  }
+  const float* row = x + (size_t)i * d;
+  double s = 0.0;
+  bool bad = false;
+  for (uint32_t t = 0; t < d; ++t) {
+    float v = row[t];
+    bad |= !isfinite(v);
+    s += (double)v * (double)v;
+  }
+  const double nrm = sqrt(s);
+  norms[i] = nrm;
+  if (bad) atomicMin(bad_row, i);
+  for (uint32_t t = 0; t < dmax; ++t) {
+    float v = (t < d && nrm > 0.0) ? (float)((double)row[t] / nrm) : 0.0f;
+    if (t < dp32) xs32[(size_t)i * dp32 + t] = v;
+    if (xs16 && t < dp16) xs16[(size_t)i * dp16 + t] = (_Float16)v;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// coarse_kernel
+//
+// Workgroup = NW waves; each wave owns 32 queries, held in registers for the whole launch.  The
+// workgroup streams its candidate range through LDS in tiles of TC rows (double buffered, one
+// barrier per tile).  For every 32-candidate sub-tile a wave issues the MFMAs for
+// D[cand][query] = sum_k cand[k] * query[k]: lane l owns query (l % 32) and the 16 candidates
+// 8*(i/4) + 4*(l/32) + (i%4), i = 0..15 (the 32x32 accumulator layout).  The scan reduces those 16
+// values with max3 and compares once with the query's threshold; only waves with a hit take the
+// insert path.
+//
+// Row layout (global and LDS): a row is 2*J 16-byte chunks; chunk q feeds lane half h = q % 2,
+// slot j = q / 2.  Queries and candidates use the same chunk -> (lane half, slot) map, so each
+// MFMA contracts matching feature indices (any consistent permutation of k leaves the sum a sum of
+// the same products).  In LDS the chunks of one (sub-tile, slot, half) are stored for the 32 lanes
+// contiguously, so each ds_read_b128 is conflict free.
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Insert into a query's candidate list (capacity kp).  Returns the new threshold: t0 while the
+// list is not full, else max(t0, smallest score in the list).
+__device__ __forceinline__ float list_insert(float* __restrict__ sc, uint32_t* __restrict__ id,
+                                             int* cnt_p, int* minpos_p, float* th_p, float v,
+                                             uint32_t cand, int kp, float t0) {
+  int c = *cnt_p;
+  if (c < kp) {
+    sc[c] = v;
+    id[c] = cand;
+    ++c;
+    *cnt_p = c;
+    if (c < kp) return t0;
+  } else {
+    int mp = *minpos_p;
+    sc[mp] = v;
+    id[mp] = cand;
+  }
+  float mn = sc[0];
+  int mp = 0;
+  for (int e = 1; e < kp; ++e) {
+    float s = sc[e];
+    if (s < mn) {
+      mn = s;
+      mp = e;
+    }
+  }
+  *minpos_p = mp;
+  float th = fmaxf(t0, mn);
+  *th_p = th;
+  return th;
+}
+
+template <int PREC, int J, int KP, int NW>
+__global__ __launch_bounds__(64 * NW) void coarse_kernel(
+    const uint8_t* __restrict__ xsq,   // n_pad rows of 32*J bytes (f32 or f16 working copy)
+    const float* __restrict__ xs32,    // f32 working copy (dp32 per row), for the FP16 re-score
+    uint32_t dp32, uint32_t n, int kp,
+    const uint32_t* __restrict__ rowlist, uint32_t nq,  // queries: rowlist[0..nq) or rows 0..nq-1
+    const float* __restrict__ tinit,   // per-row admission threshold (by row) or null
+    uint32_t cols_per_split,           // candidate columns per split, a multiple of the tile
+    float eps_pre,                     // FP16 form: bound on |FP16 score - f32 score|
+    uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc, uint32_t* __restrict__ cand_cnt,
+    float* __restrict__ cand_floor) {
+  constexpr int QB = 32 * NW;
+  constexpr int TC = (J <= 4) ? 128 : 64;
+  constexpr int SUBT = TC / 32;
+  constexpr int ROWB = 32 * J;
+  constexpr int TILEB = TC * ROWB;
+  constexpr int CHUNKS = TC * 2 * J;
+  constexpr int NT = 64 * NW;
+  constexpr int CPT = (CHUNKS + NT - 1) / NT;
+
+  __shared__ __attribute__((aligned(16))) uint8_t tile[2][TILEB];
+  __shared__ float l_sc[QB * KP];
+  __shared__ uint32_t l_id[QB * KP];
+  __shared__ int l_cnt[QB];
+  __shared__ int l_minpos[QB];
+  __shared__ float l_th[QB];
+
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, l = tid & 63, h = l >> 5, l32 = l & 31;
+  const int qloc = w * 32 + l32;
+  const uint32_t qq = blockIdx.x * QB + qloc;
+  const bool qvalid = qq < nq;
+  const uint32_t qg = qvalid ? (rowlist ? rowlist[qq] : qq) : 0;
+  const uint32_t split = blockIdx.y, nsplit = gridDim.y;
+
+  // the wave's queries, in registers for the whole launch
+  uint4 qreg[J];
+  {
+    const uint8_t* qrow = xsq + (size_t)qg * ROWB;
+#pragma unroll
+    for (int j = 0; j < J; ++j) qreg[j] = *reinterpret_cast<const uint4*>(qrow + (2 * j + h) * 16);
+  }
+  const float t0 = (tinit && qvalid) ? tinit[qg] : -INFINITY;
+  // padding queries get an infinite threshold so they never take the insert path
+  const float th0 = qvalid ? t0 : INFINITY;
+  if (h == 0) {
+    l_cnt[qloc] = 0;
+    l_minpos[qloc] = 0;
+    l_th[qloc] = th0;
+  }
+  float th = th0;
+  float* my_sc = l_sc + qloc * KP;
+  uint32_t* my_id = l_id + qloc * KP;
+
+  const uint32_t c_begin = split * cols_per_split;
+  uint32_t c_end = c_begin + cols_per_split;
+  if (c_end > n) c_end = n;
+  const int ntile = c_begin < c_end ? (int)((c_end - c_begin + TC - 1) / TC) : 0;
+
+  uint4 pre[CPT];
+  auto load = [&](int t) {
+    const uint8_t* src = xsq + (size_t)(c_begin + (uint32_t)t * TC) * ROWB;
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      int e = tid + c * NT;
+      if (CHUNKS % NT == 0 || e < CHUNKS) pre[c] = *reinterpret_cast<const uint4*>(src + (size_t)e * 16);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      int e = tid + c * NT;
+      if (CHUNKS % NT == 0 || e < CHUNKS) {
+        int r = e / (2 * J), q = e % (2 * J);
+        int j = q >> 1, hh = q & 1, sub = r >> 5, lr = r & 31;
+        *reinterpret_cast<uint4*>(&tile[buf][(((sub * J + j) * 2 + hh) * 32 + lr) * 16]) = pre[c];
+      }
+    }
+  };
+
+  if (ntile > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+
+  for (int t = 0; t < ntile; ++t) {
+    const int buf = t & 1;
+    const bool more = t + 1 < ntile;
+    if (more) load(t + 1);
+    const uint8_t* T = tile[buf];
+#pragma unroll
+    for (int sub = 0; sub < SUBT; ++sub) {
+      f32x16 acc = {};
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        uint4 bv = *reinterpret_cast<const uint4*>(T + (((sub * J + j) * 2 + h) * 32 + l32) * 16);
+        if constexpr (PREC == PREC_F32) {
+          f32x4 b4 = __builtin_bit_cast(f32x4, bv);
+          f32x4 q4 = __builtin_bit_cast(f32x4, qreg[j]);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[0], q4[0], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[1], q4[1], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[2], q4[2], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[3], q4[3], acc, 0, 0, 0);
+        } else {
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, bv),
+                                                       __builtin_bit_cast(f16x8, qreg[j]), acc, 0,
+                                                       0, 0);
+        }
+      }
+      float m01 = fmaxf(fmaxf(acc[0], acc[1]), acc[2]);
+      float m02 = fmaxf(fmaxf(acc[3], acc[4]), acc[5]);
+      float m03 = fmaxf(fmaxf(acc[6], acc[7]), acc[8]);
+      float m04 = fmaxf(fmaxf(acc[9], acc[10]), acc[11]);
+      float m05 = fmaxf(fmaxf(acc[12], acc[13]), acc[14]);
+      float m = fmaxf(fmaxf(fmaxf(m01, m02), m03), fmaxf(fmaxf(m04, m05), acc[15]));
+      float the = (PREC == PREC_F16) ? th - eps_pre : th;
+      if (__builtin_amdgcn_ballot_w64(m > the) != 0) {
+        const uint32_t cbase = c_begin + (uint32_t)t * TC + sub * 32;
+        for (int hh = 0; hh < 2; ++hh) {
+          if (h == hh && qvalid) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+              float v = acc[i];
+              if (v > the) {
+                const uint32_t cand = cbase + 8 * (i >> 2) + 4 * h + (i & 3);
+                if (cand < c_end && cand != qg) {
+                  if constexpr (PREC == PREC_F16) {
+                    v = dot32(xs32 + (size_t)qg * dp32, xs32 + (size_t)cand * dp32, dp32);
+                    if (!(v > th)) continue;
+                  }
+                  th = list_insert(my_sc, my_id, &l_cnt[qloc], &l_minpos[qloc], &l_th[qloc], v,
+                                   cand, kp, t0);
+                  the = (PREC == PREC_F16) ? th - eps_pre : th;
+                }
+              }
+            }
+          }
+          wave_sync();
+        }
+        th = l_th[qloc];
+      }
+    }
+    if (more) store(buf ^ 1);
+    __syncthreads();
+  }
+
+  if (h == 0 && qvalid) {
+    const int c = l_cnt[qloc];
+    const size_t slot = (size_t)qq * nsplit + split;
+    const size_t base = slot * (size_t)kp;
+    for (int e = 0; e < c; ++e) {
+      cand_idx[base + e] = my_id[e];
+      cand_sc[base + e] = my_sc[e];
+    }
+    cand_cnt[slot] = (uint32_t)c;
+    cand_floor[slot] = (c == kp) ? l_th[qloc] : t0;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Sorting helpers: bitonic sort of P (power of two) entries in LDS into "ranks_before" order,
+// by NT threads of one workgroup.
+
+template <int NT>
+__device__ void bitonic_lds(double* __restrict__ sc, double* __restrict__ cs, uint32_t* __restrict__ id,
+                            int P) {
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int e = threadIdx.x; e < P / 2; e += NT) {
+        int a = 2 * e - (e & (stride - 1));
+        int b = a + stride;
+        bool up = (a & size) == 0;
+        bool ab = ranks_before(sc[a], id[a], sc[b], id[b]);
+        if (up != ab) {
+          double ts = sc[a];
+          sc[a] = sc[b];
+          sc[b] = ts;
+          if (cs) {
+            double tc = cs[a];
+            cs[a] = cs[b];
+            cs[b] = tc;
+          }
+          uint32_t ti = id[a];
+          id[a] = id[b];
+          id[b] = ti;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// rescore_kernel: one 64-thread workgroup per query.  Gathers the query's nsplit lists, rescores
+// each candidate in f64 over the original rows, sorts, applies the certificate and writes the
+// top k (or queues the row for the full-row path).
+
+template <int CMAX>
+__global__ __launch_bounds__(64) void rescore_kernel(
+    const float* __restrict__ x, uint32_t d, const double* __restrict__ norms, uint32_t n,
+    uint32_t k, int kp, uint32_t nsplit, const uint32_t* __restrict__ rowlist, uint32_t nq,
+    const uint32_t* __restrict__ cand_idx, const uint32_t* __restrict__ cand_cnt,
+    const float* __restrict__ cand_floor, int all_admitted, double eps32,
+    uint32_t* __restrict__ idx, double* __restrict__ cos_out, double* __restrict__ wts,
+    uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo, uint32_t* __restrict__ counters) {
+  __shared__ double s_sc[CMAX];
+  __shared__ uint32_t s_id[CMAX];
+  const uint32_t qq = blockIdx.x;
+  if (qq >= nq) return;
+  const uint32_t i = rowlist ? rowlist[qq] : qq;
+  const double ni = norms[i];
+  const float* xi = x + (size_t)i * d;
+  const int tot_slots = (int)nsplit * kp;
+  int P = 1;
+  while (P < tot_slots) P <<= 1;
+  if (P < 2) P = 2;
+
+  uint32_t total = 0;
+  double floor_ = -INFINITY;
+  for (uint32_t s = 0; s < nsplit; ++s) {
+    total += cand_cnt[(size_t)qq * nsplit + s];
+    floor_ = fmax(floor_, (double)cand_floor[(size_t)qq * nsplit + s]);
+  }
+  for (int e = threadIdx.x; e < P; e += 64) {
+    double sc = -INFINITY;
+    uint32_t j = NONE;
+    if (e < tot_slots) {
+      const uint32_t s = (uint32_t)e / (uint32_t)kp, r = (uint32_t)e % (uint32_t)kp;
+      const size_t slot = (size_t)qq * nsplit + s;
+      if (r < cand_cnt[slot]) {
+        j = cand_idx[slot * kp + r];
+        sc = cos64(dot64(xi, x + (size_t)j * d, d), ni, norms[j]);
+      }
+    }
+    s_sc[e] = sc;
+    s_id[e] = j;
+  }
+  __syncthreads();
+  bitonic_lds<64>(s_sc, nullptr, s_id, P);
+
+  bool ok;
+  if (all_admitted) {
+    ok = true;
+  } else if (total < k) {
+    ok = false;
+  } else {
+    ok = s_sc[k - 1] > floor_ + eps32;
+  }
+  if (ok) {
+    for (uint32_t t = threadIdx.x; t < k; t += 64) {
+      const uint32_t j = s_id[t];
+      const double c = s_sc[t];
+      idx[(size_t)i * k + t] = j;
+      cos_out[(size_t)i * k + t] = c;
+      if (wts) wts[(size_t)i * k + t] = tani_from_cos(ni, norms[j], c);
+    }
+  } else if (threadIdx.x == 0) {
+    const uint32_t p = atomicAdd(&counters[1], 1u);
+    fb_rows[p] = i;
+    fb_lo[p] = total >= k ? s_sc[k - 1] - eps32 : -INFINITY;
+    if (total < k) atomicAdd(&counters[2], 1u);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// rowscan_kernel: exact top-k of whole rows.  One 256-thread workgroup per row (grid-stride over
+// the row list).  Candidates stream in chunks of 256; survivors are appended to an LDS buffer that
+// is merged into the running top-k (bitonic sort of list + buffer) when it nears capacity.
+//   COARSE = true:  the survivor test is an f32 inner product of the normalized rows >= lo, with lo
+//                   raised to (k-th exact score - eps) once k survivors are known (gather_row,
+//                   knn.rs:723-743, with min_sim = s_k - eps).
+//   COARSE = false: every candidate is scored exactly (knn_bruteforce_select, knn.rs:884-924),
+//                   for either selection metric.
+
+template <int CAP, bool COARSE>
+__global__ __launch_bounds__(256) void rowscan_kernel(
+    const float* __restrict__ x, uint32_t d, const float* __restrict__ xs32, uint32_t dp32,
+    const double* __restrict__ norms, uint32_t n, uint32_t k, uint32_t select,
+    const uint32_t* __restrict__ rows, const uint32_t* __restrict__ nrows_dev, uint32_t nrows_host,
+    const double* __restrict__ lo_arr, double eps32, uint32_t* __restrict__ idx,
+    double* __restrict__ cos_out, double* __restrict__ wts) {
+  __shared__ double s_sc[CAP];
+  __shared__ double s_cs[CAP];
+  __shared__ uint32_t s_id[CAP];
+  __shared__ int s_L, s_B, s_full;
+  __shared__ double s_lo, s_ksc;
+  __shared__ uint32_t s_kid;
+  const uint32_t nrows = nrows_dev ? *nrows_dev : nrows_host;
+  const int tid = threadIdx.x;
+
+  for (uint32_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+    const uint32_t i = rows ? rows[r] : r;
+    if (tid == 0) {
+      s_L = 0;
+      s_B = 0;
+      s_full = 0;
+      s_lo = lo_arr ? lo_arr[r] : -INFINITY;
+      s_ksc = -INFINITY;
+      s_kid = NONE;
+    }
+    __syncthreads();
+    const double ni = norms[i];
+    const float* xi = x + (size_t)i * d;
+
+    auto merge = [&]() {
+      const int tot = s_L + s_B;
+      int P = 2;
+      while (P < tot) P <<= 1;
+      for (int e = tot + tid; e < P; e += 256) {
+        s_sc[e] = -INFINITY;
+        s_cs[e] = 0.0;
+        s_id[e] = NONE;
+      }
+      __syncthreads();
+      bitonic_lds<256>(s_sc, s_cs, s_id, P);
+      if (tid == 0) {
+        s_L = tot < (int)k ? tot : (int)k;
+        s_B = 0;
+        if (s_L == (int)k) {
+          s_full = 1;
+          s_ksc = s_sc[k - 1];
+          s_kid = s_id[k - 1];
+          if (COARSE) s_lo = fmax(s_lo, s_ksc - eps32);
+        }
+      }
+      __syncthreads();
+    };
+
+    for (uint32_t base = 0; base < n; base += 256) {
+      const uint32_t j = base + tid;
+      if (j < n && j != i) {
+        bool pass = true;
+        if (COARSE) {
+          float c32 = dot32(xs32 + (size_t)i * dp32, xs32 + (size_t)j * dp32, dp32);
+          pass = (double)c32 >= s_lo;
+        }
+        if (pass) {
+          const double nj = norms[j];
+          const double c = cos64(dot64(xi, x + (size_t)j * d, d), ni, nj);
+          const double sc = select == COSTAL_SELECT_TANIMOTO ? tani_from_cos(ni, nj, c) : c;
+          if (!s_full || ranks_before(sc, j, s_ksc, s_kid)) {
+            const int p = atomicAdd(&s_B, 1);
+            const int slot = s_L + p;
+            s_sc[slot] = sc;
+            s_cs[slot] = c;
+            s_id[slot] = j;
+          }
+        }
+      }
+      __syncthreads();
+      const bool full_buffer = s_L + s_B > CAP - 256;
+      __syncthreads();
+      if (full_buffer) merge();
+    }
+    merge();
+    for (uint32_t t = tid; t < k; t += 256) {
+      const uint32_t j = s_id[t];
+      idx[(size_t)i * k + t] = j;
+      cos_out[(size_t)i * k + t] = s_cs[t];
+      if (wts) wts[(size_t)i * k + t] = j < n ? tani_from_cos(ni, norms[j], s_cs[t]) : 0.0;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// weight::reweight (weight.rs:41-60): tanimoto / cosine.  Jaccard runs on the host.
+
+__global__ void reweight_kernel(uint64_t n, uint32_t k, const uint32_t* __restrict__ idx,
+                                const double* __restrict__ cos, const double* __restrict__ norms,
+                                uint32_t weight, double* __restrict__ out,
+                                uint32_t* __restrict__ bad) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n * k) return;
+  const uint32_t j = idx[e];
+  if (j >= n) {
+    atomicMin(bad, (uint32_t)(e < 0xffffffffull ? e : 0xfffffffeull));
+    return;
+  }
+  out[e] = weight == COSTAL_WEIGHT_TANIMOTO ? tani_from_cos(norms[e / k], norms[j], cos[e]) : cos[e];
+}
+
+// ------------------------------------------------------------------------------------------------
+// Host side
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(COSTAL_E_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));      \
+  } while (0)
+
+struct Device {
+  std::mutex mu;
+  bool init = false;
+  int id = 0;
+  int cus = 0;
+  hipStream_t stream = nullptr;
+  void* ws = nullptr;
+  size_t ws_cap = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+Device g_dev;
+
+int device_init(Device& D) {
+  if (D.init) return 0;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0)
+    return fail(COSTAL_E_DEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+  HIP_TRY(hipGetDevice(&D.id));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, D.id));
+  D.cus = prop.multiProcessorCount;
+  HIP_TRY(hipStreamCreateWithFlags(&D.stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&D.ev0));
+  HIP_TRY(hipEventCreate(&D.ev1));
+  D.init = true;
+  return 0;
+}
+
+int ws_reserve(Device& D, size_t bytes) {
+  if (bytes <= D.ws_cap) return 0;
+  if (D.ws) {
+    HIP_TRY(hipStreamSynchronize(D.stream));
+    HIP_TRY(hipFree(D.ws));
+    D.ws = nullptr;
+    D.ws_cap = 0;
+  }
+  size_t cap = bytes + bytes / 8 + (1u << 20);
+  HIP_TRY(hipMalloc(&D.ws, cap));
+  D.ws_cap = cap;
+  return 0;
+}
+
+// bump allocator over the workspace
+struct Carve {
+  uint8_t* base;
+  size_t off = 0;
+  explicit Carve(void* b) : base((uint8_t*)b) {}
+  template <class T>
+  T* take(size_t count) {
+    off = (off + 255) & ~(size_t)255;
+    T* p = reinterpret_cast<T*>(base + off);
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
+inline uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
+
+// The fused path handles d <= 64 (FP32) / 128 (FP16) and K' <= 64; everything else runs the
+// exact row scan.
+constexpr uint32_t FUSED_KP_MAX = 64;
+constexpr uint32_t ROWSCAN_K_MAX = 2048;
+
+struct Plan {
+  bool fused;
+  int prec;
+  uint32_t J;       // 16-byte chunks per lane half per row
+  uint32_t kp;      // K'
+  int KP;           // list capacity (template)
+  int NW;           // waves per workgroup
+  uint32_t QB, TC;  // queries per workgroup, candidates per tile
+  uint32_t nsplit, cols_per_split;
+  uint32_t dp32, dp16, n_pad;
+};
+
+typedef void (*CoarseFn)(const uint8_t*, const float*, uint32_t, uint32_t, int, const uint32_t*,
+                         uint32_t, const float*, uint32_t, float, uint32_t*, float*, uint32_t*,
+                         float*);
+
+template <int PREC, int J, int KP, int NW>
+CoarseFn coarse_ptr() {
+  return coarse_kernel<PREC, J, KP, NW>;
+}
+
+template <int PREC, int KP, int NW>
+CoarseFn coarse_by_j(uint32_t J) {
+  switch (J) {
+    case 1: return coarse_ptr<PREC, 1, KP, NW>();
+    case 2: return coarse_ptr<PREC, 2, KP, NW>();
+    case 3: return coarse_ptr<PREC, 3, KP, NW>();
+    case 4: return coarse_ptr<PREC, 4, KP, NW>();
+    case 5: return coarse_ptr<PREC, 5, KP, NW>();
+    case 6: return coarse_ptr<PREC, 6, KP, NW>();
+    case 7: return coarse_ptr<PREC, 7, KP, NW>();
+    case 8: return coarse_ptr<PREC, 8, KP, NW>();
+  }
+  return nullptr;
+}
+
+// List capacity and workgroup width go together: 32-entry lists with 4 waves, 64-entry lists
+// with 2 waves (the same LDS footprint).
+CoarseFn coarse_select(int prec, uint32_t J, int KP) {
+  if (prec == PREC_F32) return KP == 32 ? coarse_by_j<PREC_F32, 32, 4>(J) : coarse_by_j<PREC_F32, 64, 2>(J);
+  return KP == 32 ? coarse_by_j<PREC_F16, 32, 4>(J) : coarse_by_j<PREC_F16, 64, 2>(J);
+}
+
+// FP16 candidate error bound.  x16 = fp16(fp32(x/|x|)): |x16 - x| <= U|x| + eta per component with
+// U = 2^-11 (1 + 2^-24) + 2^-24 and eta = 2^-25 (half the FP16 subnormal spacing).  With |x| = 1,
+// sum |x16 y16 - x y| <= (U + eta sqrt d)(2 + U + eta sqrt d); the FP32 accumulation of the exact
+// FP16 products adds gamma_d (1 + U + eta sqrt d)^2.  eps_pre also covers the f32 re-score error
+// ((d+4) 2^-23) and the rounding of (threshold - eps_pre) in f32.
+double fp16_eps(uint32_t d) {
+  const double u32 = std::ldexp(1.0, -24);
+  const double U = std::ldexp(1.0, -11) * (1.0 + u32) + u32;
+  const double e = std::ldexp(1.0, -25) * std::sqrt((double)d);
+  const double gam = d * u32 / (1.0 - d * u32);
+  return (U + e) * (2.0 + U + e) + gam * (1.0 + U + e) * (1.0 + U + e);
+}
+
+double fp32_eps(uint32_t d) { return ((double)d + 4.0) * (double)std::ldexp(1.0, -23); }
+
+Plan make_plan(const Device& D, uint64_t n, uint64_t d, uint64_t k, const costal_knn_params& p) {
+  Plan P{};
+  const uint64_t margin = p.margin < 0 ? k : (uint64_t)p.margin;
+  P.kp = (uint32_t)std::min<uint64_t>(k + margin, n - 1);
+  P.prec = (int)p.precision;
+  P.dp32 = round_up((uint32_t)std::max<uint64_t>(d, 1), 8);
+  P.dp16 = round_up((uint32_t)std::max<uint64_t>(d, 1), 16);
+  const uint32_t dmax = P.prec == PREC_F16 ? 128 : 64;
+  P.fused = p.select == COSTAL_SELECT_COSINE && d <= dmax && P.kp <= FUSED_KP_MAX;
+  P.J = P.prec == PREC_F16 ? P.dp16 / 16 : P.dp32 / 8;
+  P.KP = P.kp <= 32 ? 32 : 64;
+  P.NW = P.KP == 32 ? 4 : 2;
+  P.QB = 32 * P.NW;
+  P.TC = P.J <= 4 ? 128 : 64;
+  P.n_pad = round_up((uint32_t)n, 128);
+  const uint32_t ntile = (uint32_t)((n + P.TC - 1) / P.TC);
+  const uint32_t nblk = (uint32_t)((n + P.QB - 1) / P.QB);
+  uint32_t ns = p.col_splits;
+  if (!ns) {
+    const uint32_t target = 4u * (uint32_t)std::max(D.cus, 1);
+    ns = (target + nblk - 1) / nblk;
+    ns = std::min<uint32_t>(ns, 8);
+  }
+  ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, std::max<uint32_t>(1, ntile / 2)));
+  const uint32_t tiles_per = (ntile + ns - 1) / ns;
+  P.cols_per_split = tiles_per * P.TC;
+  P.nsplit = (ntile + tiles_per - 1) / tiles_per;
+  if (P.nsplit < 1) P.nsplit = 1;
+  return P;
+}
+
+int check_args(uint64_t n, uint64_t d, uint64_t k) {
+  if (n > 0xffffffffull) return fail(COSTAL_E_TOO_MANY_ROWS, "n=%llu exceeds the u32 index range", (unsigned long long)n);
+  if (k == 0 || k >= n)
+    return fail(COSTAL_E_BAD_K, "invalid k=%llu: need 1 <= k <= n-1 (n=%llu)", (unsigned long long)k,
+                (unsigned long long)n);
+  if (d > 0xffffffffull / 8) return fail(COSTAL_E_UNSUPPORTED, "d=%llu is too large", (unsigned long long)d);
+  return 0;
+}
+
+// Launch the exact row scan over `rows` (device list, count on device or host).
+int launch_rowscan(Device& D, hipStream_t s, bool coarse, const float* x, uint32_t d, const float* xs32,
+                   uint32_t dp32, const double* norms, uint32_t n, uint32_t k, uint32_t select,
+                   const uint32_t* rows, const uint32_t* nrows_dev, uint32_t nrows_host, const double* lo,
+                   double eps32, uint32_t* idx, double* cos, double* wts, uint32_t grid) {
+  if (k > ROWSCAN_K_MAX)
+    return fail(COSTAL_E_UNSUPPORTED, "k=%u above %u is not supported by the row-scan path", k, ROWSCAN_K_MAX);
+  if (grid == 0) return 0;
+  const bool small = k <= 512;
+  if (coarse) {
+    if (small)
+      rowscan_kernel<1024, true><<<grid, 256, 0, s>>>(x, d, xs32, dp32, norms, n, k, select, rows, nrows_dev,
+                                                      nrows_host, lo, eps32, idx, cos, wts);
+    else
+      rowscan_kernel<4096, true><<<grid, 256, 0, s>>>(x, d, xs32, dp32, norms, n, k, select, rows, nrows_dev,
+                                                      nrows_host, lo, eps32, idx, cos, wts);
+  } else {
+    if (small)
+      rowscan_kernel<1024, false><<<grid, 256, 0, s>>>(x, d, xs32, dp32, norms, n, k, select, rows, nrows_dev,
+                                                       nrows_host, lo, eps32, idx, cos, wts);
+    else
+      rowscan_kernel<4096, false><<<grid, 256, 0, s>>>(x, d, xs32, dp32, norms, n, k, select, rows, nrows_dev,
+                                                       nrows_host, lo, eps32, idx, cos, wts);
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+// The whole path on device pointers.  Serialized by the caller (D.mu held).
+int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, const costal_knn_params& p,
+                      uint32_t* idx, double* cos, double* norms_out, double* wts, costal_knn_stats* st,
+                      hipStream_t s, bool brute) {
+  const uint32_t n = (uint32_t)n64, d = (uint32_t)d64, k = (uint32_t)p.k;
+  if (p.precision > COSTAL_PREC_FP16) return fail(COSTAL_E_INVALID, "unknown precision %u", p.precision);
+  if (p.select > COSTAL_SELECT_TANIMOTO) return fail(COSTAL_E_INVALID, "unknown select %u", p.select);
+  Plan P = make_plan(D, n, d, k, p);
+  if (brute) P.fused = false;
+  const bool need16 = P.fused && P.prec == PREC_F16;
+  const size_t nq_slots = P.fused ? (size_t)n * P.nsplit : 0;
+
+  size_t bytes = 0;
+  {
+    Carve c(nullptr);
+    c.take<float>((size_t)P.n_pad * P.dp32);
+    if (need16) c.take<_Float16>((size_t)P.n_pad * P.dp16);
+    c.take<double>(n);
+    c.take<uint32_t>(nq_slots * P.kp);
+    c.take<float>(nq_slots * P.kp);
+    c.take<uint32_t>(nq_slots);
+    c.take<float>(nq_slots);
+    c.take<uint32_t>(n);
+    c.take<double>(n);
+    c.take<uint32_t>(64);
+    bytes = c.off + 256;
+  }
+  int rc = ws_reserve(D, bytes);
+  if (rc) return rc;
+  Carve c(D.ws);
+  float* xs32 = c.take<float>((size_t)P.n_pad * P.dp32);
+  _Float16* xs16 = need16 ? c.take<_Float16>((size_t)P.n_pad * P.dp16) : nullptr;
+  double* norms_ws = c.take<double>(n);
+  uint32_t* cand_idx = c.take<uint32_t>(nq_slots * P.kp);
+  float* cand_sc = c.take<float>(nq_slots * P.kp);
+  uint32_t* cand_cnt = c.take<uint32_t>(nq_slots);
+  float* cand_floor = c.take<float>(nq_slots);
+  uint32_t* fb_rows = c.take<uint32_t>(n);
+  double* fb_lo = c.take<double>(n);
+  uint32_t* counters = c.take<uint32_t>(64);  // [0] first bad row, [1] fallback rows, [2] short rows
+  double* norms = norms_out ? norms_out : norms_ws;
+
+  HIP_TRY(hipEventRecord(D.ev0, s));
+  {
+    uint32_t init[4] = {NONE, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(counters, init, sizeof init, hipMemcpyHostToDevice, s));
+  }
+  prep_kernel<<<(P.n_pad + 255) / 256, 256, 0, s>>>(x, n, d, P.n_pad, P.dp32, P.dp16, norms, xs32, xs16,
+                                                    counters);
+  HIP_TRY(hipGetLastError());
+  uint32_t bad = NONE;
+  HIP_TRY(hipMemcpyAsync(&bad, counters, sizeof bad, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (bad != NONE) return fail(COSTAL_E_NON_FINITE, "row %u contains NaN or inf", bad);
+
+  const double eps32 = fp32_eps(d);
+  if (P.fused) {
+    CoarseFn fn = coarse_select(P.prec, P.J, P.KP);
+    if (!fn) return fail(COSTAL_E_UNSUPPORTED, "no coarse kernel for d=%u", d);
+    const float eps_pre = P.prec == PREC_F16 ? (float)(fp16_eps(d) + eps32 + std::ldexp(1.0, -22)) : 0.0f;
+    const uint8_t* xsq = need16 ? (const uint8_t*)xs16 : (const uint8_t*)xs32;
+    dim3 grid((n + P.QB - 1) / P.QB, P.nsplit);
+    fn<<<grid, 64 * P.NW, 0, s>>>(xsq, xs32, P.dp32, n, (int)P.kp, nullptr, n, p.row_threshold,
+                                  P.cols_per_split, eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor);
+    HIP_TRY(hipGetLastError());
+    const int all_admitted = (P.nsplit == 1 && P.kp >= n - 1 && !p.row_threshold) ? 1 : 0;
+    const uint32_t slots = P.nsplit * P.kp;
+#define RESCORE(CM)                                                                                    \
+  rescore_kernel<CM><<<n, 64, 0, s>>>(x, d, norms, n, k, (int)P.kp, P.nsplit, nullptr, n, cand_idx,      \
+                                      cand_cnt, cand_floor, all_admitted, eps32, idx, cos, wts, fb_rows, \
+                                      fb_lo, counters)
+    if (slots <= 64) RESCORE(64);
+    else if (slots <= 128) RESCORE(128);
+    else if (slots <= 256) RESCORE(256);
+    else RESCORE(512);
+#undef RESCORE
+    HIP_TRY(hipGetLastError());
+    const uint32_t grid_fb = std::min<uint32_t>(n, 4u * (uint32_t)std::max(D.cus, 1));
+    rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb_rows,
+                        counters + 1, 0, fb_lo, eps32, idx, cos, wts, grid_fb);
+    if (rc) return rc;
+  } else {
+    const uint32_t grid = std::min<uint32_t>(n, 8u * (uint32_t)std::max(D.cus, 1));
+    rc = launch_rowscan(D, s, false, x, d, xs32, P.dp32, norms, n, k, p.select, nullptr, nullptr, n, nullptr,
+                        eps32, idx, cos, wts, grid);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipEventRecord(D.ev1, s));
+  uint32_t cnt[4] = {0, 0, 0, 0};
+  HIP_TRY(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (st) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, D.ev0, D.ev1) != hipSuccess) ms = 0.0f;
+    st->fallback_rows = P.fused ? cnt[1] : 0;
+    st->short_rows = cnt[2];
+    st->row_block = P.fused ? P.QB : 1;
+    st->col_block = P.fused ? P.TC : 256;
+    st->blocks_skipped = 0;
+    st->blocks_total = P.fused ? (uint64_t)((n + P.QB - 1) / P.QB) * ((n + P.TC - 1) / P.TC) : n;
+    st->candidates = P.kp;
+    st->col_splits = P.fused ? P.nsplit : 1;
+    st->path = P.fused ? 0 : 1;
+    st->device_ms = ms;
+  }
+  return 0;
+}
+
+hipStream_t pick_stream(Device& D, void* stream) { return stream ? (hipStream_t)stream : D.stream; }
+
+}  // namespace
+
+// ==================================================================================================
+// C ABI
+
+extern "C" {
+
+const char* costal_last_error(void) { return g_err.c_str(); }
+
+const char* costal_version(void) { return COSTAL_VERSION; }
+
+int costal_device_info(char* buf, size_t len) {
+  std::lock_guard<std::mutex> lk(g_dev.mu);
+  int rc = device_init(g_dev);
+  if (rc) return rc;
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, g_dev.id));
+  snprintf(buf, len, "%s (%s), %d CUs, %.1f GiB, clock %d kHz", prop.name, prop.gcnArchName,
+           prop.multiProcessorCount, prop.totalGlobalMem / 1073741824.0, prop.clockRate);
+  return 0;
+}
+
+int costal_knn_exact_device(const float* x, uint64_t n, uint64_t d, const costal_knn_params* params,
+                            uint32_t* idx, double* cos, double* norms, double* weights,
+                            costal_knn_stats* stats, void* stream) {
+  if (!params || !x || !idx || !cos) return fail(COSTAL_E_INVALID, "null pointer argument");
+  int rc = check_args(n, d, params->k);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_dev.mu);
+  rc = device_init(g_dev);
+  if (rc) return rc;
+  return knn_device_locked(g_dev, x, n, d, *params, idx, cos, norms, weights, stats,
+                           pick_stream(g_dev, stream), false);
+}
+
+static int knn_host(const float* x, uint64_t n, uint64_t d, const costal_knn_params* params, uint32_t* idx,
+                    double* cos, double* norms, double* weights, costal_knn_stats* stats, bool brute) {
+  if (!params || !x || !idx || !cos) return fail(COSTAL_E_INVALID, "null pointer argument");
+  int rc = check_args(n, d, params->k);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_dev.mu);
+  rc = device_init(g_dev);
+  if (rc) return rc;
+  const uint64_t k = params->k;
+  hipStream_t s = g_dev.stream;
+  float* dx = nullptr;
+  uint32_t* didx = nullptr;
+  double *dcos = nullptr, *dnorms = nullptr, *dw = nullptr;
+  float* dth = nullptr;
+  auto cleanup = [&]() {
+    (void)hipFree(dx);
+    (void)hipFree(didx);
+    (void)hipFree(dcos);
+    (void)hipFree(dnorms);
+    (void)hipFree(dw);
+    (void)hipFree(dth);
+  };
+#define HT(expr)                                                                      \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      cleanup();                                                                      \
+      return fail(COSTAL_E_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));    \
+    }                                                                                 \
+  } while (0)
+  HT(hipMalloc(&dx, std::max<size_t>(n * d, 1) * sizeof(float)));
+  HT(hipMalloc(&didx, n * k * sizeof(uint32_t)));
+  HT(hipMalloc(&dcos, n * k * sizeof(double)));
+  HT(hipMalloc(&dnorms, n * sizeof(double)));
+  if (weights) HT(hipMalloc(&dw, n * k * sizeof(double)));
+  costal_knn_params p = *params;
+  if (params->row_threshold) {
+    HT(hipMalloc(&dth, n * sizeof(float)));
+    HT(hipMemcpyAsync(dth, params->row_threshold, n * sizeof(float), hipMemcpyHostToDevice, s));
+    p.row_threshold = dth;
+  }
+  HT(hipMemcpyAsync(dx, x, n * d * sizeof(float), hipMemcpyHostToDevice, s));
+  rc = knn_device_locked(g_dev, dx, n, d, p, didx, dcos, dnorms, dw, stats, s, brute);
+  if (rc) {
+    (void)hipStreamSynchronize(s);
+    cleanup();
+    return rc;
+  }
+  HT(hipMemcpyAsync(idx, didx, n * k * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HT(hipMemcpyAsync(cos, dcos, n * k * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (norms) HT(hipMemcpyAsync(norms, dnorms, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (weights) HT(hipMemcpyAsync(weights, dw, n * k * sizeof(double), hipMemcpyDeviceToHost, s));
+  HT(hipStreamSynchronize(s));
+#undef HT
+  cleanup();
+  return 0;
+}
+
+int costal_knn_exact(const float* x, uint64_t n, uint64_t d, const costal_knn_params* params,
+                     uint32_t* idx, double* cos, double* norms, double* weights,
+                     costal_knn_stats* stats) {
+  return knn_host(x, n, d, params, idx, cos, norms, weights, stats, false);
+}
+
+int costal_knn_bruteforce(const float* x, uint64_t n, uint64_t d, uint64_t k, uint32_t select,
+                          uint32_t* idx, double* cos, double* norms) {
+  costal_knn_params p;
+  memset(&p, 0, sizeof p);
+  p.k = k;
+  p.margin = -1;
+  p.select = select;
+  return knn_host(x, n, d, &p, idx, cos, norms, nullptr, nullptr, true);
+}
+
+int costal_reweight_device(uint64_t n, uint64_t k, const uint32_t* idx, const double* cos,
+                           const double* norms, uint32_t weight, double* out, void* stream) {
+  if (weight != COSTAL_WEIGHT_TANIMOTO && weight != COSTAL_WEIGHT_COSINE)
+    return fail(COSTAL_E_INVALID, "weight %u is not computed on the device", weight);
+  std::lock_guard<std::mutex> lk(g_dev.mu);
+  int rc = device_init(g_dev);
+  if (rc) return rc;
+  if (n * k == 0) return 0;
+  hipStream_t s = pick_stream(g_dev, stream);
+  rc = ws_reserve(g_dev, 256);
+  if (rc) return rc;
+  uint32_t* bad = (uint32_t*)g_dev.ws;
+  uint32_t init = NONE, got = NONE;
+  HIP_TRY(hipMemcpyAsync(bad, &init, sizeof init, hipMemcpyHostToDevice, s));
+  const uint64_t total = n * k;
+  reweight_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(n, (uint32_t)k, idx, cos, norms, weight, out,
+                                                                   bad);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(&got, bad, sizeof got, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (got != NONE) return fail(COSTAL_E_INVALID, "neighbor index out of range (n=%llu)", (unsigned long long)n);
+  return 0;
+}
+
+int costal_reweight(uint64_t n, uint64_t k, const uint32_t* idx, const double* cos, const double* norms,
+                    uint32_t weight, double* out) {
+  if (!idx || !cos || !norms || !out) return fail(COSTAL_E_INVALID, "null pointer argument");
+  for (uint64_t e = 0; e < n * k; ++e)
+    if (idx[e] >= n)
+      return fail(COSTAL_E_INVALID, "neighbor index %u out of range (n=%llu)", idx[e], (unsigned long long)n);
+  if (weight == COSTAL_WEIGHT_JACCARD) {
+    // weight.rs:64-93: O(N k^2) set intersection of sorted neighbor lists (integer work on the host)
+    std::vector<uint32_t> sorted(idx, idx + n * k);
+    for (uint64_t i = 0; i < n; ++i) std::sort(sorted.begin() + i * k, sorted.begin() + (i + 1) * k);
+    for (uint64_t i = 0; i < n; ++i) {
+      const uint32_t* a = sorted.data() + i * k;
+      for (uint64_t t = 0; t < k; ++t) {
+        const uint32_t* b = sorted.data() + (uint64_t)idx[i * k + t] * k;
+        uint64_t p = 0, q = 0, sh = 0;
+        while (p < k && q < k) {
+          if (a[p] < b[q]) ++p;
+          else if (a[p] > b[q]) ++q;
+          else { ++sh; ++p; ++q; }
+        }
+        out[i * k + t] = (double)sh / (2.0 * (double)k - (double)sh);
+      }
+    }
+    return 0;
+  }
+  if (weight != COSTAL_WEIGHT_TANIMOTO && weight != COSTAL_WEIGHT_COSINE)
+    return fail(COSTAL_E_INVALID, "unknown edge weight %u", weight);
+  if (n * k == 0) return 0;
+  int rc;
+  {
+    std::lock_guard<std::mutex> lk(g_dev.mu);
+    rc = device_init(g_dev);
+    if (rc) return rc;
+  }
+  uint32_t* didx = nullptr;
+  double *dcos = nullptr, *dn = nullptr, *dout = nullptr;
+  hipStream_t s = g_dev.stream;
+  rc = 0;
+  if (hipMalloc(&didx, n * k * 4) != hipSuccess || hipMalloc(&dcos, n * k * 8) != hipSuccess ||
+      hipMalloc(&dn, n * 8) != hipSuccess || hipMalloc(&dout, n * k * 8) != hipSuccess) {
+    rc = fail(COSTAL_E_DEVICE, "hipMalloc failed in costal_reweight");
+  } else if (hipMemcpyAsync(didx, idx, n * k * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+             hipMemcpyAsync(dcos, cos, n * k * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+             hipMemcpyAsync(dn, norms, n * 8, hipMemcpyHostToDevice, s) != hipSuccess) {
+    rc = fail(COSTAL_E_DEVICE, "copy to device failed in costal_reweight");
+  } else {
+    rc = costal_reweight_device(n, k, didx, dcos, dn, weight, dout, s);
+    if (!rc && (hipMemcpyAsync(out, dout, n * k * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess))
+      rc = fail(COSTAL_E_DEVICE, "copy to host failed in costal_reweight");
+  }
+  (void)hipFree(didx);
+  (void)hipFree(dcos);
+  (void)hipFree(dn);
+  (void)hipFree(dout);
+  return rc;
+}
+
+int costal_fuzzy_union(uint64_t n, uint64_t k, const uint32_t* idx, const double* w, uint32_t* rows,
+                       uint32_t* cols, double* wout, uint64_t* m) {
+  // graph.rs:24-68: key = lo << 32 | hi, sort, combine (a + b) - a*b, drop exact zeros.
+  if (!idx || !w || !rows || !cols || !wout || !m) return fail(COSTAL_E_INVALID, "null pointer argument");
+  for (uint64_t e = 0; e < n * k; ++e)
+    if (idx[e] >= n)
+      return fail(COSTAL_E_INVALID, "neighbor index %u out of range (n=%llu)", idx[e], (unsigned long long)n);
+  std::vector<std::pair<uint64_t, double>> e;
+  e.reserve(n * k);
+  for (uint64_t t = 0; t < n * k; ++t) {
+    const uint64_t i = t / k, j = idx[t];
+    if (i == j) continue;
+    const uint64_t lo = i < j ? i : j, hi = i < j ? j : i;
+    e.emplace_back((lo << 32) | hi, w[t]);
+  }
+  std::sort(e.begin(), e.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+  uint64_t out = 0;
+  for (size_t t = 0; t < e.size();) {
+    const uint64_t key = e[t].first;
+    const double a = e[t].second;
+    double v;
+    if (t + 1 < e.size() && e[t + 1].first == key) {
+      const double b = e[t + 1].second;
+      v = (a + b) - a * b;
+      t += 2;
+    } else {
+      v = a;
+      t += 1;
+    }
+    if (v != 0.0) {
+      rows[out] = (uint32_t)(key >> 32);
+      cols[out] = (uint32_t)key;
+      wout[out] = v;
+      ++out;
+    }
+  }
+  *m = out;
+  return 0;
+}
+
+}  // extern "C"

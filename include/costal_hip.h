@@ -1,0 +1,133 @@
+/*
+ * costal_hip.h -- C ABI of the MI355X (gfx950) exact cosine kNN + Tanimoto edge weights.
+ *
+ * This is the drop-in boundary for CosTaL's graph-construction hot path.  Each entry point names
+ * the reference interface it replaces (ike12th/CosTaL 2.1.0):
+ *
+ *   costal_knn_exact        crates/costal-core/src/knn.rs:360-379  knn_exact / knn_exact_with_stats
+ *                           crates/costal-py/src/lib.rs:113-167     costal._costal._knn_exact
+ *   costal_knn_bruteforce   crates/costal-core/src/knn.rs:879-924  knn_bruteforce[_select]
+ *                           crates/costal-py/src/lib.rs:171-191     costal._costal._knn_bruteforce
+ *   costal_reweight         crates/costal-core/src/weight.rs:41-60 weight::reweight
+ *                           crates/costal-py/src/lib.rs:194-214     costal._costal._reweight
+ *   costal_fuzzy_union      crates/costal-core/src/graph.rs:24-68  graph::fuzzy_union
+ *                           crates/costal-py/src/lib.rs:218-239     costal._costal._fuzzy_union
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  The *_device variants take device pointers and an optional
+ *     hipStream_t (passed as void*, NULL = the library's own stream) and return when the work is
+ *     complete; the plain variants take host pointers (row-major, C-contiguous) and copy.
+ *   - The caller allocates every output: idx n*k (u32, original indices), cos n*k (f64),
+ *     norms n (f64), weights n*k (f64).  Outputs are in original row order.
+ *   - Every entry point returns 0 on success or one of the COSTAL_E_* codes below; the message is
+ *     in costal_last_error() (thread-local), with the reference's Display text for KnnError
+ *     (knn.rs:111-125).  No entry point aborts the process.
+ *   - Results are identical for every tile shape, precision and thread layout: neighbor indices
+ *     and cosines equal the f64 brute force bit for bit (knn.rs:884-924).
+ *   - Thread-safe: calls on one device are serialized internally.
+ */
+#ifndef COSTAL_HIP_H
+#define COSTAL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* KnnError (knn.rs:102-109) plus the device-side failures */
+#define COSTAL_OK 0
+#define COSTAL_E_SHAPE 1          /* "data length {len} does not match shape {n}x{d}" */
+#define COSTAL_E_BAD_K 2          /* "invalid k={k}: need 1 <= k <= n-1 (n={n})" */
+#define COSTAL_E_TOO_MANY_ROWS 3  /* "n={n} exceeds the u32 index range" */
+#define COSTAL_E_NON_FINITE 4     /* "row {row} contains NaN or inf" */
+#define COSTAL_E_UNAVAILABLE 5    /* "matmul backend {name} is not available on this platform" */
+#define COSTAL_E_DEVICE 6         /* HIP runtime error (no GPU, out of memory, launch failure) */
+#define COSTAL_E_UNSUPPORTED 7    /* a size this build does not handle (e.g. k above 2048) */
+#define COSTAL_E_INVALID 8        /* bad argument (null pointer, index out of range, bad enum) */
+
+/* selection metric (knn.rs:62-86) */
+#define COSTAL_SELECT_COSINE 0
+#define COSTAL_SELECT_TANIMOTO 1
+
+/* coarse-tile precision: plain FP32 MFMA, or FP16 MFMA candidates refined in FP32 */
+#define COSTAL_PREC_FP32 0
+#define COSTAL_PREC_FP16 1
+
+/* edge weights (weight.rs:10-26) */
+#define COSTAL_WEIGHT_TANIMOTO 0
+#define COSTAL_WEIGHT_COSINE 1
+#define COSTAL_WEIGHT_JACCARD 2
+
+/* KnnParams (knn.rs:26-60).  Zero-initialise, then set k; every other zero means "default". */
+typedef struct costal_knn_params {
+  uint64_t k;            /* neighbors per row, excluding the row itself */
+  int64_t margin;        /* candidate list = min(k + margin, n - 1); negative = k (the default) */
+  uint32_t select;       /* COSTAL_SELECT_* */
+  uint32_t precision;    /* COSTAL_PREC_* */
+  uint64_t row_block;    /* query rows per workgroup hint; 0 = tuned (speed only) */
+  uint64_t col_block;    /* candidate columns per LDS tile hint; 0 = tuned (speed only) */
+  uint32_t col_splits;   /* split the candidate range in this many parts; 0 = tuned (speed only) */
+  uint32_t reserved;
+  /* Optional per-row admission thresholds (f32, n entries; host memory for costal_knn_exact,
+   * device memory for costal_knn_exact_device): candidates whose coarse similarity does not exceed
+   * the row's threshold are never inserted.  Rows left with fewer than k admitted candidates, or
+   * whose k-th rescored similarity is within the error bound of the threshold, are recomputed by
+   * the plain scan, so the result stays exact whatever the thresholds are.  NULL = none. */
+  const float* row_threshold;
+} costal_knn_params;
+
+/* KnnStats (knn.rs:344-357) plus GPU-side counters */
+typedef struct costal_knn_stats {
+  uint64_t fallback_rows;     /* rows recomputed by the full-row path */
+  uint64_t row_block;         /* query rows per workgroup used */
+  uint64_t col_block;         /* candidate columns per LDS tile used */
+  uint64_t blocks_skipped;    /* always 0 for cosine selection */
+  uint64_t blocks_total;      /* row blocks x column tiles */
+  uint64_t candidates;        /* K' = min(k + margin, n - 1) */
+  uint64_t col_splits;        /* candidate-range splits used */
+  uint64_t short_rows;        /* rows with fewer than k admitted candidates (thresholds only) */
+  uint64_t path;              /* 0 fused MFMA scan, 1 exact row scan (wide d, large K', tanimoto) */
+  double device_ms;           /* device time of the call (hipEvent), 0 when not measured */
+} costal_knn_stats;
+
+/* Exact kNN by params->select from host memory.  norms may be NULL; weights (Tanimoto, n*k) may be
+ * NULL.  Replaces knn_exact_with_stats (knn.rs:365-379). */
+int costal_knn_exact(const float* x, uint64_t n, uint64_t d, const costal_knn_params* params,
+                     uint32_t* idx, double* cos, double* norms, double* weights,
+                     costal_knn_stats* stats);
+
+/* The same from device memory; every pointer is a device pointer. */
+int costal_knn_exact_device(const float* x, uint64_t n, uint64_t d,
+                            const costal_knn_params* params, uint32_t* idx, double* cos,
+                            double* norms, double* weights, costal_knn_stats* stats, void* stream);
+
+/* Brute force by the definition: every j != i scored in f64, ranked by (score desc, index asc).
+ * Replaces knn_bruteforce_select (knn.rs:884-924). */
+int costal_knn_bruteforce(const float* x, uint64_t n, uint64_t d, uint64_t k, uint32_t select,
+                          uint32_t* idx, double* cos, double* norms);
+
+/* Edge weights on an n x k kNN (weight.rs:41-93); every idx must be < n (lib.rs:66-78). */
+int costal_reweight(uint64_t n, uint64_t k, const uint32_t* idx, const double* cos,
+                    const double* norms, uint32_t weight, double* out);
+int costal_reweight_device(uint64_t n, uint64_t k, const uint32_t* idx, const double* cos,
+                           const double* norms, uint32_t weight, double* out, void* stream);
+
+/* Fuzzy union S + S^T - S.S^T of an n x k weighted kNN (graph.rs:24-68): the upper-triangular
+ * edge list in ascending (row, col) order with self-loops and exact zeros dropped.  rows, cols and
+ * w must hold n*k entries; the edge count is written to *m. */
+int costal_fuzzy_union(uint64_t n, uint64_t k, const uint32_t* idx, const double* w,
+                       uint32_t* rows, uint32_t* cols, double* wout, uint64_t* m);
+
+/* Thread-local message of the last failing call on this thread. */
+const char* costal_last_error(void);
+/* Library version string. */
+const char* costal_version(void);
+/* Human-readable device description ("gfx950, 256 CUs, ...") into buf; 0 on success. */
+int costal_device_info(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

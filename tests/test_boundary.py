@@ -1,0 +1,159 @@
+"""The C-ABI library and its Python mirror, without a GPU: exports, argument checks, error text.
+
+Argument checks that the reference performs before any work (ike12th/CosTaL
+crates/costal-py/src/lib.rs:104-214, crates/costal-core/src/knn.rs:325-342, python/costal/_api.py)
+must behave the same here; they run on the host before the device is touched.
+"""
+
+import os
+import re
+
+import numpy as np
+import pytest
+
+from costal_amd import _api, _costal, _lib, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "costal_hip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(costal_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = header_functions()
+    assert len(names) >= 9
+    L = _lib.load()
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) == set(_lib.EXPORTS)
+
+
+def test_version_string():
+    assert "gfx950" in _lib.version()
+
+
+def _has_gpu():
+    try:
+        _lib.device_info()
+        return True
+    except _lib.CostalError:
+        return False
+
+
+def test_no_device_is_a_clean_error():
+    if _has_gpu():
+        pytest.skip("a GPU is present")
+    x = synth.clustered(50, 4, 7)
+    with pytest.raises(_lib.CostalError) as e:
+        _costal._knn_exact(x, 5)
+    assert e.value.code == _lib.E_DEVICE
+
+
+@pytest.mark.parametrize("k", [0, 50, 51])
+def test_bad_k_message_matches_reference(k):
+    x = synth.clustered(50, 4, 7)
+    with pytest.raises(ValueError, match=re.escape(f"invalid k={k}: need 1 <= k <= n-1 (n=50)")):
+        _costal._knn_exact(x, k)
+
+
+def test_input_type_checks():
+    x = synth.clustered(50, 4, 7)
+    with pytest.raises(TypeError):
+        _costal._knn_exact(x.astype(np.float64), 5)
+    with pytest.raises(ValueError, match="C-contiguous"):
+        _costal._knn_exact(np.asfortranarray(x), 5)
+    with pytest.raises(ValueError, match='unknown select "jaccard"'):
+        _costal._knn_exact(x, 5, select="jaccard")
+    with pytest.raises(ValueError, match='unknown matmul backend "blas"'):
+        _costal._knn_exact(x, 5, matmul="blas")
+    with pytest.raises(ValueError, match="accelerate is not available"):
+        _costal._knn_exact(x, 5, matmul="accelerate")
+
+
+def test_reweight_argument_checks():
+    idx = np.array([[1], [5]], np.int32)
+    with pytest.raises(ValueError, match="neighbor index 5 out of range"):
+        _costal._reweight(idx, np.zeros((2, 1)), np.ones(2), "tanimoto")
+    with pytest.raises(ValueError, match='unknown edge weight "l2"'):
+        _costal._reweight(np.array([[1], [0]], np.int32), np.zeros((2, 1)), np.ones(2), "l2")
+    with pytest.raises(ValueError, match="shapes"):
+        _costal._reweight(np.array([[1], [0]], np.int32), np.zeros((2, 2)), np.ones(2), "cosine")
+
+
+def test_jaccard_and_fuzzy_union_run_on_host(oracle):
+    # weight.rs:111-121 and graph.rs:74-95 through the library's C ABI
+    idx = np.array([1, 2, 0, 2, 0, 1, 0, 1], np.int32).reshape(4, 2)
+    w = _costal._reweight(idx, np.zeros((4, 2)), np.ones(4), "jaccard").ravel()
+    assert w[0] == 1 / 3 and w[4] == 1 / 3 and w[6] == 1 / 3
+    idx = np.array([1, 2, 0, 2, 1, 0], np.int32).reshape(3, 2)
+    ww = np.array([0.5, 0.3, 0.4, 0.2, 0.0, 0.6]).reshape(3, 2)
+    rows, cols, wo = _costal._fuzzy_union(idx, ww)
+    assert rows.tolist() == [0, 0, 1] and cols.tolist() == [1, 2, 2]
+    assert wo.tolist() == [(0.5 + 0.4) - 0.5 * 0.4, (0.3 + 0.6) - 0.3 * 0.6, 0.2]
+    # a larger random graph against the oracle's union
+    rng = np.random.default_rng(0)
+    n, k = 500, 7
+    idx = np.stack([rng.choice(np.delete(np.arange(n), i), k, replace=False) for i in range(n)]).astype(np.int32)
+    w = rng.uniform(-0.3, 1, size=(n, k))
+    w[rng.random((n, k)) < 0.05] = 0.0
+    got = _costal._fuzzy_union(idx, w)
+    want = oracle.fuzzy_union(idx, w)
+    for a, b in zip(got, want):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_api_choices_match_reference():
+    # python/costal/_api.py:66-100
+    x = synth.clustered(50, 4, 7)
+    with pytest.raises(ValueError):
+        _api.knn(x, 5, select="jaccard")
+    with pytest.raises(ValueError):
+        _api.knn(x, 5, compat="nope")
+    with pytest.raises(NotImplementedError):
+        _api.knn(x, 5, backend="hnsw")
+    sp = pytest.importorskip("scipy.sparse")
+    with pytest.raises(TypeError):
+        _api.knn(sp.csr_matrix(x), 5)
+    m = _api._as_matrix(np.asfortranarray(x.astype(np.float64)))
+    assert m.dtype == np.float32 and m.flags.c_contiguous
+    np.testing.assert_array_equal(m, x)
+    with pytest.raises(ValueError, match="k must be within"):
+        _api.truncate(_api.KnnGraph(np.zeros((3, 2), np.int32), np.zeros((3, 2)), np.ones(3)), 3)
+
+
+def test_diagnostics_flags():
+    # python/costal/_api.py:112-144 and tests/test_knn.py:243-261
+    x = synth.clustered(300, 8, 11)
+    x /= np.linalg.norm(x, axis=1, keepdims=True)
+    norms = np.linalg.norm(x.astype(np.float64), axis=1)
+    assert _api._diagnose(x, norms)["row_norm_cv"] < 1e-4
+    y = synth.clustered(300, 8, 12)
+    y -= y.mean(axis=0)
+    assert _api._diagnose(y, np.linalg.norm(y.astype(np.float64), axis=1))["columns_centered"]
+    z = synth.clustered(300, 8, 13)
+    dz = _api._diagnose(z, np.linalg.norm(z.astype(np.float64), axis=1))
+    assert not dz["columns_centered"] and dz["row_norm_cv"] > 1e-3
+
+
+def test_synthetic_workloads_are_deterministic_and_shaped():
+    a = synth.cytometry_86864x39(n=5000)
+    b = synth.cytometry_86864x39(n=5000)
+    np.testing.assert_array_equal(a, b)
+    assert a.dtype == np.float32 and a.shape == (5000, 39) and (a < 0).any()
+    c = synth.counts_3605x2000(n=400)
+    assert c.shape == (400, 2000) and abs((c == 0).mean() - 0.70) < 0.03
+    r = synth.rare_396460x15(n=20000)
+    assert len(np.unique(r, axis=0)) == 20000 - 100
+    s = synth.batched_64(n_samples=6)
+    assert all(2000 <= len(v) <= 60000 and 32 <= v.shape[1] <= 48 for v in s)
+
+
+def test_lcg_matches_rust_generator():
+    # knn.rs:931-940: first draws of Lcg(1)
+    g = synth.Lcg(1)
+    s = 1
+    for _ in range(5):
+        s = (s * 6364136223846793005 + 1442695040888963407) % 2**64
+        assert g.next_f32() == np.float32(s >> 40) / np.float32(1 << 24)

@@ -141,6 +141,8 @@ def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None
         "short_rows": int(st.short_rows),
         "path": "fused" if st.path == 0 else "rowscan",
         "device_ms": float(st.device_ms),
+        "kernel_ms": {"prep": st.prep_ms, "coarse": st.coarse_ms, "rescore": st.rescore_ms,
+                      "fallback": st.fallback_ms},
     }
     if w is not None:
         stats["weights"] = w

@@ -60,6 +60,10 @@ class KnnStats(C.Structure):
         ("short_rows", C.c_uint64),
         ("path", C.c_uint64),
         ("device_ms", C.c_double),
+        ("prep_ms", C.c_double),
+        ("coarse_ms", C.c_double),
+        ("rescore_ms", C.c_double),
+        ("fallback_ms", C.c_double),
     ]
 
 

@@ -606,7 +606,9 @@ struct Device {
   hipStream_t stream = nullptr;
   void* ws = nullptr;
   size_t ws_cap = 0;
+  // call start/end and per-kernel brackets: prep, coarse, rescore, fallback
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t evk[8] = {};
 };
 
 Device g_dev;
@@ -624,6 +626,7 @@ int device_init(Device& D) {
   HIP_TRY(hipStreamCreateWithFlags(&D.stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&D.ev0));
   HIP_TRY(hipEventCreate(&D.ev1));
+  for (auto& e : D.evk) HIP_TRY(hipEventCreate(&e));
   D.init = true;
   return 0;
 }
@@ -836,9 +839,11 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     uint32_t init[4] = {NONE, 0, 0, 0};
     HIP_TRY(hipMemcpyAsync(counters, init, sizeof init, hipMemcpyHostToDevice, s));
   }
+  HIP_TRY(hipEventRecord(D.evk[0], s));
   prep_kernel<<<(P.n_pad + 255) / 256, 256, 0, s>>>(x, n, d, P.n_pad, P.dp32, P.dp16, norms, xs32, xs16,
                                                     counters);
   HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(D.evk[1], s));
   uint32_t bad = NONE;
   HIP_TRY(hipMemcpyAsync(&bad, counters, sizeof bad, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -851,9 +856,11 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     const float eps_pre = P.prec == PREC_F16 ? (float)(fp16_eps(d) + eps32 + std::ldexp(1.0, -22)) : 0.0f;
     const uint8_t* xsq = need16 ? (const uint8_t*)xs16 : (const uint8_t*)xs32;
     dim3 grid((n + P.QB - 1) / P.QB, P.nsplit);
+    HIP_TRY(hipEventRecord(D.evk[2], s));
     fn<<<grid, 64 * P.NW, 0, s>>>(xsq, xs32, P.dp32, n, (int)P.kp, nullptr, n, p.row_threshold,
                                   P.cols_per_split, eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(D.evk[3], s));
     const int all_admitted = (P.nsplit == 1 && P.kp >= n - 1 && !p.row_threshold) ? 1 : 0;
     const uint32_t slots = P.nsplit * P.kp;
 #define RESCORE(CM)                                                                                    \
@@ -866,15 +873,21 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     else RESCORE(512);
 #undef RESCORE
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(D.evk[4], s));
     const uint32_t grid_fb = std::min<uint32_t>(n, 4u * (uint32_t)std::max(D.cus, 1));
     rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb_rows,
                         counters + 1, 0, fb_lo, eps32, idx, cos, wts, grid_fb);
     if (rc) return rc;
+    HIP_TRY(hipEventRecord(D.evk[5], s));
   } else {
     const uint32_t grid = std::min<uint32_t>(n, 8u * (uint32_t)std::max(D.cus, 1));
+    HIP_TRY(hipEventRecord(D.evk[2], s));
+    HIP_TRY(hipEventRecord(D.evk[3], s));
+    HIP_TRY(hipEventRecord(D.evk[4], s));
     rc = launch_rowscan(D, s, false, x, d, xs32, P.dp32, norms, n, k, p.select, nullptr, nullptr, n, nullptr,
                         eps32, idx, cos, wts, grid);
     if (rc) return rc;
+    HIP_TRY(hipEventRecord(D.evk[5], s));
   }
   HIP_TRY(hipEventRecord(D.ev1, s));
   uint32_t cnt[4] = {0, 0, 0, 0};
@@ -893,6 +906,14 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     st->col_splits = P.fused ? P.nsplit : 1;
     st->path = P.fused ? 0 : 1;
     st->device_ms = ms;
+    auto span = [&](int a, int b) {
+      float v = 0.0f;
+      return hipEventElapsedTime(&v, D.evk[a], D.evk[b]) == hipSuccess ? (double)v : 0.0;
+    };
+    st->prep_ms = span(0, 1);
+    st->coarse_ms = span(2, 3);
+    st->rescore_ms = span(3, 4);
+    st->fallback_ms = span(4, 5);
   }
   return 0;
 }

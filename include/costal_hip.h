@@ -90,6 +90,10 @@ typedef struct costal_knn_stats {
   uint64_t short_rows;        /* rows with fewer than k admitted candidates (thresholds only) */
   uint64_t path;              /* 0 fused MFMA scan, 1 exact row scan (wide d, large K', tanimoto) */
   double device_ms;           /* device time of the call (hipEvent), 0 when not measured */
+  double prep_ms;             /* norms + working copies kernel (hipEvent) */
+  double coarse_ms;           /* fused MFMA tile + scan kernel (hipEvent) */
+  double rescore_ms;          /* f64 rescoring + certificate kernel (hipEvent) */
+  double fallback_ms;         /* full-row kernel (hipEvent) */
 } costal_knn_stats;
 
 /* Exact kNN by params->select from host memory.  norms may be NULL; weights (Tanimoto, n*k) may be

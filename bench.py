@@ -30,8 +30,13 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
-PEAK_TFLOPS = {"fp32": 157.3, "fp16": 2500.0}  # MI355X dense MFMA peaks
+PEAK_TFLOPS = {"fp32": 157.3, "fp16": 2500.0, "fp16x3": 2500.0}  # MI355X dense MFMA peaks
 PEAK_HBM_GBS = 8000.0
+FORM_DESC = {
+    "fp32": "fp32 MFMA candidates + f64 rescoring",
+    "fp16": "fp16 MFMA candidates + f32 re-score + f64 rescoring",
+    "fp16x3": "split-fp16 MFMA candidates (hi.hi + hi.lo + lo.hi) + f64 rescoring",
+}
 
 
 def parse():
@@ -42,7 +47,7 @@ def parse():
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline duration")
-    ap.add_argument("--forms", default="fp32,fp16", help="forms to time (headline = last)")
+    ap.add_argument("--forms", default="fp32,fp16,fp16x3", help="forms to time (headline = last)")
     return ap.parse_args()
 
 
@@ -115,11 +120,15 @@ def cpu_baseline(x: np.ndarray, k: int, target_s: float):
 
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     n = len(x)
-    probe = min(n, 1000)
-    t0 = time.perf_counter()
-    oracle.knn_blocked(x, k, sort_by_norm=False, rows=(0, probe), n_threads=threads)
-    dt = time.perf_counter() - t0
-    rows = int(min(n, max(probe, probe * target_s / max(dt, 1e-3))))
+    # calibrate the per-row cost from two probes (the difference removes the fixed per-call cost)
+    times = []
+    probes = (min(n, 2000), min(n, 10000))
+    for r in probes:
+        t0 = time.perf_counter()
+        oracle.knn_blocked(x, k, sort_by_norm=False, rows=(0, r), n_threads=threads)
+        times.append(time.perf_counter() - t0)
+    per_row = max((times[1] - times[0]) / max(probes[1] - probes[0], 1), 1e-7)
+    rows = int(min(n, max(probes[1], target_s / per_row)))
     t0 = time.perf_counter()
     (idx, cos, norms), _ = oracle.knn_blocked(x, k, sort_by_norm=False, rows=(0, rows),
                                                n_threads=threads)
@@ -153,7 +162,8 @@ def main():
         ms, kern, st = time_form(run, form, args.steps, args.warmup)
         results[form] = {"ms_per_step": ms, "cells_per_s": n / (ms / 1000.0), "kernel_ms": kern,
                          "fallback_rows": int(st.fallback_rows), "short_rows": int(st.short_rows),
-                         "col_splits": int(st.col_splits), "candidates": int(st.candidates)}
+                         "col_splits": int(st.col_splits), "candidates": int(st.candidates),
+                         "inserts": int(st.inserts)}
         outs[form] = run.outputs()
     head = forms[-1]
     ref = outs[forms[0]]
@@ -194,11 +204,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp16" if head == "fp16" else "fp32",
+        "dtype": {"fp32": "fp32", "fp16": "fp16", "fp16x3": "fp16"}[head],
         "data": "synthetic",
         "config": {"workload": "rare_396460x15", "cells": n, "features": d, "k": k,
-                   "form": {"fp16": "fp16 MFMA candidates + f32 re-score + f64 rescoring",
-                            "fp32": "fp32 MFMA candidates + f64 rescoring"}[head],
+                   "form": FORM_DESC[head],
                    "parallelism": "single GPU"},
         "forms": results,
         "match": match,

@@ -13,8 +13,9 @@ Every error is a ``ValueError`` carrying the library's message (lib.rs:31-33); w
 ``TypeError`` as PyO3's array extraction does.
 
 The ``matmul`` argument names the coarse-tile backend. On this build it is an MFMA precision:
-``"mfma-f32"`` (plain FP32 tiles) or ``"mfma-f16"`` (FP16 tiles whose candidates are re-scored in
-f32); ``"gemm"`` is accepted as the reference's portable name and means ``"mfma-f32"``;
+``"mfma-f32"`` (plain FP32 tiles), ``"mfma-f16"`` (FP16 tiles whose candidates are re-scored in
+f32) or ``"mfma-f16x3"`` (FP16 hi/lo split tiles, three FP16 MFMAs, an FP32-class error bound; the
+default); ``"gemm"`` is accepted as the reference's portable name and means ``"mfma-f32"``;
 ``"accelerate"`` is unavailable here, as on the reference's Linux builds. The output never depends
 on the backend. ``sort_by_norm``, ``prune`` and ``n_threads`` are accepted for compatibility and
 cannot change the output (cosine selection never prunes; the GPU path has no host threads).
@@ -33,8 +34,8 @@ __version__ = "2.1.0+mi355x.0.1.0"
 
 NOT_CONTIGUOUS = "input must be a C-contiguous array; call np.ascontiguousarray first"
 
-_BACKENDS = {"mfma-f32": "fp32", "mfma-f16": "fp16", "gemm": "fp32"}
-_DEFAULT_MATMUL = "mfma-f16"
+_BACKENDS = {"mfma-f32": "fp32", "mfma-f16": "fp16", "mfma-f16x3": "fp16x3", "gemm": "fp32"}
+_DEFAULT_MATMUL = "mfma-f16x3"
 
 
 def default_matmul() -> str:
@@ -140,6 +141,7 @@ def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None
         "col_splits": int(st.col_splits),
         "short_rows": int(st.short_rows),
         "path": "fused" if st.path == 0 else "rowscan",
+        "inserts": int(st.inserts),
         "device_ms": float(st.device_ms),
         "kernel_ms": {"prep": st.prep_ms, "coarse": st.coarse_ms, "rescore": st.rescore_ms,
                       "fallback": st.fallback_ms},

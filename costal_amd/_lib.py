@@ -17,7 +17,7 @@ OK = 0
 E_SHAPE, E_BAD_K, E_TOO_MANY_ROWS, E_NON_FINITE, E_UNAVAILABLE, E_DEVICE, E_UNSUPPORTED, E_INVALID = range(1, 9)
 
 SELECT = {"cosine": 0, "tanimoto": 1}
-PRECISION = {"fp32": 0, "fp16": 1}
+PRECISION = {"fp32": 0, "fp16": 1, "fp16x3": 2}
 WEIGHT = {"tanimoto": 0, "cosine": 1, "jaccard": 2}
 
 # every symbol the header declares
@@ -59,6 +59,7 @@ class KnnStats(C.Structure):
         ("col_splits", C.c_uint64),
         ("short_rows", C.c_uint64),
         ("path", C.c_uint64),
+        ("inserts", C.c_uint64),
         ("device_ms", C.c_double),
         ("prep_ms", C.c_double),
         ("coarse_ms", C.c_double),

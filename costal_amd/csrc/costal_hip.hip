@@ -106,17 +106,20 @@ __device__ __forceinline__ bool ranks_before(double sa, uint32_t ia, double sb, 
 // prep: norms, normalized copies, non-finite check.  One thread per row; rows in [n, n_pad) are
 // written as zeros so that full tiles can be read without bounds checks.
 
+// xs16 rows are dp16 halves (FP16 form) or 2*dp16 halves, hi then lo (split form):
+// hi = fp16(v), lo = fp16(v - hi), where v - hi is exact in f32.
 __global__ void prep_kernel(const float* __restrict__ x, uint32_t n, uint32_t d, uint32_t n_pad,
-                            uint32_t dp32, uint32_t dp16, double* __restrict__ norms,
+                            uint32_t dp32, uint32_t dp16, int split16, double* __restrict__ norms,
                             float* __restrict__ xs32, _Float16* __restrict__ xs16,
                             uint32_t* __restrict__ bad_row) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_pad) return;
   const uint32_t dmax = dp32 > dp16 ? dp32 : dp16;
+  const uint32_t s16 = split16 ? 2 * dp16 : dp16;
   if (i >= n) {
     for (uint32_t t = 0; t < dp32; ++t) xs32[(size_t)i * dp32 + t] = 0.0f;
     if (xs16)
-      for (uint32_t t = 0; t < dp16; ++t) xs16[(size_t)i * dp16 + t] = (_Float16)0.0f;
+      for (uint32_t t = 0; t < s16; ++t) xs16[(size_t)i * s16 + t] = (_Float16)0.0f;
     return;
   }
   const float* row = x + (size_t)i * d;
@@ -133,7 +136,11 @@ __global__ void prep_kernel(const float* __restrict__ x, uint32_t n, uint32_t d,
   for (uint32_t t = 0; t < dmax; ++t) {
     float v = (t < d && nrm > 0.0) ? (float)((double)row[t] / nrm) : 0.0f;
     if (t < dp32) xs32[(size_t)i * dp32 + t] = v;
-    if (xs16 && t < dp16) xs16[(size_t)i * dp16 + t] = (_Float16)v;
+    if (xs16 && t < dp16) {
+      const _Float16 hi = (_Float16)v;
+      xs16[(size_t)i * s16 + t] = hi;
+      if (split16) xs16[(size_t)i * s16 + dp16 + t] = (_Float16)(v - (float)hi);
+    }
   }
 }
 
@@ -160,8 +167,10 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Insert into a query's candidate list (capacity kp).  Returns the new threshold: t0 while the
-// list is not full, else max(t0, smallest score in the list).
+// Insert into a query's candidate list.  Entries [kp, KP) and unused entries hold +inf, so the
+// minimum over all KP entries is the list's smallest score once it is full.  Returns the new
+// threshold: t0 while the list is not full, else max(t0, smallest score).
+template <int KP>
 __device__ __forceinline__ float list_insert(float* __restrict__ sc, uint32_t* __restrict__ id,
                                              int* cnt_p, int* minpos_p, float* th_p, float v,
                                              uint32_t cand, int kp, float t0) {
@@ -173,28 +182,84 @@ __device__ __forceinline__ float list_insert(float* __restrict__ sc, uint32_t* _
     *cnt_p = c;
     if (c < kp) return t0;
   } else {
-    int mp = *minpos_p;
+    // the caller's threshold may predate an insert by the query's other lane: re-check against
+    // the list itself, so a full list only ever trades its minimum for something larger
+    const float cur = *th_p;
+    if (!(v > cur)) return cur;
+    const int mp = *minpos_p;
     sc[mp] = v;
     id[mp] = cand;
   }
-  float mn = sc[0];
+  // min-scan with independent 16-byte LDS reads
+  const f32x4* s4 = reinterpret_cast<const f32x4*>(sc);
+  float mn = INFINITY;
   int mp = 0;
-  for (int e = 1; e < kp; ++e) {
-    float s = sc[e];
-    if (s < mn) {
-      mn = s;
-      mp = e;
+#pragma unroll
+  for (int e = 0; e < KP / 4; ++e) {
+    const f32x4 v4 = s4[e];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (v4[u] < mn) {
+        mn = v4[u];
+        mp = 4 * e + u;
+      }
     }
   }
   *minpos_p = mp;
-  float th = fmaxf(t0, mn);
+  const float th = fmaxf(t0, mn);
   *th_p = th;
   return th;
 }
 
+// Precision forms of the coarse tile:
+//   PREC_F32   f32 rows, v_mfma_f32_32x32x2f32; a 16-byte slot is 4 features (4 MFMAs).
+//   PREC_F16   FP16 rows, v_mfma_f32_32x32x16_f16; a slot is 8 features (1 MFMA); a hit is re-scored
+//              in f32 before admission.
+//   PREC_F16X3 FP16 hi/lo split rows (x = hi + lo + O(2^-22)): slots [0, J/2) hold hi, [J/2, J) lo;
+//              hi.hi + hi.lo + lo.hi per slot (3 MFMAs), an f32-class score without re-scoring.
+constexpr int PREC_F16X3 = COSTAL_PREC_FP16X3;
+
+template <int PREC, int J>
+__device__ __forceinline__ f32x16 tile_mfma(const uint8_t* __restrict__ T, int sub, int h, int l32,
+                                            const uint4 (&qreg)[J]) {
+  f32x16 acc = {};
+  if constexpr (PREC == PREC_F32) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const uint4 bv = *reinterpret_cast<const uint4*>(T + (((sub * J + j) * 2 + h) * 32 + l32) * 16);
+      const f32x4 b4 = __builtin_bit_cast(f32x4, bv);
+      const f32x4 q4 = __builtin_bit_cast(f32x4, qreg[j]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[0], q4[0], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[1], q4[1], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[2], q4[2], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[3], q4[3], acc, 0, 0, 0);
+    }
+  } else if constexpr (PREC == PREC_F16) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const uint4 bv = *reinterpret_cast<const uint4*>(T + (((sub * J + j) * 2 + h) * 32 + l32) * 16);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, bv),
+                                                   __builtin_bit_cast(f16x8, qreg[j]), acc, 0, 0, 0);
+    }
+  } else {
+    constexpr int JH = J / 2;
+#pragma unroll
+    for (int j = 0; j < JH; ++j) {
+      const uint4 bh = *reinterpret_cast<const uint4*>(T + (((sub * J + j) * 2 + h) * 32 + l32) * 16);
+      const uint4 bl = *reinterpret_cast<const uint4*>(T + (((sub * J + j + JH) * 2 + h) * 32 + l32) * 16);
+      const f16x8 ch = __builtin_bit_cast(f16x8, bh), cl = __builtin_bit_cast(f16x8, bl);
+      const f16x8 qh = __builtin_bit_cast(f16x8, qreg[j]), ql = __builtin_bit_cast(f16x8, qreg[j + JH]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(cl, qh, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ch, ql, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ch, qh, acc, 0, 0, 0);
+    }
+  }
+  return acc;
+}
+
 template <int PREC, int J, int KP, int NW>
 __global__ __launch_bounds__(64 * NW) void coarse_kernel(
-    const uint8_t* __restrict__ xsq,   // n_pad rows of 32*J bytes (f32 or f16 working copy)
+    const uint8_t* __restrict__ xsq,   // n_pad rows of 32*J bytes (the working copy of this form)
     const float* __restrict__ xs32,    // f32 working copy (dp32 per row), for the FP16 re-score
     uint32_t dp32, uint32_t n, int kp,
     const uint32_t* __restrict__ rowlist, uint32_t nq,  // queries: rowlist[0..nq) or rows 0..nq-1
@@ -202,7 +267,7 @@ __global__ __launch_bounds__(64 * NW) void coarse_kernel(
     uint32_t cols_per_split,           // candidate columns per split, a multiple of the tile
     float eps_pre,                     // FP16 form: bound on |FP16 score - f32 score|
     uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc, uint32_t* __restrict__ cand_cnt,
-    float* __restrict__ cand_floor) {
+    float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
   constexpr int QB = 32 * NW;
   constexpr int TC = (J <= 4) ? 128 : 64;
   constexpr int SUBT = TC / 32;
@@ -213,7 +278,7 @@ __global__ __launch_bounds__(64 * NW) void coarse_kernel(
   constexpr int CPT = (CHUNKS + NT - 1) / NT;
 
   __shared__ __attribute__((aligned(16))) uint8_t tile[2][TILEB];
-  __shared__ float l_sc[QB * KP];
+  __shared__ __attribute__((aligned(16))) float l_sc[QB * KP];
   __shared__ uint32_t l_id[QB * KP];
   __shared__ int l_cnt[QB];
   __shared__ int l_minpos[QB];
@@ -237,14 +302,16 @@ __global__ __launch_bounds__(64 * NW) void coarse_kernel(
   const float t0 = (tinit && qvalid) ? tinit[qg] : -INFINITY;
   // padding queries get an infinite threshold so they never take the insert path
   const float th0 = qvalid ? t0 : INFINITY;
+  float* my_sc = l_sc + qloc * KP;
+  uint32_t* my_id = l_id + qloc * KP;
   if (h == 0) {
     l_cnt[qloc] = 0;
     l_minpos[qloc] = 0;
     l_th[qloc] = th0;
   }
+  for (int e = h; e < KP; e += 2) my_sc[e] = INFINITY;
   float th = th0;
-  float* my_sc = l_sc + qloc * KP;
-  uint32_t* my_id = l_id + qloc * KP;
+  uint32_t my_inserts = 0;
 
   const uint32_t c_begin = split * cols_per_split;
   uint32_t c_end = c_begin + cols_per_split;
@@ -256,17 +323,17 @@ __global__ __launch_bounds__(64 * NW) void coarse_kernel(
     const uint8_t* src = xsq + (size_t)(c_begin + (uint32_t)t * TC) * ROWB;
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
-      int e = tid + c * NT;
+      const int e = tid + c * NT;
       if (CHUNKS % NT == 0 || e < CHUNKS) pre[c] = *reinterpret_cast<const uint4*>(src + (size_t)e * 16);
     }
   };
   auto store = [&](int buf) {
 #pragma unroll
     for (int c = 0; c < CPT; ++c) {
-      int e = tid + c * NT;
+      const int e = tid + c * NT;
       if (CHUNKS % NT == 0 || e < CHUNKS) {
-        int r = e / (2 * J), q = e % (2 * J);
-        int j = q >> 1, hh = q & 1, sub = r >> 5, lr = r & 31;
+        const int r = e / (2 * J), q = e % (2 * J);
+        const int j = q >> 1, hh = q & 1, sub = r >> 5, lr = r & 31;
         *reinterpret_cast<uint4*>(&tile[buf][(((sub * J + j) * 2 + hh) * 32 + lr) * 16]) = pre[c];
       }
     }
@@ -285,47 +352,37 @@ __global__ __launch_bounds__(64 * NW) void coarse_kernel(
     const uint8_t* T = tile[buf];
 #pragma unroll
     for (int sub = 0; sub < SUBT; ++sub) {
-      f32x16 acc = {};
-#pragma unroll
-      for (int j = 0; j < J; ++j) {
-        uint4 bv = *reinterpret_cast<const uint4*>(T + (((sub * J + j) * 2 + h) * 32 + l32) * 16);
-        if constexpr (PREC == PREC_F32) {
-          f32x4 b4 = __builtin_bit_cast(f32x4, bv);
-          f32x4 q4 = __builtin_bit_cast(f32x4, qreg[j]);
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[0], q4[0], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[1], q4[1], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[2], q4[2], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[3], q4[3], acc, 0, 0, 0);
-        } else {
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, bv),
-                                                       __builtin_bit_cast(f16x8, qreg[j]), acc, 0,
-                                                       0, 0);
-        }
-      }
-      float m01 = fmaxf(fmaxf(acc[0], acc[1]), acc[2]);
-      float m02 = fmaxf(fmaxf(acc[3], acc[4]), acc[5]);
-      float m03 = fmaxf(fmaxf(acc[6], acc[7]), acc[8]);
-      float m04 = fmaxf(fmaxf(acc[9], acc[10]), acc[11]);
-      float m05 = fmaxf(fmaxf(acc[12], acc[13]), acc[14]);
-      float m = fmaxf(fmaxf(fmaxf(m01, m02), m03), fmaxf(fmaxf(m04, m05), acc[15]));
+      const f32x16 acc = tile_mfma<PREC, J>(T, sub, h, l32, qreg);
+      const float m01 = fmaxf(fmaxf(acc[0], acc[1]), acc[2]);
+      const float m02 = fmaxf(fmaxf(acc[3], acc[4]), acc[5]);
+      const float m03 = fmaxf(fmaxf(acc[6], acc[7]), acc[8]);
+      const float m04 = fmaxf(fmaxf(acc[9], acc[10]), acc[11]);
+      const float m05 = fmaxf(fmaxf(acc[12], acc[13]), acc[14]);
+      const float m = fmaxf(fmaxf(fmaxf(m01, m02), m03), fmaxf(fmaxf(m04, m05), acc[15]));
       float the = (PREC == PREC_F16) ? th - eps_pre : th;
       if (__builtin_amdgcn_ballot_w64(m > the) != 0) {
-        const uint32_t cbase = c_begin + (uint32_t)t * TC + sub * 32;
+        // rare path: the two lanes of a query insert one after the other
+        const uint32_t cbase = c_begin + (uint32_t)t * TC + sub * 32 + 4 * h;
+#pragma unroll
         for (int hh = 0; hh < 2; ++hh) {
-          if (h == hh && qvalid) {
+          if (h == hh && m > the) {
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
               float v = acc[i];
               if (v > the) {
-                const uint32_t cand = cbase + 8 * (i >> 2) + 4 * h + (i & 3);
+                const uint32_t cand = cbase + 8 * (i >> 2) + (i & 3);
                 if (cand < c_end && cand != qg) {
+                  bool admit = true;
                   if constexpr (PREC == PREC_F16) {
                     v = dot32(xs32 + (size_t)qg * dp32, xs32 + (size_t)cand * dp32, dp32);
-                    if (!(v > th)) continue;
+                    admit = v > th;
                   }
-                  th = list_insert(my_sc, my_id, &l_cnt[qloc], &l_minpos[qloc], &l_th[qloc], v,
-                                   cand, kp, t0);
-                  the = (PREC == PREC_F16) ? th - eps_pre : th;
+                  if (admit) {
+                    th = list_insert<KP>(my_sc, my_id, &l_cnt[qloc], &l_minpos[qloc], &l_th[qloc], v,
+                                         cand, kp, t0);
+                    the = (PREC == PREC_F16) ? th - eps_pre : th;
+                    ++my_inserts;
+                  }
                 }
               }
             }
@@ -339,6 +396,13 @@ __global__ __launch_bounds__(64 * NW) void coarse_kernel(
     __syncthreads();
   }
 
+  if (inserts) {
+    // one atomic per wave
+    uint32_t tot = my_inserts;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+    if (l == 0) atomicAdd(inserts, (unsigned long long)tot);
+  }
   if (h == 0 && qvalid) {
     const int c = l_cnt[qloc];
     const size_t slot = (size_t)qq * nsplit + split;
@@ -395,7 +459,7 @@ __global__ __launch_bounds__(64) void rescore_kernel(
     const float* __restrict__ x, uint32_t d, const double* __restrict__ norms, uint32_t n,
     uint32_t k, int kp, uint32_t nsplit, const uint32_t* __restrict__ rowlist, uint32_t nq,
     const uint32_t* __restrict__ cand_idx, const uint32_t* __restrict__ cand_cnt,
-    const float* __restrict__ cand_floor, int all_admitted, double eps32,
+    const float* __restrict__ cand_floor, int all_admitted, double eps_list, double eps32,
     uint32_t* __restrict__ idx, double* __restrict__ cos_out, double* __restrict__ wts,
     uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo, uint32_t* __restrict__ counters) {
   __shared__ double s_sc[CMAX];
@@ -439,7 +503,7 @@ __global__ __launch_bounds__(64) void rescore_kernel(
   } else if (total < k) {
     ok = false;
   } else {
-    ok = s_sc[k - 1] > floor_ + eps32;
+    ok = s_sc[k - 1] > floor_ + eps_list;
   }
   if (ok) {
     for (uint32_t t = threadIdx.x; t < k; t += 64) {
@@ -680,7 +744,7 @@ struct Plan {
 
 typedef void (*CoarseFn)(const uint8_t*, const float*, uint32_t, uint32_t, int, const uint32_t*,
                          uint32_t, const float*, uint32_t, float, uint32_t*, float*, uint32_t*,
-                         float*);
+                         float*, unsigned long long*);
 
 template <int PREC, int J, int KP, int NW>
 CoarseFn coarse_ptr() {
@@ -706,7 +770,9 @@ CoarseFn coarse_by_j(uint32_t J) {
 // with 2 waves (the same LDS footprint).
 CoarseFn coarse_select(int prec, uint32_t J, int KP) {
   if (prec == PREC_F32) return KP == 32 ? coarse_by_j<PREC_F32, 32, 4>(J) : coarse_by_j<PREC_F32, 64, 2>(J);
-  return KP == 32 ? coarse_by_j<PREC_F16, 32, 4>(J) : coarse_by_j<PREC_F16, 64, 2>(J);
+  if (prec == PREC_F16) return KP == 32 ? coarse_by_j<PREC_F16, 32, 4>(J) : coarse_by_j<PREC_F16, 64, 2>(J);
+  if (J % 2) return nullptr;
+  return KP == 32 ? coarse_by_j<PREC_F16X3, 32, 4>(J) : coarse_by_j<PREC_F16X3, 64, 2>(J);
 }
 
 // FP16 candidate error bound.  x16 = fp16(fp32(x/|x|)): |x16 - x| <= U|x| + eta per component with
@@ -724,6 +790,20 @@ double fp16_eps(uint32_t d) {
 
 double fp32_eps(uint32_t d) { return ((double)d + 4.0) * (double)std::ldexp(1.0, -23); }
 
+// Split-FP16 bound.  v = f32(x/|x|) (relative error 2^-24), hi = fp16(v), lo = fp16(v - hi): the
+// represented value hi + lo is within a|x| + eta of x per component, a = 2^-22 (1 + 2^-24) + 2^-24.
+// The MFMA sum hi.hi + hi.lo + lo.hi differs from (hi+lo).(hi+lo) by the dropped lo.lo
+// (<= (2^-11 + eta sqrt d)^2), the represented rows differ from the unit rows by
+// (a + eta sqrt d)(2 + a + eta sqrt d), and the FP32 accumulation of 3d exact products adds
+// gamma_3d (1 + 2^-11 + eta sqrt d)^2.
+double fp16x3_eps(uint32_t d) {
+  const double u32 = std::ldexp(1.0, -24), u16 = std::ldexp(1.0, -11);
+  const double a = std::ldexp(1.0, -22) * (1.0 + u32) + u32;
+  const double e = std::ldexp(1.0, -25) * std::sqrt((double)d);
+  const double gam = 3.0 * d * u32 / (1.0 - 3.0 * d * u32);
+  return (a + e) * (2.0 + a + e) + (u16 + e) * (u16 + e) + gam * (1.0 + u16 + e) * (1.0 + u16 + e);
+}
+
 Plan make_plan(const Device& D, uint64_t n, uint64_t d, uint64_t k, const costal_knn_params& p) {
   Plan P{};
   const uint64_t margin = p.margin < 0 ? k : (uint64_t)p.margin;
@@ -733,7 +813,7 @@ Plan make_plan(const Device& D, uint64_t n, uint64_t d, uint64_t k, const costal
   P.dp16 = round_up((uint32_t)std::max<uint64_t>(d, 1), 16);
   const uint32_t dmax = P.prec == PREC_F16 ? 128 : 64;
   P.fused = p.select == COSTAL_SELECT_COSINE && d <= dmax && P.kp <= FUSED_KP_MAX;
-  P.J = P.prec == PREC_F16 ? P.dp16 / 16 : P.dp32 / 8;
+  P.J = P.prec == PREC_F16 ? P.dp16 / 16 : P.prec == PREC_F16X3 ? P.dp16 / 8 : P.dp32 / 8;
   P.KP = P.kp <= 32 ? 32 : 64;
   P.NW = P.KP == 32 ? 4 : 2;
   P.QB = 32 * P.NW;
@@ -797,18 +877,20 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
                       uint32_t* idx, double* cos, double* norms_out, double* wts, costal_knn_stats* st,
                       hipStream_t s, bool brute) {
   const uint32_t n = (uint32_t)n64, d = (uint32_t)d64, k = (uint32_t)p.k;
-  if (p.precision > COSTAL_PREC_FP16) return fail(COSTAL_E_INVALID, "unknown precision %u", p.precision);
+  if (p.precision > COSTAL_PREC_FP16X3) return fail(COSTAL_E_INVALID, "unknown precision %u", p.precision);
   if (p.select > COSTAL_SELECT_TANIMOTO) return fail(COSTAL_E_INVALID, "unknown select %u", p.select);
   Plan P = make_plan(D, n, d, k, p);
   if (brute) P.fused = false;
-  const bool need16 = P.fused && P.prec == PREC_F16;
+  const bool need16 = P.fused && (P.prec == PREC_F16 || P.prec == PREC_F16X3);
+  const int split16 = P.prec == PREC_F16X3 ? 1 : 0;
+  const size_t row16 = (size_t)P.dp16 * (split16 ? 2 : 1);
   const size_t nq_slots = P.fused ? (size_t)n * P.nsplit : 0;
 
   size_t bytes = 0;
   {
     Carve c(nullptr);
     c.take<float>((size_t)P.n_pad * P.dp32);
-    if (need16) c.take<_Float16>((size_t)P.n_pad * P.dp16);
+    if (need16) c.take<_Float16>((size_t)P.n_pad * row16);
     c.take<double>(n);
     c.take<uint32_t>(nq_slots * P.kp);
     c.take<float>(nq_slots * P.kp);
@@ -823,7 +905,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   if (rc) return rc;
   Carve c(D.ws);
   float* xs32 = c.take<float>((size_t)P.n_pad * P.dp32);
-  _Float16* xs16 = need16 ? c.take<_Float16>((size_t)P.n_pad * P.dp16) : nullptr;
+  _Float16* xs16 = need16 ? c.take<_Float16>((size_t)P.n_pad * row16) : nullptr;
   double* norms_ws = c.take<double>(n);
   uint32_t* cand_idx = c.take<uint32_t>(nq_slots * P.kp);
   float* cand_sc = c.take<float>(nq_slots * P.kp);
@@ -831,17 +913,18 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   float* cand_floor = c.take<float>(nq_slots);
   uint32_t* fb_rows = c.take<uint32_t>(n);
   double* fb_lo = c.take<double>(n);
-  uint32_t* counters = c.take<uint32_t>(64);  // [0] first bad row, [1] fallback rows, [2] short rows
+  // [0] first bad row, [1] fallback rows, [2] short rows, [4..5] list inserts (u64)
+  uint32_t* counters = c.take<uint32_t>(64);
   double* norms = norms_out ? norms_out : norms_ws;
 
   HIP_TRY(hipEventRecord(D.ev0, s));
   {
-    uint32_t init[4] = {NONE, 0, 0, 0};
+    uint32_t init[8] = {NONE, 0, 0, 0, 0, 0, 0, 0};
     HIP_TRY(hipMemcpyAsync(counters, init, sizeof init, hipMemcpyHostToDevice, s));
   }
   HIP_TRY(hipEventRecord(D.evk[0], s));
-  prep_kernel<<<(P.n_pad + 255) / 256, 256, 0, s>>>(x, n, d, P.n_pad, P.dp32, P.dp16, norms, xs32, xs16,
-                                                    counters);
+  prep_kernel<<<(P.n_pad + 255) / 256, 256, 0, s>>>(x, n, d, P.n_pad, P.dp32, P.dp16, split16, norms, xs32,
+                                                    xs16, counters);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(D.evk[1], s));
   uint32_t bad = NONE;
@@ -858,14 +941,18 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     dim3 grid((n + P.QB - 1) / P.QB, P.nsplit);
     HIP_TRY(hipEventRecord(D.evk[2], s));
     fn<<<grid, 64 * P.NW, 0, s>>>(xsq, xs32, P.dp32, n, (int)P.kp, nullptr, n, p.row_threshold,
-                                  P.cols_per_split, eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor);
+                                  P.cols_per_split, eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor,
+                                  reinterpret_cast<unsigned long long*>(counters + 4));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(D.evk[3], s));
     const int all_admitted = (P.nsplit == 1 && P.kp >= n - 1 && !p.row_threshold) ? 1 : 0;
+    // bound on |list score - exact| for the certificate: f32-accurate scores, or split FP16
+    const double eps_list = P.prec == PREC_F16X3 ? fp16x3_eps(d) : eps32;
     const uint32_t slots = P.nsplit * P.kp;
 #define RESCORE(CM)                                                                                    \
   rescore_kernel<CM><<<n, 64, 0, s>>>(x, d, norms, n, k, (int)P.kp, P.nsplit, nullptr, n, cand_idx,      \
-                                      cand_cnt, cand_floor, all_admitted, eps32, idx, cos, wts, fb_rows, \
+                                      cand_cnt, cand_floor, all_admitted, eps_list, eps32, idx, cos, wts,  \
+                                      fb_rows,                                                          \
                                       fb_lo, counters)
     if (slots <= 64) RESCORE(64);
     else if (slots <= 128) RESCORE(128);
@@ -890,7 +977,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     HIP_TRY(hipEventRecord(D.evk[5], s));
   }
   HIP_TRY(hipEventRecord(D.ev1, s));
-  uint32_t cnt[4] = {0, 0, 0, 0};
+  uint32_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   HIP_TRY(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   if (st) {
@@ -905,6 +992,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     st->candidates = P.kp;
     st->col_splits = P.fused ? P.nsplit : 1;
     st->path = P.fused ? 0 : 1;
+    st->inserts = (uint64_t)cnt[4] | ((uint64_t)cnt[5] << 32);
     st->device_ms = ms;
     auto span = [&](int a, int b) {
       float v = 0.0f;

@@ -51,9 +51,11 @@ extern "C" {
 #define COSTAL_SELECT_COSINE 0
 #define COSTAL_SELECT_TANIMOTO 1
 
-/* coarse-tile precision: plain FP32 MFMA, or FP16 MFMA candidates refined in FP32 */
+/* coarse-tile precision: plain FP32 MFMA; FP16 MFMA whose hits are re-scored in FP32 before
+ * admission; or FP16 hi/lo split rows (3 FP16 MFMAs per slot) with an FP32-class error bound */
 #define COSTAL_PREC_FP32 0
 #define COSTAL_PREC_FP16 1
+#define COSTAL_PREC_FP16X3 2
 
 /* edge weights (weight.rs:10-26) */
 #define COSTAL_WEIGHT_TANIMOTO 0
@@ -89,6 +91,7 @@ typedef struct costal_knn_stats {
   uint64_t col_splits;        /* candidate-range splits used */
   uint64_t short_rows;        /* rows with fewer than k admitted candidates (thresholds only) */
   uint64_t path;              /* 0 fused MFMA scan, 1 exact row scan (wide d, large K', tanimoto) */
+  uint64_t inserts;           /* candidate-list inserts made by the fused scan */
   double device_ms;           /* device time of the call (hipEvent), 0 when not measured */
   double prep_ms;             /* norms + working copies kernel (hipEvent) */
   double coarse_ms;           /* fused MFMA tile + scan kernel (hipEvent) */

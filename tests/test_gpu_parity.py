@@ -17,7 +17,7 @@ from costal_amd import synth
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-BACKENDS = ["mfma-f32", "mfma-f16"]
+BACKENDS = ["mfma-f32", "mfma-f16", "mfma-f16x3"]
 W_RTOL = 1e-12
 
 

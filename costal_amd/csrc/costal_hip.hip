@@ -26,7 +26,9 @@
 // admission threshold.
 //
 // Built with -ffp-contract=off: the f64 expressions (cos64, tani_from_cos) must round as written.
-// f32 inner products that only feed the coarse stage use explicit fmaf.
+// f32 inner products that only feed the coarse stage use explicit fmaf.  Device code is built
+// without IEEE NaN quieting (inputs are checked for non-finite values before any scan runs), so
+// max/compare chains need no canonicalizing moves, and MFMA results stay in VGPRs.
 
 #include <hip/hip_runtime.h>
 
@@ -123,11 +125,14 @@ __global__ void prep_kernel(const float* __restrict__ x, uint32_t n, uint32_t d,
     return;
   }
   const float* row = x + (size_t)i * d;
+  // the non-finite test reads the bits as integers, so no floating-point assumption can fold it
+  const uint32_t* row_bits = reinterpret_cast<const uint32_t*>(row);
   double s = 0.0;
   bool bad = false;
   for (uint32_t t = 0; t < d; ++t) {
-    float v = row[t];
-    bad |= !isfinite(v);
+    const uint32_t b = row_bits[t];
+    bad |= (b & 0x7f800000u) == 0x7f800000u;  // inf or NaN
+    const float v = __uint_as_float(b);
     s += (double)v * (double)v;
   }
   const double nrm = sqrt(s);
@@ -148,67 +153,64 @@ __global__ void prep_kernel(const float* __restrict__ x, uint32_t n, uint32_t d,
 // coarse_kernel
 //
 // Workgroup = NW waves; each wave owns 32 queries, held in registers for the whole launch.  The
-// workgroup streams its candidate range through LDS in tiles of TC rows (double buffered, one
-// barrier per tile).  For every 32-candidate sub-tile a wave issues the MFMAs for
-// D[cand][query] = sum_k cand[k] * query[k]: lane l owns query (l % 32) and the 16 candidates
-// 8*(i/4) + 4*(l/32) + (i%4), i = 0..15 (the 32x32 accumulator layout).  The scan reduces those 16
-// values with max3 and compares once with the query's threshold; only waves with a hit take the
-// insert path.
+// workgroup streams its candidate range through a 4-stage LDS ring of TC-row tiles filled by
+// global_load_lds (direct to LDS, no register staging), three tiles ahead of the compute.  For every
+// 32-candidate sub-tile a wave issues the MFMAs for D[cand][query] = sum_k cand[k] * query[k]:
+// lane l owns query (l % 32) and the 16 candidates 8*(i/4) + 4*(l/32) + (i%4), i = 0..15 (the
+// 32x32 accumulator layout).  The scan takes the max of a group of sub-tiles (max3 chains) and
+// compares once with the lane's threshold; only waves with a hit take the insert path.
+//
+// Candidate lists live in registers: each lane keeps its own sorted list of the best K' (score,
+// candidate) pairs among the candidates it sees (its half of every sub-tile), so the two lanes of a
+// query never wait on each other.  Both lists go to the rescoring, which treats them like two
+// candidate-range splits: a candidate outside both lists scored at most the larger of the two
+// floors.
 //
 // Row layout (global and LDS): a row is 2*J 16-byte chunks; chunk q feeds lane half h = q % 2,
 // slot j = q / 2.  Queries and candidates use the same chunk -> (lane half, slot) map, so each
 // MFMA contracts matching feature indices (any consistent permutation of k leaves the sum a sum of
 // the same products).  In LDS the chunks of one (sub-tile, slot, half) are stored for the 32 lanes
-// contiguously, so each ds_read_b128 is conflict free.
+// contiguously (1 KB per wave-wide load), so each ds_read_b128 is conflict free.
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#define LDS_PTR(p) ((__attribute__((address_space(3))) void*)(p))
+
+// s_waitcnt vmcnt(N) with lgkmcnt and expcnt left alone (gfx9 encoding)
+#define WAIT_VMCNT(N) __builtin_amdgcn_s_waitcnt(0x0f70 | ((N) & 15) | (((N) >> 4) << 14))
+
+__device__ __forceinline__ float max16(const f32x16& a) {
+  const float m0 = fmaxf(fmaxf(a[0], a[1]), a[2]);
+  const float m1 = fmaxf(fmaxf(a[3], a[4]), a[5]);
+  const float m2 = fmaxf(fmaxf(a[6], a[7]), a[8]);
+  const float m3 = fmaxf(fmaxf(a[9], a[10]), a[11]);
+  const float m4 = fmaxf(fmaxf(a[12], a[13]), a[14]);
+  return fmaxf(fmaxf(fmaxf(m0, m1), m2), fmaxf(fmaxf(m3, m4), a[15]));
 }
 
-// Insert into a query's candidate list.  Entries [kp, KP) and unused entries hold +inf, so the
-// minimum over all KP entries is the list's smallest score once it is full.  Returns the new
-// threshold: t0 while the list is not full, else max(t0, smallest score).
-template <int KP>
-__device__ __forceinline__ float list_insert(float* __restrict__ sc, uint32_t* __restrict__ id,
-                                             int* cnt_p, int* minpos_p, float* th_p, float v,
-                                             uint32_t cand, int kp, float t0) {
-  int c = *cnt_p;
-  if (c < kp) {
-    sc[c] = v;
-    id[c] = cand;
-    ++c;
-    *cnt_p = c;
-    if (c < kp) return t0;
-  } else {
-    // the caller's threshold may predate an insert by the query's other lane: re-check against
-    // the list itself, so a full list only ever trades its minimum for something larger
-    const float cur = *th_p;
-    if (!(v > cur)) return cur;
-    const int mp = *minpos_p;
-    sc[mp] = v;
-    id[mp] = cand;
-  }
-  // min-scan with independent 16-byte LDS reads
-  const f32x4* s4 = reinterpret_cast<const f32x4*>(sc);
-  float mn = INFINITY;
-  int mp = 0;
+// a[i] for a lane-varying i, as a select chain (keeps the accumulator in registers)
+__device__ __forceinline__ float pick16(const f32x16& a, int i) {
+  float r = a[0];
 #pragma unroll
-  for (int e = 0; e < KP / 4; ++e) {
-    const f32x4 v4 = s4[e];
+  for (int e = 1; e < 16; ++e) r = (i == e) ? a[e] : r;
+  return r;
+}
+
+// Sorted (ascending) register list: ls[0] is the smallest live score.  Entries [kp, KPR) hold +inf
+// and never move; empty entries hold -inf.  push() replaces the minimum and bubbles the new entry
+// up to its place.
+template <int KPR>
+__device__ __forceinline__ void list_push(float (&ls)[KPR], uint32_t (&li)[KPR], float v, uint32_t c) {
+  ls[0] = v;
+  li[0] = c;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (v4[u] < mn) {
-        mn = v4[u];
-        mp = 4 * e + u;
-      }
-    }
+  for (int e = 0; e < KPR - 1; ++e) {
+    const float a = ls[e], b = ls[e + 1];
+    if (!(a > b)) break;
+    const uint32_t ia = li[e], ib = li[e + 1];
+    ls[e] = b;
+    ls[e + 1] = a;
+    li[e] = ib;
+    li[e + 1] = ia;
   }
-  *minpos_p = mp;
-  const float th = fmaxf(t0, mn);
-  *th_p = th;
-  return th;
 }
 
 // Precision forms of the coarse tile:
@@ -257,7 +259,7 @@ __device__ __forceinline__ f32x16 tile_mfma(const uint8_t* __restrict__ T, int s
   return acc;
 }
 
-template <int PREC, int J, int KP, int NW>
+template <int PREC, int J, int KPR, int NW>
 __global__ __launch_bounds__(64 * NW) void coarse_kernel(
     const uint8_t* __restrict__ xsq,   // n_pad rows of 32*J bytes (the working copy of this form)
     const float* __restrict__ xs32,    // f32 working copy (dp32 per row), for the FP16 re-score
@@ -269,25 +271,24 @@ __global__ __launch_bounds__(64 * NW) void coarse_kernel(
     uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc, uint32_t* __restrict__ cand_cnt,
     float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
   constexpr int QB = 32 * NW;
-  constexpr int TC = (J <= 4) ? 128 : 64;
+  constexpr int TC = J <= 2 ? 256 : J <= 4 ? 128 : 64;  // 16 KB tiles
   constexpr int SUBT = TC / 32;
   constexpr int ROWB = 32 * J;
   constexpr int TILEB = TC * ROWB;
-  constexpr int CHUNKS = TC * 2 * J;
   constexpr int NT = 64 * NW;
+  constexpr int CHUNKS = TC * 2 * J;        // 16-byte chunks per tile
   constexpr int CPT = (CHUNKS + NT - 1) / NT;
+  // sub-tiles whose MFMAs are issued together before one scan (FP32 has 4x the MFMAs per sub-tile)
+  constexpr int GMAX = PREC == PREC_F32 ? 2 : 4;
+  constexpr int GS = SUBT < GMAX ? SUBT : GMAX;
+  static_assert(SUBT % GS == 0, "sub-tile grouping");
 
-  __shared__ __attribute__((aligned(16))) uint8_t tile[2][TILEB];
-  __shared__ __attribute__((aligned(16))) float l_sc[QB * KP];
-  __shared__ uint32_t l_id[QB * KP];
-  __shared__ int l_cnt[QB];
-  __shared__ int l_minpos[QB];
-  __shared__ float l_th[QB];
+  __shared__ __attribute__((aligned(16))) uint8_t tile0[TILEB];
+  __shared__ __attribute__((aligned(16))) uint8_t tile1[TILEB];
 
   const int tid = threadIdx.x;
   const int w = tid >> 6, l = tid & 63, h = l >> 5, l32 = l & 31;
-  const int qloc = w * 32 + l32;
-  const uint32_t qq = blockIdx.x * QB + qloc;
+  const uint32_t qq = blockIdx.x * QB + w * 32 + l32;
   const bool qvalid = qq < nq;
   const uint32_t qg = qvalid ? (rowlist ? rowlist[qq] : qq) : 0;
   const uint32_t split = blockIdx.y, nsplit = gridDim.y;
@@ -300,17 +301,15 @@ __global__ __launch_bounds__(64 * NW) void coarse_kernel(
     for (int j = 0; j < J; ++j) qreg[j] = *reinterpret_cast<const uint4*>(qrow + (2 * j + h) * 16);
   }
   const float t0 = (tinit && qvalid) ? tinit[qg] : -INFINITY;
-  // padding queries get an infinite threshold so they never take the insert path
-  const float th0 = qvalid ? t0 : INFINITY;
-  float* my_sc = l_sc + qloc * KP;
-  uint32_t* my_id = l_id + qloc * KP;
-  if (h == 0) {
-    l_cnt[qloc] = 0;
-    l_minpos[qloc] = 0;
-    l_th[qloc] = th0;
+  float ls[KPR];
+  uint32_t li[KPR];
+#pragma unroll
+  for (int e = 0; e < KPR; ++e) {
+    ls[e] = e < kp ? -INFINITY : INFINITY;
+    li[e] = NONE;
   }
-  for (int e = h; e < KP; e += 2) my_sc[e] = INFINITY;
-  float th = th0;
+  // padding queries get an infinite threshold so they never take the insert path
+  float th = qvalid ? fmaxf(t0, ls[0]) : INFINITY;
   uint32_t my_inserts = 0;
 
   const uint32_t c_begin = split * cols_per_split;
@@ -318,83 +317,114 @@ __global__ __launch_bounds__(64 * NW) void coarse_kernel(
   if (c_end > n) c_end = n;
   const int ntile = c_begin < c_end ? (int)((c_end - c_begin + TC - 1) / TC) : 0;
 
-  uint4 pre[CPT];
-  auto load = [&](int t) {
-    const uint8_t* src = xsq + (size_t)(c_begin + (uint32_t)t * TC) * ROWB;
-#pragma unroll
-    for (int c = 0; c < CPT; ++c) {
-      const int e = tid + c * NT;
-      if (CHUNKS % NT == 0 || e < CHUNKS) pre[c] = *reinterpret_cast<const uint4*>(src + (size_t)e * 16);
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int c = 0; c < CPT; ++c) {
-      const int e = tid + c * NT;
-      if (CHUNKS % NT == 0 || e < CHUNKS) {
-        const int r = e / (2 * J), q = e % (2 * J);
-        const int j = q >> 1, hh = q & 1, sub = r >> 5, lr = r & 31;
-        *reinterpret_cast<uint4*>(&tile[buf][(((sub * J + j) * 2 + hh) * 32 + lr) * 16]) = pre[c];
-      }
-    }
-  };
-
-  if (ntile > 0) {
-    load(0);
-    store(0);
+  // Tiles go global -> registers -> LDS, two tiles ahead of the scan (register sets ra and rb, LDS
+  // buffers tile0 and tile1, the loop unrolled by two so every register and buffer is static).
+  static_assert(CPT <= 4, "tile staging uses at most four 16-byte registers per thread");
+  uint4 ra0 = {}, ra1 = {}, ra2 = {}, ra3 = {}, rb0 = {}, rb1 = {}, rb2 = {}, rb3 = {};
+  // Thread e fills LDS slot e (consecutive lanes, consecutive 16-byte slots: conflict free); slot
+  // e = ((sub*J + j)*2 + half)*32 + lane32 holds chunk 2j + half of tile row sub*32 + lane32.
+#define LOAD_ONE(R, C)                                                                            \
+  if constexpr (C < CPT) {                                                                        \
+    const int e_ = tid + C * NT;                                                                  \
+    if (CHUNKS % NT == 0 || e_ < CHUNKS) {                                                        \
+      const int lr_ = e_ & 31, hh_ = (e_ >> 5) & 1, j_ = (e_ >> 6) % J, sub_ = (e_ >> 6) / J;     \
+      R##C = *reinterpret_cast<const uint4*>(src_ + (size_t)(sub_ * 32 + lr_) * ROWB + (2 * j_ + hh_) * 16); \
+    }                                                                                             \
   }
-  __syncthreads();
+#define LOAD_REGS(R, T)                                                                           \
+  {                                                                                               \
+    const uint8_t* src_ = xsq + (size_t)(c_begin + (uint32_t)(T) * TC) * ROWB;                     \
+    LOAD_ONE(R, 0) LOAD_ONE(R, 1) LOAD_ONE(R, 2) LOAD_ONE(R, 3)                                   \
+  }
+#define STORE_ONE(R, C, BUF)                                                                      \
+  if constexpr (C < CPT) {                                                                        \
+    const int e_ = tid + C * NT;                                                                  \
+    if (CHUNKS % NT == 0 || e_ < CHUNKS) *reinterpret_cast<uint4*>(&BUF[e_ * 16]) = R##C;         \
+  }
+#define STORE_REGS(R, BUF) \
+  { STORE_ONE(R, 0, BUF) STORE_ONE(R, 1, BUF) STORE_ONE(R, 2, BUF) STORE_ONE(R, 3, BUF) }
+  // LDS writes of this wave done, then everyone meets; no wait on the global loads in flight
+#define TILE_BARRIER()                                                                            \
+  {                                                                                               \
+    __builtin_amdgcn_s_waitcnt(0xc07f); /* lgkmcnt(0) */                                          \
+    __builtin_amdgcn_s_barrier();                                                                 \
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");                                        \
+  }
 
-  for (int t = 0; t < ntile; ++t) {
-    const int buf = t & 1;
-    const bool more = t + 1 < ntile;
-    if (more) load(t + 1);
-    const uint8_t* T = tile[buf];
+  const float* xq32 = xs32 + (size_t)qg * dp32;
+  f32x16 acc[GS];
+  int t = 0;
+  auto scan_group = [&](int g) __attribute__((always_inline)) {
+    float mu[GS];
 #pragma unroll
-    for (int sub = 0; sub < SUBT; ++sub) {
-      const f32x16 acc = tile_mfma<PREC, J>(T, sub, h, l32, qreg);
-      const float m01 = fmaxf(fmaxf(acc[0], acc[1]), acc[2]);
-      const float m02 = fmaxf(fmaxf(acc[3], acc[4]), acc[5]);
-      const float m03 = fmaxf(fmaxf(acc[6], acc[7]), acc[8]);
-      const float m04 = fmaxf(fmaxf(acc[9], acc[10]), acc[11]);
-      const float m05 = fmaxf(fmaxf(acc[12], acc[13]), acc[14]);
-      const float m = fmaxf(fmaxf(fmaxf(m01, m02), m03), fmaxf(fmaxf(m04, m05), acc[15]));
-      float the = (PREC == PREC_F16) ? th - eps_pre : th;
-      if (__builtin_amdgcn_ballot_w64(m > the) != 0) {
-        // rare path: the two lanes of a query insert one after the other
-        const uint32_t cbase = c_begin + (uint32_t)t * TC + sub * 32 + 4 * h;
+    for (int u = 0; u < GS; ++u) mu[u] = max16(acc[u]);
+    float m = mu[0];
 #pragma unroll
-        for (int hh = 0; hh < 2; ++hh) {
-          if (h == hh && m > the) {
+    for (int u = 1; u < GS; ++u) m = fmaxf(m, mu[u]);
+    float the = (PREC == PREC_F16) ? th - eps_pre : th;
+    if (__builtin_amdgcn_ballot_w64(m > the) != 0) {
+      // rare path: each lane walks its own hits, sub-tile by sub-tile
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-              float v = acc[i];
-              if (v > the) {
-                const uint32_t cand = cbase + 8 * (i >> 2) + (i & 3);
-                if (cand < c_end && cand != qg) {
-                  bool admit = true;
-                  if constexpr (PREC == PREC_F16) {
-                    v = dot32(xs32 + (size_t)qg * dp32, xs32 + (size_t)cand * dp32, dp32);
-                    admit = v > th;
-                  }
-                  if (admit) {
-                    th = list_insert<KP>(my_sc, my_id, &l_cnt[qloc], &l_minpos[qloc], &l_th[qloc], v,
-                                         cand, kp, t0);
-                    the = (PREC == PREC_F16) ? th - eps_pre : th;
-                    ++my_inserts;
-                  }
-                }
+      for (int u = 0; u < GS; ++u) {
+        if (mu[u] > the) {
+          uint32_t mask = 0;
+#pragma unroll
+          for (int i = 0; i < 16; ++i) mask |= (acc[u][i] > the ? 1u : 0u) << i;
+          const uint32_t cbase = c_begin + (uint32_t)t * TC + (uint32_t)(g + u) * 32 + 4 * h;
+          while (mask) {
+            const int i = __builtin_ctz(mask);
+            mask &= mask - 1;
+            float v = pick16(acc[u], i);
+            const uint32_t cand = cbase + 8 * (i >> 2) + (i & 3);
+            if (v > the && cand < c_end && cand != qg) {
+              if constexpr (PREC == PREC_F16) {
+                v = dot32(xq32, xs32 + (size_t)cand * dp32, dp32);
+                if (!(v > th)) continue;
               }
+              list_push<KPR>(ls, li, v, cand);
+              th = fmaxf(t0, ls[0]);
+              the = (PREC == PREC_F16) ? th - eps_pre : th;
+              ++my_inserts;
             }
           }
-          wave_sync();
         }
-        th = l_th[qloc];
       }
     }
-    if (more) store(buf ^ 1);
-    __syncthreads();
+  };
+#define SCAN_TILE(BUF)                                                                            \
+  _Pragma("unroll") for (int g = 0; g < SUBT; g += GS) {                                          \
+    _Pragma("unroll") for (int u = 0; u < GS; ++u) acc[u] = tile_mfma<PREC, J>(BUF, g + u, h, l32, qreg); \
+    scan_group(g);                                                                                \
   }
+
+  if (ntile > 0) {
+    LOAD_REGS(ra, 0);
+    LOAD_REGS(rb, ntile > 1 ? 1 : 0);
+    STORE_REGS(ra, tile0);
+    TILE_BARRIER();
+    LOAD_REGS(ra, ntile > 2 ? 2 : ntile - 1);
+    for (;;) {
+      // tile t in tile0, rb = tile t+1, ra = tile t+2 in flight
+      SCAN_TILE(tile0);
+      STORE_REGS(rb, tile1);
+      TILE_BARRIER();
+      LOAD_REGS(rb, t + 3 < ntile ? t + 3 : ntile - 1);
+      if (++t >= ntile) break;
+      // tile t in tile1, ra = tile t+1, rb = tile t+2 in flight
+      SCAN_TILE(tile1);
+      STORE_REGS(ra, tile0);
+      TILE_BARRIER();
+      LOAD_REGS(ra, t + 3 < ntile ? t + 3 : ntile - 1);
+      if (++t >= ntile) break;
+    }
+  }
+#undef SCAN_TILE
+#undef TILE_BARRIER
+#undef LOAD_REGS
+#undef STORE_REGS
+#undef LOAD_ONE
+#undef STORE_ONE
+
 
   if (inserts) {
     // one atomic per wave
@@ -403,16 +433,20 @@ __global__ __launch_bounds__(64 * NW) void coarse_kernel(
     for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
     if (l == 0) atomicAdd(inserts, (unsigned long long)tot);
   }
-  if (h == 0 && qvalid) {
-    const int c = l_cnt[qloc];
-    const size_t slot = (size_t)qq * nsplit + split;
+  if (qvalid) {
+    const size_t slot = ((size_t)qq * nsplit + split) * 2 + h;
     const size_t base = slot * (size_t)kp;
-    for (int e = 0; e < c; ++e) {
-      cand_idx[base + e] = my_id[e];
-      cand_sc[base + e] = my_sc[e];
+    uint32_t c = 0;
+#pragma unroll
+    for (int e = 0; e < KPR; ++e) {
+      if (e < kp && ls[e] > -INFINITY) {
+        cand_idx[base + c] = li[e];
+        cand_sc[base + c] = ls[e];
+        ++c;
+      }
     }
-    cand_cnt[slot] = (uint32_t)c;
-    cand_floor[slot] = (c == kp) ? l_th[qloc] : t0;
+    cand_cnt[slot] = c;
+    cand_floor[slot] = ((int)c == kp) ? th : t0;
   }
 }
 
@@ -766,13 +800,12 @@ CoarseFn coarse_by_j(uint32_t J) {
   return nullptr;
 }
 
-// List capacity and workgroup width go together: 32-entry lists with 4 waves, 64-entry lists
-// with 2 waves (the same LDS footprint).
+// Register-list capacity 32 or 64 (K' <= capacity), 4 waves per workgroup.
 CoarseFn coarse_select(int prec, uint32_t J, int KP) {
-  if (prec == PREC_F32) return KP == 32 ? coarse_by_j<PREC_F32, 32, 4>(J) : coarse_by_j<PREC_F32, 64, 2>(J);
-  if (prec == PREC_F16) return KP == 32 ? coarse_by_j<PREC_F16, 32, 4>(J) : coarse_by_j<PREC_F16, 64, 2>(J);
+  if (prec == PREC_F32) return KP == 32 ? coarse_by_j<PREC_F32, 32, 4>(J) : coarse_by_j<PREC_F32, 64, 4>(J);
+  if (prec == PREC_F16) return KP == 32 ? coarse_by_j<PREC_F16, 32, 4>(J) : coarse_by_j<PREC_F16, 64, 4>(J);
   if (J % 2) return nullptr;
-  return KP == 32 ? coarse_by_j<PREC_F16X3, 32, 4>(J) : coarse_by_j<PREC_F16X3, 64, 2>(J);
+  return KP == 32 ? coarse_by_j<PREC_F16X3, 32, 4>(J) : coarse_by_j<PREC_F16X3, 64, 4>(J);
 }
 
 // FP16 candidate error bound.  x16 = fp16(fp32(x/|x|)): |x16 - x| <= U|x| + eta per component with
@@ -815,10 +848,10 @@ Plan make_plan(const Device& D, uint64_t n, uint64_t d, uint64_t k, const costal
   P.fused = p.select == COSTAL_SELECT_COSINE && d <= dmax && P.kp <= FUSED_KP_MAX;
   P.J = P.prec == PREC_F16 ? P.dp16 / 16 : P.prec == PREC_F16X3 ? P.dp16 / 8 : P.dp32 / 8;
   P.KP = P.kp <= 32 ? 32 : 64;
-  P.NW = P.KP == 32 ? 4 : 2;
+  P.NW = 4;
   P.QB = 32 * P.NW;
-  P.TC = P.J <= 4 ? 128 : 64;
-  P.n_pad = round_up((uint32_t)n, 128);
+  P.TC = P.J <= 2 ? 256 : P.J <= 4 ? 128 : 64;  // the kernel's 16 KB tiles
+  P.n_pad = round_up((uint32_t)n, 256);
   const uint32_t ntile = (uint32_t)((n + P.TC - 1) / P.TC);
   const uint32_t nblk = (uint32_t)((n + P.QB - 1) / P.QB);
   uint32_t ns = p.col_splits;
@@ -884,7 +917,8 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   const bool need16 = P.fused && (P.prec == PREC_F16 || P.prec == PREC_F16X3);
   const int split16 = P.prec == PREC_F16X3 ? 1 : 0;
   const size_t row16 = (size_t)P.dp16 * (split16 ? 2 : 1);
-  const size_t nq_slots = P.fused ? (size_t)n * P.nsplit : 0;
+  // two candidate lists per query and split (one per lane of the query's lane pair)
+  const size_t nq_slots = P.fused ? (size_t)n * P.nsplit * 2 : 0;
 
   size_t bytes = 0;
   {
@@ -948,16 +982,18 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     const int all_admitted = (P.nsplit == 1 && P.kp >= n - 1 && !p.row_threshold) ? 1 : 0;
     // bound on |list score - exact| for the certificate: f32-accurate scores, or split FP16
     const double eps_list = P.prec == PREC_F16X3 ? fp16x3_eps(d) : eps32;
-    const uint32_t slots = P.nsplit * P.kp;
+    const uint32_t nlists = 2 * P.nsplit;
+    const uint32_t slots = nlists * P.kp;
 #define RESCORE(CM)                                                                                    \
-  rescore_kernel<CM><<<n, 64, 0, s>>>(x, d, norms, n, k, (int)P.kp, P.nsplit, nullptr, n, cand_idx,      \
+  rescore_kernel<CM><<<n, 64, 0, s>>>(x, d, norms, n, k, (int)P.kp, nlists, nullptr, n, cand_idx,        \
                                       cand_cnt, cand_floor, all_admitted, eps_list, eps32, idx, cos, wts,  \
                                       fb_rows,                                                          \
                                       fb_lo, counters)
     if (slots <= 64) RESCORE(64);
     else if (slots <= 128) RESCORE(128);
     else if (slots <= 256) RESCORE(256);
-    else RESCORE(512);
+    else if (slots <= 512) RESCORE(512);
+    else RESCORE(1024);
 #undef RESCORE
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(D.evk[4], s));

@@ -36,7 +36,9 @@ FORM_DESC = {
     "fp32": "fp32 MFMA candidates + f64 rescoring",
     "fp16": "fp16 MFMA candidates + f32 re-score + f64 rescoring",
     "fp16x3": "split-fp16 MFMA candidates (hi.hi + hi.lo + lo.hi) + f64 rescoring",
+    "fp16x3+learned": "split-fp16 MFMA candidates + learned BF16 admission thresholds + f64 rescoring",
 }
+PRECISION_OF = {"fp32": "fp32", "fp16": "fp16", "fp16x3": "fp16x3", "fp16x3+learned": "fp16x3"}
 
 
 def parse():
@@ -47,7 +49,8 @@ def parse():
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline duration")
-    ap.add_argument("--forms", default="fp32,fp16,fp16x3", help="forms to time (headline = last)")
+    ap.add_argument("--forms", default="fp32,fp16x3,fp16x3+learned", help="forms to time (headline = last)")
+    ap.add_argument("--train-cells", type=int, default=200_000, help="threshold-model training cells")
     return ap.parse_args()
 
 
@@ -73,14 +76,22 @@ class DeviceRun:
         self.stream = torch.cuda.current_stream().cuda_stream
         torch.cuda.synchronize()
 
-    def step(self, precision: str, threshold=None):
+    def set_model(self, model):
+        self.model = model
+        self.model_params = self.torch.from_numpy(model.params).cuda()
+
+    def step(self, form: str, threshold=None):
         p = self.lib.KnnParams()
         p.k = self.k
         p.margin = -1
         p.select = 0
-        p.precision = self.lib.PRECISION[precision]
+        p.precision = self.lib.PRECISION[PRECISION_OF[form]]
         if threshold is not None:
             p.row_threshold = threshold.data_ptr()
+        if form.endswith("+learned"):
+            p.mlp_params = self.model_params.data_ptr()
+            p.mlp_offset = float(self.model.offset)
+            p.mlp_bf16 = 1
         st = self.lib.KnnStats()
         rc = self.L.costal_knn_exact_device(
             C.c_void_p(self.x.data_ptr()), self.n, self.d, C.byref(p), C.c_void_p(self.idx.data_ptr()),
@@ -156,6 +167,18 @@ def main():
     k = args.k
     run = DeviceRun(x, k)
     forms = args.forms.split(",")
+    model_info = None
+    if any(f.endswith("+learned") for f in forms):
+        from costal_amd import learned
+
+        t0 = time.perf_counter()
+        xt = synth.rare_396460x15(n=args.train_cells, sample_seed=1)
+        model = learned.train(xt, k, n_target=n, seed=0)
+        run.set_model(model)
+        model_info = {"train_cells": args.train_cells, "sample_seed": 1, "k_label": model.meta["k_label"],
+                      "holdout_rmse": model.meta["holdout_rmse"], "offset": model.offset,
+                      "quantile": model.meta["quantile"], "train_seconds": time.perf_counter() - t0,
+                      "arch": f"{d}->64->64->1 relu, fp32 weights, bf16 inference"}
     results = {}
     outs = {}
     for form in forms:
@@ -163,7 +186,7 @@ def main():
         results[form] = {"ms_per_step": ms, "cells_per_s": n / (ms / 1000.0), "kernel_ms": kern,
                          "fallback_rows": int(st.fallback_rows), "short_rows": int(st.short_rows),
                          "col_splits": int(st.col_splits), "candidates": int(st.candidates),
-                         "inserts": int(st.inserts)}
+                         "inserts": int(st.inserts), "rowscan_rows": int(st.rowscan_rows)}
         outs[form] = run.outputs()
     head = forms[-1]
     ref = outs[forms[0]]
@@ -180,7 +203,7 @@ def main():
     flops = 2.0 * n * n * d
     coarse_ms = results[head]["kernel_ms"]["coarse"]
     achieved = flops / (coarse_ms / 1000.0) / 1e12
-    peak = PEAK_TFLOPS[head]
+    peak = PEAK_TFLOPS[PRECISION_OF[head]]
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": None,
                 "kernel": "coarse_kernel", "flops_per_launch": flops, "avg_launch_ms": coarse_ms}
@@ -204,7 +227,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "fp32", "fp16": "fp16", "fp16x3": "fp16"}[head],
+        "dtype": "fp32" if PRECISION_OF[head] == "fp32" else "fp16",
         "data": "synthetic",
         "config": {"workload": "rare_396460x15", "cells": n, "features": d, "k": k,
                    "form": FORM_DESC[head],
@@ -213,6 +236,7 @@ def main():
         "match": match,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "threshold_model": model_info,
         "device": _lib.device_info(),
     }
     print(json.dumps(line))

@@ -86,13 +86,15 @@ def _call(fn, *args):
 
 def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None, matmul=None,
                sort_by_norm=True, prune=True, n_threads=None, row_threshold=None,
-               weights=False, col_splits=None):
+               weights=False, col_splits=None, threshold_model=None, threshold_bf16=True):
     """Exact kNN: ``((idx int32[n,k], cos f64[n,k], norms f64[n]), stats)``.
 
     Extensions beyond the reference signature (keyword-only, default off):
     ``row_threshold`` -- per-row admission thresholds (float32[n]); the result stays exact.
     ``weights=True`` -- also return the fused Tanimoto weights as ``stats["weights"]``.
     ``col_splits`` -- number of candidate-range splits (speed only; 0/None = tuned).
+    ``threshold_model`` -- a :class:`costal_amd.learned.ThresholdModel`; its thresholds are computed
+    on the GPU inside the call (BF16 unless ``threshold_bf16=False``); the result stays exact.
     """
     x = _matrix_f32(x)
     n, d = x.shape
@@ -118,6 +120,14 @@ def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None
         if th.shape != (n,):
             raise ValueError("row_threshold must have one entry per row")
         p.row_threshold = th.ctypes.data
+    mp = None
+    if threshold_model is not None and row_threshold is None:
+        if threshold_model.d != d:
+            raise ValueError(f"threshold model takes {threshold_model.d} features, x has {d}")
+        mp = np.ascontiguousarray(threshold_model.params, dtype=np.float32)
+        p.mlp_params = mp.ctypes.data
+        p.mlp_offset = float(threshold_model.offset)
+        p.mlp_bf16 = int(bool(threshold_bf16))
     idx = np.empty((n, max(k, 0)), np.uint32)
     cos = np.empty((n, max(k, 0)), np.float64)
     norms = np.empty(n, np.float64)
@@ -142,6 +152,7 @@ def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None
         "short_rows": int(st.short_rows),
         "path": "fused" if st.path == 0 else "rowscan",
         "inserts": int(st.inserts),
+        "rowscan_rows": int(st.rowscan_rows),
         "device_ms": float(st.device_ms),
         "kernel_ms": {"prep": st.prep_ms, "coarse": st.coarse_ms, "rescore": st.rescore_ms,
                       "fallback": st.fallback_ms},

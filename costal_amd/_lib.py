@@ -31,6 +31,12 @@ EXPORTS = (
     "costal_last_error",
     "costal_version",
     "costal_device_info",
+    "costal_mlp_param_count",
+    "costal_mlp_act_floats",
+    "costal_mlp_predict_device",
+    "costal_mlp_grad_device",
+    "costal_mlp_adam_device",
+    "costal_mlp_last_error",
 )
 
 
@@ -45,6 +51,9 @@ class KnnParams(C.Structure):
         ("col_splits", C.c_uint32),
         ("reserved", C.c_uint32),
         ("row_threshold", C.c_void_p),
+        ("mlp_params", C.c_void_p),
+        ("mlp_offset", C.c_float),
+        ("mlp_bf16", C.c_uint32),
     ]
 
 
@@ -60,6 +69,7 @@ class KnnStats(C.Structure):
         ("short_rows", C.c_uint64),
         ("path", C.c_uint64),
         ("inserts", C.c_uint64),
+        ("rowscan_rows", C.c_uint64),
         ("device_ms", C.c_double),
         ("prep_ms", C.c_double),
         ("coarse_ms", C.c_double),
@@ -101,6 +111,15 @@ def load():
     L.costal_reweight.argtypes = [U64, U64, P, P, P, U32, P]
     L.costal_reweight_device.argtypes = [U64, U64, P, P, P, U32, P, P]
     L.costal_fuzzy_union.argtypes = [U64, U64, P, P, P, P, P, C.POINTER(C.c_uint64)]
+    F = C.c_float
+    L.costal_mlp_param_count.restype = U64
+    L.costal_mlp_param_count.argtypes = [U64]
+    L.costal_mlp_act_floats.restype = U64
+    L.costal_mlp_act_floats.argtypes = [U64]
+    L.costal_mlp_predict_device.argtypes = [P, U64, U64, P, U32, F, F, U32, P, P]
+    L.costal_mlp_grad_device.argtypes = [P, P, U64, U64, P, P, P, P, P]
+    L.costal_mlp_adam_device.argtypes = [U64, P, P, P, P, F, F, F, F, U64, P]
+    L.costal_mlp_last_error.restype = C.c_char_p
     for name in EXPORTS:
         getattr(L, name).restype = getattr(L, name).restype or C.c_int
     L.costal_last_error.restype = C.c_char_p

@@ -495,7 +495,8 @@ __global__ __launch_bounds__(64) void rescore_kernel(
     const uint32_t* __restrict__ cand_idx, const uint32_t* __restrict__ cand_cnt,
     const float* __restrict__ cand_floor, int all_admitted, double eps_list, double eps32,
     uint32_t* __restrict__ idx, double* __restrict__ cos_out, double* __restrict__ wts,
-    uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo, uint32_t* __restrict__ counters) {
+    uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo, uint32_t* __restrict__ fb_count,
+    uint32_t* __restrict__ short_count) {
   __shared__ double s_sc[CMAX];
   __shared__ uint32_t s_id[CMAX];
   const uint32_t qq = blockIdx.x;
@@ -548,10 +549,10 @@ __global__ __launch_bounds__(64) void rescore_kernel(
       if (wts) wts[(size_t)i * k + t] = tani_from_cos(ni, norms[j], c);
     }
   } else if (threadIdx.x == 0) {
-    const uint32_t p = atomicAdd(&counters[1], 1u);
+    const uint32_t p = atomicAdd(fb_count, 1u);
     fb_rows[p] = i;
     fb_lo[p] = total >= k ? s_sc[k - 1] - eps32 : -INFINITY;
-    if (total < k) atomicAdd(&counters[2], 1u);
+    if (total < k) atomicAdd(short_count, 1u);
   }
 }
 
@@ -932,7 +933,10 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     c.take<float>(nq_slots);
     c.take<uint32_t>(n);
     c.take<double>(n);
+    c.take<uint32_t>(n);
+    c.take<double>(n);
     c.take<uint32_t>(64);
+    if (P.fused && p.mlp_params && !p.row_threshold) c.take<float>(n);
     bytes = c.off + 256;
   }
   int rc = ws_reserve(D, bytes);
@@ -947,13 +951,19 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   float* cand_floor = c.take<float>(nq_slots);
   uint32_t* fb_rows = c.take<uint32_t>(n);
   double* fb_lo = c.take<double>(n);
-  // [0] first bad row, [1] fallback rows, [2] short rows, [4..5] list inserts (u64)
+  uint32_t* fb2_rows = c.take<uint32_t>(n);
+  double* fb2_lo = c.take<double>(n);
+  // [0] first bad row, [1] rows failing the first scan, [2] of them with < k admitted, [3] rows
+  // for the full-row kernel after a threshold-free rescan, [4] short rows of that rescan,
+  // [8..9] candidate-list inserts (u64)
   uint32_t* counters = c.take<uint32_t>(64);
+  const bool learned = P.fused && p.mlp_params && !p.row_threshold;
+  float* th_ws = learned ? c.take<float>(n) : nullptr;
   double* norms = norms_out ? norms_out : norms_ws;
 
   HIP_TRY(hipEventRecord(D.ev0, s));
   {
-    uint32_t init[8] = {NONE, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t init[16] = {NONE, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     HIP_TRY(hipMemcpyAsync(counters, init, sizeof init, hipMemcpyHostToDevice, s));
   }
   HIP_TRY(hipEventRecord(D.evk[0], s));
@@ -972,34 +982,74 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     if (!fn) return fail(COSTAL_E_UNSUPPORTED, "no coarse kernel for d=%u", d);
     const float eps_pre = P.prec == PREC_F16 ? (float)(fp16_eps(d) + eps32 + std::ldexp(1.0, -22)) : 0.0f;
     const uint8_t* xsq = need16 ? (const uint8_t*)xs16 : (const uint8_t*)xs32;
-    dim3 grid((n + P.QB - 1) / P.QB, P.nsplit);
-    HIP_TRY(hipEventRecord(D.evk[2], s));
-    fn<<<grid, 64 * P.NW, 0, s>>>(xsq, xs32, P.dp32, n, (int)P.kp, nullptr, n, p.row_threshold,
-                                  P.cols_per_split, eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor,
-                                  reinterpret_cast<unsigned long long*>(counters + 4));
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(D.evk[3], s));
-    const int all_admitted = (P.nsplit == 1 && P.kp >= n - 1 && !p.row_threshold) ? 1 : 0;
-    // bound on |list score - exact| for the certificate: f32-accurate scores, or split FP16
     const double eps_list = P.prec == PREC_F16X3 ? fp16x3_eps(d) : eps32;
-    const uint32_t nlists = 2 * P.nsplit;
-    const uint32_t slots = nlists * P.kp;
+    const float* row_th = p.row_threshold;
+    if (learned) {
+      // learned thresholds, lowered by twice the list error bound so that a row whose k-th
+      // similarity the model bounds correctly also clears the certificate
+      rc = costal_mlp_predict_device(x, n, d, p.mlp_params, p.mlp_bf16, p.mlp_offset, (float)(2.0 * eps_list), 0,
+                                     th_ws, s);
+      if (rc) return fail(rc, "threshold model: %s", costal_mlp_last_error());
+      row_th = th_ws;
+    }
+    unsigned long long* inserts = reinterpret_cast<unsigned long long*>(counters + 8);
+    const uint32_t ntile = (n + P.TC - 1) / P.TC;
+    // one scan pass over the queries `rows` (null: every row), then rescoring; rows that fail the
+    // certificate are appended to (fbr, fbl) and counted in *fbc
+    auto scan_pass = [&](const uint32_t* rows, uint32_t nq, const float* th, uint32_t nsplit, uint32_t cols_per,
+                         uint32_t* fbr, double* fbl, uint32_t* fbc, uint32_t* shc) -> int {
+      dim3 grid((nq + P.QB - 1) / P.QB, nsplit);
+      fn<<<grid, 64 * P.NW, 0, s>>>(xsq, xs32, P.dp32, n, (int)P.kp, rows, nq, th, cols_per, eps_pre, cand_idx,
+                                    cand_sc, cand_cnt, cand_floor, inserts);
+      HIP_TRY(hipGetLastError());
+      if (!rows) HIP_TRY(hipEventRecord(D.evk[3], s));
+      const int all_admitted = (nsplit == 1 && P.kp >= n - 1 && !th) ? 1 : 0;
+      const uint32_t nlists = 2 * nsplit;
+      const uint32_t slots = nlists * P.kp;
 #define RESCORE(CM)                                                                                    \
-  rescore_kernel<CM><<<n, 64, 0, s>>>(x, d, norms, n, k, (int)P.kp, nlists, nullptr, n, cand_idx,        \
-                                      cand_cnt, cand_floor, all_admitted, eps_list, eps32, idx, cos, wts,  \
-                                      fb_rows,                                                          \
-                                      fb_lo, counters)
-    if (slots <= 64) RESCORE(64);
-    else if (slots <= 128) RESCORE(128);
-    else if (slots <= 256) RESCORE(256);
-    else if (slots <= 512) RESCORE(512);
-    else RESCORE(1024);
+  rescore_kernel<CM><<<nq, 64, 0, s>>>(x, d, norms, n, k, (int)P.kp, nlists, rows, nq, cand_idx, cand_cnt,  \
+                                       cand_floor, all_admitted, eps_list, eps32, idx, cos, wts, fbr, fbl,  \
+                                       fbc, shc)
+      if (slots <= 64) RESCORE(64);
+      else if (slots <= 128) RESCORE(128);
+      else if (slots <= 256) RESCORE(256);
+      else if (slots <= 512) RESCORE(512);
+      else if (slots <= 1024) RESCORE(1024);
+      else RESCORE(2048);
 #undef RESCORE
-    HIP_TRY(hipGetLastError());
+      HIP_TRY(hipGetLastError());
+      return 0;
+    };
+    HIP_TRY(hipEventRecord(D.evk[2], s));
+    rc = scan_pass(nullptr, n, row_th, P.nsplit, P.cols_per_split, fb_rows, fb_lo, counters + 1, counters + 2);
+    if (rc) return rc;
     HIP_TRY(hipEventRecord(D.evk[4], s));
     const uint32_t grid_fb = std::min<uint32_t>(n, 4u * (uint32_t)std::max(D.cus, 1));
-    rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb_rows,
-                        counters + 1, 0, fb_lo, eps32, idx, cos, wts, grid_fb);
+    if (row_th) {
+      // Rows that failed under admission thresholds are rescanned without them, as one batched
+      // scan over the row list (wide candidate splits keep the few rows' scan parallel); only rows
+      // that fail again (exact ties) go to the full-row kernel.
+      uint32_t nfb = 0;
+      HIP_TRY(hipMemcpyAsync(&nfb, counters + 1, sizeof nfb, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      if (nfb > 0) {
+        const uint32_t blocks = (nfb + P.QB - 1) / P.QB;
+        uint32_t ns = (4u * (uint32_t)std::max(D.cus, 1) + blocks - 1) / blocks;
+        ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, ntile / 2));
+        ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, 2048 / (2 * P.kp)));
+        ns = std::min<uint64_t>(ns, std::max<uint64_t>(1, (uint64_t)n * P.nsplit / nfb));
+        ns = std::max<uint32_t>(ns, 1);
+        const uint32_t tiles_per = (ntile + ns - 1) / ns;
+        ns = (ntile + tiles_per - 1) / tiles_per;
+        rc = scan_pass(fb_rows, nfb, nullptr, ns, tiles_per * P.TC, fb2_rows, fb2_lo, counters + 3, counters + 4);
+        if (rc) return rc;
+      }
+      rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb2_rows,
+                          counters + 3, 0, fb2_lo, eps32, idx, cos, wts, grid_fb);
+    } else {
+      rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb_rows,
+                          counters + 1, 0, fb_lo, eps32, idx, cos, wts, grid_fb);
+    }
     if (rc) return rc;
     HIP_TRY(hipEventRecord(D.evk[5], s));
   } else {
@@ -1013,7 +1063,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     HIP_TRY(hipEventRecord(D.evk[5], s));
   }
   HIP_TRY(hipEventRecord(D.ev1, s));
-  uint32_t cnt[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t cnt[16] = {};
   HIP_TRY(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   if (st) {
@@ -1028,7 +1078,8 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     st->candidates = P.kp;
     st->col_splits = P.fused ? P.nsplit : 1;
     st->path = P.fused ? 0 : 1;
-    st->inserts = (uint64_t)cnt[4] | ((uint64_t)cnt[5] << 32);
+    st->inserts = (uint64_t)cnt[8] | ((uint64_t)cnt[9] << 32);
+    st->rowscan_rows = P.fused ? (p.row_threshold || learned ? cnt[3] : cnt[1]) : n;
     st->device_ms = ms;
     auto span = [&](int a, int b) {
       float v = 0.0f;
@@ -1093,6 +1144,7 @@ static int knn_host(const float* x, uint64_t n, uint64_t d, const costal_knn_par
   uint32_t* didx = nullptr;
   double *dcos = nullptr, *dnorms = nullptr, *dw = nullptr;
   float* dth = nullptr;
+  float* dmlp = nullptr;
   auto cleanup = [&]() {
     (void)hipFree(dx);
     (void)hipFree(didx);
@@ -1100,6 +1152,7 @@ static int knn_host(const float* x, uint64_t n, uint64_t d, const costal_knn_par
     (void)hipFree(dnorms);
     (void)hipFree(dw);
     (void)hipFree(dth);
+    (void)hipFree(dmlp);
   };
 #define HT(expr)                                                                      \
   do {                                                                                \
@@ -1115,6 +1168,12 @@ static int knn_host(const float* x, uint64_t n, uint64_t d, const costal_knn_par
   HT(hipMalloc(&dnorms, n * sizeof(double)));
   if (weights) HT(hipMalloc(&dw, n * k * sizeof(double)));
   costal_knn_params p = *params;
+  if (params->mlp_params && !params->row_threshold) {
+    const size_t cnt = costal_mlp_param_count(d);
+    HT(hipMalloc(&dmlp, cnt * sizeof(float)));
+    HT(hipMemcpyAsync(dmlp, params->mlp_params, cnt * sizeof(float), hipMemcpyHostToDevice, s));
+    p.mlp_params = dmlp;
+  }
   if (params->row_threshold) {
     HT(hipMalloc(&dth, n * sizeof(float)));
     HT(hipMemcpyAsync(dth, params->row_threshold, n * sizeof(float), hipMemcpyHostToDevice, s));

@@ -1,0 +1,279 @@
+// mlp.hip -- the learned admission threshold: a 3-layer MLP d -> 64 -> 64 -> 1 on L2-normalized
+// rows, trained on GPU by its own forward/backward/Adam kernels, run in BF16 (or FP32) to propose a
+// per-row admission threshold for the fused kNN scan.
+//
+// Parameter layout (one flat f32 array, P = 64 d + 64 + 64*64 + 64 + 64 + 1 entries):
+//   W1[64][d] (row-major), b1[64], W2[64][64] (row-major, out x in), b2[64], w3[64], b3
+// Model: xn = f32(x / |x|) with |x| the f64 sequential norm (0 for zero rows),
+//        h1 = relu(W1 xn + b1), h2 = relu(W2 h1 + b2), y = w3 . h2 + b3.
+// Loss: mean over the batch of (y - target)^2.
+//
+// Everything here is deterministic: each parameter's gradient is one thread's fixed-order sum over
+// the batch rows (no atomics), so a training run repeats bit for bit.
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "../../include/costal_hip.h"
+
+namespace mlp {
+
+constexpr int H = COSTAL_MLP_HIDDEN;
+constexpr int DMAX = COSTAL_MLP_MAX_FEATURES;
+
+__host__ __device__ constexpr uint64_t param_count(uint64_t d) { return H * d + H + H * H + H + H + 1; }
+
+// round-to-nearest-even to bfloat16, kept in an f32
+__device__ __forceinline__ float bf16r(float v) {
+  uint32_t u = __float_as_uint(v);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return __uint_as_float(u & 0xffff0000u);
+}
+
+// Row features: the row scaled by its f64 sequential norm, rounded to f32 (the same working copy
+// the kNN scan uses); zero rows give zeros.
+__device__ __forceinline__ void load_features(const float* __restrict__ row, uint32_t d, float (&xn)[DMAX]) {
+  double s = 0.0;
+  for (uint32_t t = 0; t < d; ++t) s += (double)row[t] * (double)row[t];
+  const double nrm = sqrt(s);
+#pragma unroll
+  for (int t = 0; t < DMAX; ++t) xn[t] = ((uint32_t)t < d && nrm > 0.0) ? (float)((double)row[t] / nrm) : 0.0f;
+}
+
+struct Smem {
+  float W1[H * DMAX];
+  float b1[H];
+  float W2[H * H];
+  float b2[H];
+  float w3[H];
+  float b3;
+};
+
+template <bool BF16>
+__device__ void load_params(Smem& S, const float* __restrict__ P, uint32_t d) {
+  const float* W1 = P;
+  const float* b1 = W1 + H * d;
+  const float* W2 = b1 + H;
+  const float* b2 = W2 + H * H;
+  const float* w3 = b2 + H;
+  const float* b3 = w3 + H;
+  for (int e = threadIdx.x; e < H * DMAX; e += blockDim.x) {
+    const int j = e / DMAX, t = e % DMAX;
+    const float v = (uint32_t)t < d ? W1[j * d + t] : 0.0f;
+    S.W1[e] = BF16 ? bf16r(v) : v;
+  }
+  for (int e = threadIdx.x; e < H * H; e += blockDim.x) S.W2[e] = BF16 ? bf16r(W2[e]) : W2[e];
+  for (int e = threadIdx.x; e < H; e += blockDim.x) {
+    S.b1[e] = b1[e];
+    S.b2[e] = b2[e];
+    S.w3[e] = BF16 ? bf16r(w3[e]) : w3[e];
+  }
+  if (threadIdx.x == 0) S.b3 = b3[0];
+}
+
+// Forward for one row in registers.  BF16: inputs, weights and hidden activations rounded to bf16,
+// products accumulated in f32 (fmaf).
+template <bool BF16>
+__device__ __forceinline__ float forward_row(const Smem& S, float (&xn)[DMAX], float (&h1)[H], float (&h2)[H]) {
+  if (BF16) {
+#pragma unroll
+    for (int t = 0; t < DMAX; ++t) xn[t] = bf16r(xn[t]);
+  }
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    float a = S.b1[j];
+#pragma unroll
+    for (int t = 0; t < DMAX; ++t) a = fmaf(S.W1[j * DMAX + t], xn[t], a);
+    a = fmaxf(a, 0.0f);
+    h1[j] = BF16 ? bf16r(a) : a;
+  }
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    float a = S.b2[j];
+#pragma unroll
+    for (int g = 0; g < H; ++g) a = fmaf(S.W2[j * H + g], h1[g], a);
+    a = fmaxf(a, 0.0f);
+    h2[j] = BF16 ? bf16r(a) : a;
+  }
+  float y = S.b3;
+#pragma unroll
+  for (int g = 0; g < H; ++g) y = fmaf(S.w3[g], h2[g], y);
+  return y;
+}
+
+// Inference: out[r] = transform(y_r) where the model predicts u = log(1 - s + 1e-6) of the k-th
+// neighbor similarity s; the threshold is 1 - exp(u + offset) - margin (offset and margin from the
+// calibration), or y - offset when raw != 0.
+template <bool BF16>
+__global__ __launch_bounds__(256) void predict_kernel(const float* __restrict__ x, uint32_t n, uint32_t d,
+                                                      const float* __restrict__ P, float offset, float margin,
+                                                      int raw, float* __restrict__ out) {
+  __shared__ Smem S;
+  load_params<BF16>(S, P, d);
+  __syncthreads();
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  float xn[DMAX], h1[H], h2[H];
+  load_features(x + (size_t)r * d, d, xn);
+  const float y = forward_row<BF16>(S, xn, h1, h2);
+  out[r] = raw ? y - offset : 1.0f - expf(y + offset) - margin;
+}
+
+// Training forward + backward of the activations for one batch; per row it stores
+// act = [xn (DMAX), h1 (H), h2 (H), dy, dh2 (H), dh1 (H), sq_err].
+constexpr int ACT = DMAX + 5 * H + 2;
+
+__global__ __launch_bounds__(256) void fwd_bwd_kernel(const float* __restrict__ x, const float* __restrict__ tgt,
+                                                      uint32_t n, uint32_t d, const float* __restrict__ P,
+                                                      float* __restrict__ act) {
+  __shared__ Smem S;
+  load_params<false>(S, P, d);
+  __syncthreads();
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  float xn[DMAX], h1[H], h2[H];
+  load_features(x + (size_t)r * d, d, xn);
+  const float y = forward_row<false>(S, xn, h1, h2);
+  const float err = y - tgt[r];
+  const float dy = 2.0f * err / (float)n;
+  float* a = act + (size_t)r * ACT;
+#pragma unroll
+  for (int t = 0; t < DMAX; ++t) a[t] = xn[t];
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    a[DMAX + j] = h1[j];
+    a[DMAX + H + j] = h2[j];
+  }
+  a[DMAX + 2 * H] = dy;
+  float dh2[H];
+#pragma unroll
+  for (int g = 0; g < H; ++g) {
+    dh2[g] = h2[g] > 0.0f ? S.w3[g] * dy : 0.0f;
+    a[DMAX + 2 * H + 1 + g] = dh2[g];
+  }
+#pragma unroll
+  for (int g = 0; g < H; ++g) {
+    float s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < H; ++j) s = fmaf(S.W2[j * H + g], dh2[j], s);
+    a[DMAX + 3 * H + 1 + g] = h1[g] > 0.0f ? s : 0.0f;
+  }
+  a[DMAX + 4 * H + 1] = err * err;
+}
+
+// Parameter gradients: thread p sums its parameter's contribution over the rows in order; thread P
+// writes the batch loss (mean squared error).
+__global__ __launch_bounds__(256) void wgrad_kernel(const float* __restrict__ act, uint32_t n, uint32_t d,
+                                                    float* __restrict__ grad, float* __restrict__ loss) {
+  const uint32_t P = (uint32_t)param_count(d);
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > P) return;
+  // offsets inside a row of act
+  const int XN = 0, H1 = DMAX, H2 = DMAX + H, DY = DMAX + 2 * H, DH2 = DY + 1, DH1 = DH2 + H, SQ = DH1 + H;
+  int ia, ib;  // grad = sum_r act[r][ia] * (ib < 0 ? 1 : act[r][ib])
+  if (p == P) {
+    ia = SQ;
+    ib = -1;
+  } else if (p < H * d) {
+    ia = DH1 + (int)(p / d);
+    ib = XN + (int)(p % d);
+  } else if (p < H * d + H) {
+    ia = DH1 + (int)(p - H * d);
+    ib = -1;
+  } else if (p < H * d + H + H * H) {
+    const uint32_t q = p - (H * d + H);
+    ia = DH2 + (int)(q / H);
+    ib = H1 + (int)(q % H);
+  } else if (p < H * d + 2 * H + H * H) {
+    ia = DH2 + (int)(p - (H * d + H + H * H));
+    ib = -1;
+  } else if (p < H * d + 3 * H + H * H) {
+    ia = DY;
+    ib = H2 + (int)(p - (H * d + 2 * H + H * H));
+  } else {
+    ia = DY;
+    ib = -1;
+  }
+  float s = 0.0f;
+  for (uint32_t r = 0; r < n; ++r) {
+    const float* a = act + (size_t)r * ACT;
+    s = fmaf(a[ia], ib < 0 ? 1.0f : a[ib], s);
+  }
+  if (p == P) {
+    if (loss) loss[0] = s / (float)n;
+  } else {
+    grad[p] = s;
+  }
+}
+
+__global__ void adam_kernel(uint32_t count, float* __restrict__ P, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  P[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+thread_local char g_err[256];
+
+int fail(int code, const char* msg, const char* detail = "") {
+  snprintf(g_err, sizeof g_err, "%s%s", msg, detail);
+  return code;
+}
+
+}  // namespace mlp
+
+extern "C" {
+
+const char* costal_mlp_last_error(void) { return mlp::g_err; }
+
+uint64_t costal_mlp_param_count(uint64_t d) { return mlp::param_count(d); }
+
+uint64_t costal_mlp_act_floats(uint64_t n) { return n * (uint64_t)mlp::ACT; }
+
+int costal_mlp_predict_device(const float* x, uint64_t n, uint64_t d, const float* params, uint32_t bf16,
+                              float offset, float margin, uint32_t raw, float* out, void* stream) {
+  if (d == 0 || d > COSTAL_MLP_MAX_FEATURES) return mlp::fail(COSTAL_E_UNSUPPORTED, "the threshold model takes 1..64 features");
+  if (n == 0) return 0;
+  if (n > 0xffffffffull) return mlp::fail(COSTAL_E_TOO_MANY_ROWS, "too many rows");
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  if (bf16)
+    mlp::predict_kernel<true><<<grid, 256, 0, s>>>(x, (uint32_t)n, (uint32_t)d, params, offset, margin, (int)raw, out);
+  else
+    mlp::predict_kernel<false><<<grid, 256, 0, s>>>(x, (uint32_t)n, (uint32_t)d, params, offset, margin, (int)raw, out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return mlp::fail(COSTAL_E_DEVICE, "predict launch failed: ", hipGetErrorString(e));
+  return 0;
+}
+
+int costal_mlp_grad_device(const float* x, const float* target, uint64_t n, uint64_t d, const float* params,
+                           float* act, float* grad, float* loss, void* stream) {
+  if (d == 0 || d > COSTAL_MLP_MAX_FEATURES) return mlp::fail(COSTAL_E_UNSUPPORTED, "the threshold model takes 1..64 features");
+  if (n == 0 || n > 0xffffffffull) return mlp::fail(COSTAL_E_INVALID, "bad batch size");
+  hipStream_t s = (hipStream_t)stream;
+  mlp::fwd_bwd_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(x, target, (uint32_t)n, (uint32_t)d, params, act);
+  const uint64_t P = mlp::param_count(d);
+  mlp::wgrad_kernel<<<(unsigned)((P + 1 + 255) / 256), 256, 0, s>>>(act, (uint32_t)n, (uint32_t)d, grad, loss);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return mlp::fail(COSTAL_E_DEVICE, "gradient launch failed: ", hipGetErrorString(e));
+  return 0;
+}
+
+int costal_mlp_adam_device(uint64_t count, float* params, const float* grad, float* m, float* v, float lr,
+                           float beta1, float beta2, float eps, uint64_t step, void* stream) {
+  if (step == 0) return mlp::fail(COSTAL_E_INVALID, "Adam steps count from 1");
+  const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
+  mlp::adam_kernel<<<(unsigned)((count + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+      (uint32_t)count, params, grad, m, v, lr, beta1, beta2, eps, c1, c2);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return mlp::fail(COSTAL_E_DEVICE, "adam launch failed: ", hipGetErrorString(e));
+  return 0;
+}
+
+}  // extern "C"

@@ -30,16 +30,19 @@ import numpy as np
 COUNTS_OFFSET = 1.1726  # Phi(o / sqrt(5)) = 0.70
 
 
-def _mixture(rng, n, d, n_comp, *, mean_sd=20.0, within_sd=5.0, weights=None, counts=None):
+def _mixture(rng, n, d, n_comp, *, mean_sd=20.0, within_sd=5.0, weights=None, counts=None,
+             sample_rng=None):
+    """Mixture parameters from ``rng``; cells from ``sample_rng`` (default: the same stream)."""
     means = rng.normal(0.0, mean_sd, size=(n_comp, d))
+    srng = rng if sample_rng is None else sample_rng
     if counts is None:
         if weights is None:
             weights = rng.dirichlet(np.ones(n_comp))
-        comp = rng.choice(n_comp, size=n, p=weights)
+        comp = srng.choice(n_comp, size=n, p=weights)
     else:
         comp = np.repeat(np.arange(n_comp), counts)
-        rng.shuffle(comp)
-    x = means[comp] + rng.normal(0.0, within_sd, size=(n, d))
+        srng.shuffle(comp)
+    x = means[comp] + srng.normal(0.0, within_sd, size=(n, d))
     return x, comp
 
 
@@ -61,17 +64,22 @@ def cytometry_86864x39(seed: int = 1, n: int = 86_864, d: int = 39) -> np.ndarra
     return arcsinh(x)
 
 
-def rare_396460x15(seed: int = 2, n: int = 396_460, d: int = 15, dup_frac: float = 0.005) -> np.ndarray:
+def rare_396460x15(seed: int = 2, n: int = 396_460, d: int = 15, dup_frac: float = 0.005,
+                   sample_seed: int = 0) -> np.ndarray:
+    """The mixture (weights, means) comes from ``seed``; the cells, duplicate rows included, from
+    ``(seed, sample_seed)``, so another ``sample_seed`` draws other cells from the same mixture
+    (the threshold model's training data)."""
     rng = np.random.default_rng(seed)
+    srng = np.random.default_rng([seed, sample_seed])
     n_rare = int(round(0.0005 * n))
     n_main = n - 3 * n_rare
     w = rng.dirichlet(np.ones(20))
-    main = rng.multinomial(n_main, w)
+    main = srng.multinomial(n_main, w)
     counts = np.concatenate([main, [n_rare] * 3])
-    x, _ = _mixture(rng, n, d, 23, counts=counts)
+    x, _ = _mixture(rng, n, d, 23, counts=counts, sample_rng=srng)
     x = arcsinh(x)
     n_dup = int(round(dup_frac * n))
-    perm = rng.permutation(n)
+    perm = srng.permutation(n)
     dst, src = perm[:n_dup], perm[n_dup:2 * n_dup]
     x[dst] = x[src]
     return x

@@ -78,11 +78,18 @@ typedef struct costal_knn_params {
    * whose k-th rescored similarity is within the error bound of the threshold, are recomputed by
    * the plain scan, so the result stays exact whatever the thresholds are.  NULL = none. */
   const float* row_threshold;
+  /* Optional learned thresholds (used when row_threshold is NULL): parameters of the threshold
+   * model (costal_mlp_*; device memory for the device entry point, host memory otherwise),
+   * evaluated on the GPU inside the call.  Row threshold = 1 - exp(model(row) + mlp_offset) minus
+   * twice the candidate error bound.  The result stays exact whatever the model says. */
+  const float* mlp_params;
+  float mlp_offset;
+  uint32_t mlp_bf16;     /* 1 = BF16 inference (products of bf16-rounded values, f32 sums) */
 } costal_knn_params;
 
 /* KnnStats (knn.rs:344-357) plus GPU-side counters */
 typedef struct costal_knn_stats {
-  uint64_t fallback_rows;     /* rows recomputed by the full-row path */
+  uint64_t fallback_rows;     /* rows whose first scan failed the certificate and were recomputed */
   uint64_t row_block;         /* query rows per workgroup used */
   uint64_t col_block;         /* candidate columns per LDS tile used */
   uint64_t blocks_skipped;    /* always 0 for cosine selection */
@@ -92,6 +99,7 @@ typedef struct costal_knn_stats {
   uint64_t short_rows;        /* rows with fewer than k admitted candidates (thresholds only) */
   uint64_t path;              /* 0 fused MFMA scan, 1 exact row scan (wide d, large K', tanimoto) */
   uint64_t inserts;           /* candidate-list inserts made by the fused scan */
+  uint64_t rowscan_rows;      /* rows that needed the exact full-row kernel */
   double device_ms;           /* device time of the call (hipEvent), 0 when not measured */
   double prep_ms;             /* norms + working copies kernel (hipEvent) */
   double coarse_ms;           /* fused MFMA tile + scan kernel (hipEvent) */
@@ -126,6 +134,29 @@ int costal_reweight_device(uint64_t n, uint64_t k, const uint32_t* idx, const do
  * w must hold n*k entries; the edge count is written to *m. */
 int costal_fuzzy_union(uint64_t n, uint64_t k, const uint32_t* idx, const double* w,
                        uint32_t* rows, uint32_t* cols, double* wout, uint64_t* m);
+
+/* ---- learned admission threshold ----------------------------------------------------------
+ * No counterpart in the reference: the per-row threshold model of the fused scan.  A 3-layer MLP
+ * d -> 64 -> 64 -> 1 (ReLU) on L2-normalized rows predicting u = log(1 - s_k + 1e-6) of the row's
+ * k-th neighbor cosine s_k.  Parameters are one flat f32 array:
+ *   W1[64][d], b1[64], W2[64][64], b2[64], w3[64], b3   (costal_mlp_param_count(d) floats).
+ * All pointers are device pointers; stream as above.  Training is deterministic. */
+#define COSTAL_MLP_HIDDEN 64
+#define COSTAL_MLP_MAX_FEATURES 64
+uint64_t costal_mlp_param_count(uint64_t d);
+/* floats of activation scratch a gradient call needs for n rows */
+uint64_t costal_mlp_act_floats(uint64_t n);
+/* out[r] = 1 - exp(model(x_r) + offset) - margin, or model(x_r) - offset when raw != 0 */
+int costal_mlp_predict_device(const float* x, uint64_t n, uint64_t d, const float* params, uint32_t bf16,
+                              float offset, float margin, uint32_t raw, float* out, void* stream);
+/* gradient of the mean squared error of model(x) against target over the n rows of a batch;
+ * act holds costal_mlp_act_floats(n) floats; loss (1 float) may be NULL */
+int costal_mlp_grad_device(const float* x, const float* target, uint64_t n, uint64_t d,
+                           const float* params, float* act, float* grad, float* loss, void* stream);
+/* one Adam step (step counts from 1) */
+int costal_mlp_adam_device(uint64_t count, float* params, const float* grad, float* m, float* v,
+                           float lr, float beta1, float beta2, float eps, uint64_t step, void* stream);
+const char* costal_mlp_last_error(void);
 
 /* Thread-local message of the last failing call on this thread. */
 const char* costal_last_error(void);

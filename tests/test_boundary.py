@@ -18,7 +18,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def header_functions():
     text = open(os.path.join(ROOT, "include", "costal_hip.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(costal_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|uint64_t|const char\*)\s+(costal_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():

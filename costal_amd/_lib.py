@@ -37,6 +37,7 @@ EXPORTS = (
     "costal_mlp_grad_device",
     "costal_mlp_adam_device",
     "costal_mlp_last_error",
+    "costal_knn_batched_device",
 )
 
 
@@ -78,6 +79,21 @@ class KnnStats(C.Structure):
     ]
 
 
+class Sample(C.Structure):
+    _fields_ = [
+        ("x", C.c_void_p),
+        ("n", C.c_uint64),
+        ("d", C.c_uint64),
+        ("idx", C.c_void_p),
+        ("cos", C.c_void_p),
+        ("norms", C.c_void_p),
+        ("weights", C.c_void_p),
+        ("margin", C.c_int64),
+        ("col_splits", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
 class CostalError(ValueError):
     """An error returned by the library; ``code`` is one of the E_* values."""
 
@@ -108,6 +124,7 @@ def load():
     L.costal_knn_exact_device.argtypes = [P, U64, U64, C.POINTER(KnnParams), P, P, P, P,
                                           C.POINTER(KnnStats), P]
     L.costal_knn_bruteforce.argtypes = [P, U64, U64, U64, U32, P, P, P]
+    L.costal_knn_batched_device.argtypes = [C.POINTER(Sample), U32, C.POINTER(KnnParams), C.POINTER(KnnStats), P]
     L.costal_reweight.argtypes = [U64, U64, P, P, P, U32, P]
     L.costal_reweight_device.argtypes = [U64, U64, P, P, P, U32, P, P]
     L.costal_fuzzy_union.argtypes = [U64, U64, P, P, P, P, P, C.POINTER(C.c_uint64)]

@@ -39,6 +39,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -259,13 +260,23 @@ __device__ __forceinline__ f32x16 tile_mfma(const uint8_t* __restrict__ T, int s
   return acc;
 }
 
+template <int J>
+struct TileShape {
+  static constexpr int TC = J <= 2 ? 256 : J <= 4 ? 128 : 64;  // candidates per tile: 16 KB tiles
+  static constexpr int TILEB = TC * 32 * J;
+};
+
+// One work item of the scan: query block `qblock` (QB queries) against candidate split `split` of
+// one sample.  tile0/tile1 are the workgroup's two LDS tile buffers.
 template <int PREC, int J, int KPR, int NW>
-__global__ __launch_bounds__(64 * NW) void coarse_kernel(
+__device__ __forceinline__ void scan_block(
+    uint8_t* __restrict__ tile0, uint8_t* __restrict__ tile1,
     const uint8_t* __restrict__ xsq,   // n_pad rows of 32*J bytes (the working copy of this form)
     const float* __restrict__ xs32,    // f32 working copy (dp32 per row), for the FP16 re-score
     uint32_t dp32, uint32_t n, int kp,
     const uint32_t* __restrict__ rowlist, uint32_t nq,  // queries: rowlist[0..nq) or rows 0..nq-1
-    const float* __restrict__ tinit,   // per-row admission threshold (by row) or null
+    uint32_t qblock, const float* __restrict__ tinit,   // per-row admission threshold (by row) or null
+    uint32_t split, uint32_t nsplit,
     uint32_t cols_per_split,           // candidate columns per split, a multiple of the tile
     float eps_pre,                     // FP16 form: bound on |FP16 score - f32 score|
     uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc, uint32_t* __restrict__ cand_cnt,
@@ -283,15 +294,13 @@ __global__ __launch_bounds__(64 * NW) void coarse_kernel(
   constexpr int GS = SUBT < GMAX ? SUBT : GMAX;
   static_assert(SUBT % GS == 0, "sub-tile grouping");
 
-  __shared__ __attribute__((aligned(16))) uint8_t tile0[TILEB];
-  __shared__ __attribute__((aligned(16))) uint8_t tile1[TILEB];
+  static_assert(TILEB == TileShape<J>::TILEB, "tile shape");
 
   const int tid = threadIdx.x;
   const int w = tid >> 6, l = tid & 63, h = l >> 5, l32 = l & 31;
-  const uint32_t qq = blockIdx.x * QB + w * 32 + l32;
+  const uint32_t qq = qblock * QB + w * 32 + l32;
   const bool qvalid = qq < nq;
   const uint32_t qg = qvalid ? (rowlist ? rowlist[qq] : qq) : 0;
-  const uint32_t split = blockIdx.y, nsplit = gridDim.y;
 
   // the wave's queries, in registers for the whole launch
   uint4 qreg[J];
@@ -450,6 +459,18 @@ __global__ __launch_bounds__(64 * NW) void coarse_kernel(
   }
 }
 
+template <int PREC, int J, int KPR, int NW>
+__global__ __launch_bounds__(64 * NW) void coarse_kernel(
+    const uint8_t* __restrict__ xsq, const float* __restrict__ xs32, uint32_t dp32, uint32_t n, int kp,
+    const uint32_t* __restrict__ rowlist, uint32_t nq, const float* __restrict__ tinit, uint32_t cols_per_split,
+    float eps_pre, uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc, uint32_t* __restrict__ cand_cnt,
+    float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
+  __shared__ __attribute__((aligned(16))) uint8_t tile0[TileShape<J>::TILEB];
+  __shared__ __attribute__((aligned(16))) uint8_t tile1[TileShape<J>::TILEB];
+  scan_block<PREC, J, KPR, NW>(tile0, tile1, xsq, xs32, dp32, n, kp, rowlist, nq, blockIdx.x, tinit, blockIdx.y,
+                               gridDim.y, cols_per_split, eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor, inserts);
+}
+
 // ------------------------------------------------------------------------------------------------
 // Sorting helpers: bitonic sort of P (power of two) entries in LDS into "ranks_before" order,
 // by NT threads of one workgroup.
@@ -488,20 +509,16 @@ __device__ void bitonic_lds(double* __restrict__ sc, double* __restrict__ cs, ui
 // each candidate in f64 over the original rows, sorts, applies the certificate and writes the
 // top k (or queues the row for the full-row path).
 
+// One query's rescoring (64 threads).  Failed rows are recorded as fb_tag in fb_rows.
 template <int CMAX>
-__global__ __launch_bounds__(64) void rescore_kernel(
-    const float* __restrict__ x, uint32_t d, const double* __restrict__ norms, uint32_t n,
-    uint32_t k, int kp, uint32_t nsplit, const uint32_t* __restrict__ rowlist, uint32_t nq,
-    const uint32_t* __restrict__ cand_idx, const uint32_t* __restrict__ cand_cnt,
-    const float* __restrict__ cand_floor, int all_admitted, double eps_list, double eps32,
-    uint32_t* __restrict__ idx, double* __restrict__ cos_out, double* __restrict__ wts,
-    uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo, uint32_t* __restrict__ fb_count,
-    uint32_t* __restrict__ short_count) {
-  __shared__ double s_sc[CMAX];
-  __shared__ uint32_t s_id[CMAX];
-  const uint32_t qq = blockIdx.x;
-  if (qq >= nq) return;
-  const uint32_t i = rowlist ? rowlist[qq] : qq;
+__device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_id, const float* __restrict__ x,
+                            uint32_t d, const double* __restrict__ norms, uint32_t k, int kp, uint32_t nsplit,
+                            uint32_t qq, uint32_t i, const uint32_t* __restrict__ cand_idx,
+                            const uint32_t* __restrict__ cand_cnt, const float* __restrict__ cand_floor,
+                            int all_admitted, double eps_list, double eps32, uint32_t* __restrict__ idx,
+                            double* __restrict__ cos_out, double* __restrict__ wts, uint32_t fb_tag,
+                            uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo,
+                            uint32_t* __restrict__ fb_count, uint32_t* __restrict__ short_count) {
   const double ni = norms[i];
   const float* xi = x + (size_t)i * d;
   const int tot_slots = (int)nsplit * kp;
@@ -550,10 +567,28 @@ __global__ __launch_bounds__(64) void rescore_kernel(
     }
   } else if (threadIdx.x == 0) {
     const uint32_t p = atomicAdd(fb_count, 1u);
-    fb_rows[p] = i;
+    fb_rows[p] = fb_tag;
     fb_lo[p] = total >= k ? s_sc[k - 1] - eps32 : -INFINITY;
     if (total < k) atomicAdd(short_count, 1u);
   }
+}
+
+template <int CMAX>
+__global__ __launch_bounds__(64) void rescore_kernel(
+    const float* __restrict__ x, uint32_t d, const double* __restrict__ norms, uint32_t n,
+    uint32_t k, int kp, uint32_t nsplit, const uint32_t* __restrict__ rowlist, uint32_t nq,
+    const uint32_t* __restrict__ cand_idx, const uint32_t* __restrict__ cand_cnt,
+    const float* __restrict__ cand_floor, int all_admitted, double eps_list, double eps32,
+    uint32_t* __restrict__ idx, double* __restrict__ cos_out, double* __restrict__ wts,
+    uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo, uint32_t* __restrict__ fb_count,
+    uint32_t* __restrict__ short_count) {
+  __shared__ double s_sc[CMAX];
+  __shared__ uint32_t s_id[CMAX];
+  const uint32_t qq = blockIdx.x;
+  if (qq >= nq) return;
+  const uint32_t i = rowlist ? rowlist[qq] : qq;
+  rescore_row<CMAX>(s_sc, s_id, x, d, norms, k, kp, nsplit, qq, i, cand_idx, cand_cnt, cand_floor, all_admitted,
+                    eps_list, eps32, idx, cos_out, wts, i, fb_rows, fb_lo, fb_count, short_count);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -566,6 +601,95 @@ __global__ __launch_bounds__(64) void rescore_kernel(
 //   COARSE = false: every candidate is scored exactly (knn_bruteforce_select, knn.rs:884-924),
 //                   for either selection metric.
 
+template <int CAP>
+struct RowscanSmem {
+  double sc[CAP];
+  double cs[CAP];
+  uint32_t id[CAP];
+  int L, B, full;
+  double lo, ksc;
+  uint32_t kid;
+};
+
+// The exact top-k of row i (256 threads); lo0 is the initial survivor bound (-inf: none).
+template <int CAP, bool COARSE>
+__device__ void rowscan_row(RowscanSmem<CAP>& S, const float* __restrict__ x, uint32_t d,
+                            const float* __restrict__ xs32, uint32_t dp32, const double* __restrict__ norms,
+                            uint32_t n, uint32_t k, uint32_t select, uint32_t i, double lo0, double eps32,
+                            uint32_t* __restrict__ idx, double* __restrict__ cos_out, double* __restrict__ wts) {
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    S.L = 0;
+    S.B = 0;
+    S.full = 0;
+    S.lo = lo0;
+    S.ksc = -INFINITY;
+    S.kid = NONE;
+  }
+  __syncthreads();
+  const double ni = norms[i];
+  const float* xi = x + (size_t)i * d;
+
+  auto merge = [&]() {
+    const int tot = S.L + S.B;
+    int P = 2;
+    while (P < tot) P <<= 1;
+    for (int e = tot + tid; e < P; e += 256) {
+      S.sc[e] = -INFINITY;
+      S.cs[e] = 0.0;
+      S.id[e] = NONE;
+    }
+    __syncthreads();
+    bitonic_lds<256>(S.sc, S.cs, S.id, P);
+    if (tid == 0) {
+      S.L = tot < (int)k ? tot : (int)k;
+      S.B = 0;
+      if (S.L == (int)k) {
+        S.full = 1;
+        S.ksc = S.sc[k - 1];
+        S.kid = S.id[k - 1];
+        if (COARSE) S.lo = fmax(S.lo, S.ksc - eps32);
+      }
+    }
+    __syncthreads();
+  };
+
+  for (uint32_t base = 0; base < n; base += 256) {
+    const uint32_t j = base + tid;
+    if (j < n && j != i) {
+      bool pass = true;
+      if (COARSE) {
+        float c32 = dot32(xs32 + (size_t)i * dp32, xs32 + (size_t)j * dp32, dp32);
+        pass = (double)c32 >= S.lo;
+      }
+      if (pass) {
+        const double nj = norms[j];
+        const double c = cos64(dot64(xi, x + (size_t)j * d, d), ni, nj);
+        const double sc = select == COSTAL_SELECT_TANIMOTO ? tani_from_cos(ni, nj, c) : c;
+        if (!S.full || ranks_before(sc, j, S.ksc, S.kid)) {
+          const int p = atomicAdd(&S.B, 1);
+          const int slot = S.L + p;
+          S.sc[slot] = sc;
+          S.cs[slot] = c;
+          S.id[slot] = j;
+        }
+      }
+    }
+    __syncthreads();
+    const bool full_buffer = S.L + S.B > CAP - 256;
+    __syncthreads();
+    if (full_buffer) merge();
+  }
+  merge();
+  for (uint32_t t = tid; t < k; t += 256) {
+    const uint32_t j = S.id[t];
+    idx[(size_t)i * k + t] = j;
+    cos_out[(size_t)i * k + t] = S.cs[t];
+    if (wts) wts[(size_t)i * k + t] = j < n ? tani_from_cos(ni, norms[j], S.cs[t]) : 0.0;
+  }
+  __syncthreads();
+}
+
 template <int CAP, bool COARSE>
 __global__ __launch_bounds__(256) void rowscan_kernel(
     const float* __restrict__ x, uint32_t d, const float* __restrict__ xs32, uint32_t dp32,
@@ -573,87 +697,154 @@ __global__ __launch_bounds__(256) void rowscan_kernel(
     const uint32_t* __restrict__ rows, const uint32_t* __restrict__ nrows_dev, uint32_t nrows_host,
     const double* __restrict__ lo_arr, double eps32, uint32_t* __restrict__ idx,
     double* __restrict__ cos_out, double* __restrict__ wts) {
-  __shared__ double s_sc[CAP];
-  __shared__ double s_cs[CAP];
-  __shared__ uint32_t s_id[CAP];
-  __shared__ int s_L, s_B, s_full;
-  __shared__ double s_lo, s_ksc;
-  __shared__ uint32_t s_kid;
+  __shared__ RowscanSmem<CAP> S;
   const uint32_t nrows = nrows_dev ? *nrows_dev : nrows_host;
-  const int tid = threadIdx.x;
-
   for (uint32_t r = blockIdx.x; r < nrows; r += gridDim.x) {
     const uint32_t i = rows ? rows[r] : r;
-    if (tid == 0) {
-      s_L = 0;
-      s_B = 0;
-      s_full = 0;
-      s_lo = lo_arr ? lo_arr[r] : -INFINITY;
-      s_ksc = -INFINITY;
-      s_kid = NONE;
-    }
-    __syncthreads();
-    const double ni = norms[i];
-    const float* xi = x + (size_t)i * d;
+    rowscan_row<CAP, COARSE>(S, x, d, xs32, dp32, norms, n, k, select, i, lo_arr ? lo_arr[r] : -INFINITY, eps32,
+                             idx, cos_out, wts);
+  }
+}
 
-    auto merge = [&]() {
-      const int tot = s_L + s_B;
-      int P = 2;
-      while (P < tot) P <<= 1;
-      for (int e = tot + tid; e < P; e += 256) {
-        s_sc[e] = -INFINITY;
-        s_cs[e] = 0.0;
-        s_id[e] = NONE;
-      }
-      __syncthreads();
-      bitonic_lds<256>(s_sc, s_cs, s_id, P);
-      if (tid == 0) {
-        s_L = tot < (int)k ? tot : (int)k;
-        s_B = 0;
-        if (s_L == (int)k) {
-          s_full = 1;
-          s_ksc = s_sc[k - 1];
-          s_kid = s_id[k - 1];
-          if (COARSE) s_lo = fmax(s_lo, s_ksc - eps32);
-        }
-      }
-      __syncthreads();
-    };
+// ------------------------------------------------------------------------------------------------
+// Batched samples: many independent kNN problems (uneven n, d) in one pass of each kernel.  The
+// working copies of all samples are stacked (each sample padded to 256 rows, all rows padded to the
+// batch's widest feature count, zeros being exact), and one persistent scan kernel pulls
+// (sample, query block, candidate split) work items from a queue ordered by decreasing cost.
 
-    for (uint32_t base = 0; base < n; base += 256) {
-      const uint32_t j = base + tid;
-      if (j < n && j != i) {
-        bool pass = true;
-        if (COARSE) {
-          float c32 = dot32(xs32 + (size_t)i * dp32, xs32 + (size_t)j * dp32, dp32);
-          pass = (double)c32 >= s_lo;
-        }
-        if (pass) {
-          const double nj = norms[j];
-          const double c = cos64(dot64(xi, x + (size_t)j * d, d), ni, nj);
-          const double sc = select == COSTAL_SELECT_TANIMOTO ? tani_from_cos(ni, nj, c) : c;
-          if (!s_full || ranks_before(sc, j, s_ksc, s_kid)) {
-            const int p = atomicAdd(&s_B, 1);
-            const int slot = s_L + p;
-            s_sc[slot] = sc;
-            s_cs[slot] = c;
-            s_id[slot] = j;
-          }
-        }
-      }
-      __syncthreads();
-      const bool full_buffer = s_L + s_B > CAP - 256;
-      __syncthreads();
-      if (full_buffer) merge();
+struct BatchSample {
+  const float* x;        // the sample's rows (device), n x d
+  uint32_t n, d;
+  uint32_t row_pad;      // first row of the sample in the stacked working copies
+  uint32_t row_base;     // first row of the sample in the concatenated row numbering
+  int kp;                // candidate-list length K'
+  uint32_t nsplit, cols_per_split;
+  uint64_t slot_base;    // first candidate list (slot) of the sample
+  uint64_t entry_base;   // first list entry of the sample (its lists hold kp entries each)
+  uint32_t* idx;
+  double* cos;
+  double* norms;
+  double* wts;
+};
+
+// the sample a row belongs to, by binary search in an ascending table of first rows
+__device__ __forceinline__ uint32_t find_sample(const uint32_t* __restrict__ first, uint32_t nsamples, uint32_t g) {
+  uint32_t lo = 0, hi = nsamples;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (first[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Graph-safe counter reset (no memset nodes): counters[0..1] = first bad (sample, row) as u64
+// (all ones = none), everything else zero.
+__global__ void batch_init_kernel(uint32_t* __restrict__ counters, uint32_t count) {
+  for (uint32_t e = threadIdx.x; e < count; e += blockDim.x) counters[e] = e < 2 ? NONE : 0u;
+}
+
+__global__ void prep_batch_kernel(const BatchSample* __restrict__ samples, const uint32_t* __restrict__ pad_first,
+                                  uint32_t nsamples, uint32_t total_pad, uint32_t dp32, uint32_t dp16, int split16,
+                                  float* __restrict__ xs32, _Float16* __restrict__ xs16,
+                                  unsigned long long* __restrict__ bad) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= total_pad) return;
+  const uint32_t sidx = find_sample(pad_first, nsamples, g);
+  const BatchSample& S = samples[sidx];
+  const uint32_t r = g - S.row_pad;
+  const uint32_t s16 = split16 ? 2 * dp16 : dp16;
+  const uint32_t dmax = dp32 > dp16 ? dp32 : dp16;
+  if (r >= S.n) {
+    for (uint32_t t = 0; t < dp32; ++t) xs32[(size_t)g * dp32 + t] = 0.0f;
+    if (xs16)
+      for (uint32_t t = 0; t < s16; ++t) xs16[(size_t)g * s16 + t] = (_Float16)0.0f;
+    return;
+  }
+  const uint32_t d = S.d;
+  const uint32_t* row_bits = reinterpret_cast<const uint32_t*>(S.x + (size_t)r * d);
+  double sum = 0.0;
+  bool badrow = false;
+  for (uint32_t t = 0; t < d; ++t) {
+    const uint32_t b = row_bits[t];
+    badrow |= (b & 0x7f800000u) == 0x7f800000u;
+    const float v = __uint_as_float(b);
+    sum += (double)v * (double)v;
+  }
+  const double nrm = sqrt(sum);
+  S.norms[r] = nrm;
+  if (badrow) atomicMin(bad, ((unsigned long long)sidx << 32) | r);
+  for (uint32_t t = 0; t < dmax; ++t) {
+    const float v = (t < d && nrm > 0.0) ? (float)((double)__uint_as_float(row_bits[t]) / nrm) : 0.0f;
+    if (t < dp32) xs32[(size_t)g * dp32 + t] = v;
+    if (xs16 && t < dp16) {
+      const _Float16 hi = (_Float16)v;
+      xs16[(size_t)g * s16 + t] = hi;
+      if (split16) xs16[(size_t)g * s16 + dp16 + t] = (_Float16)(v - (float)hi);
     }
-    merge();
-    for (uint32_t t = tid; t < k; t += 256) {
-      const uint32_t j = s_id[t];
-      idx[(size_t)i * k + t] = j;
-      cos_out[(size_t)i * k + t] = s_cs[t];
-      if (wts) wts[(size_t)i * k + t] = j < n ? tani_from_cos(ni, norms[j], s_cs[t]) : 0.0;
-    }
+  }
+}
+
+// items[3e..3e+2] = (sample, query block, split), in decreasing cost order
+template <int PREC, int J, int KPR, int NW>
+__global__ __launch_bounds__(64 * NW) void coarse_batch_kernel(
+    const BatchSample* __restrict__ samples, const uint32_t* __restrict__ items, uint32_t nitems,
+    uint32_t* __restrict__ queue, const uint8_t* __restrict__ xsq, const float* __restrict__ xs32, uint32_t dp32,
+    float eps_pre, uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc, uint32_t* __restrict__ cand_cnt,
+    float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
+  __shared__ __attribute__((aligned(16))) uint8_t tile0[TileShape<J>::TILEB];
+  __shared__ __attribute__((aligned(16))) uint8_t tile1[TileShape<J>::TILEB];
+  __shared__ uint32_t s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
     __syncthreads();
+    const uint32_t it = s_item;
+    __syncthreads();
+    if (it >= nitems) break;
+    const uint32_t sidx = items[3 * it], qb = items[3 * it + 1], sp = items[3 * it + 2];
+    const BatchSample& S = samples[sidx];
+    const size_t rowb = (size_t)S.row_pad;
+    scan_block<PREC, J, KPR, NW>(tile0, tile1, xsq + rowb * 32 * J, xs32 + rowb * dp32, dp32, S.n, S.kp, nullptr,
+                                 S.n, qb, nullptr, sp, S.nsplit, S.cols_per_split, eps_pre,
+                                 cand_idx + S.entry_base, cand_sc + S.entry_base, cand_cnt + S.slot_base,
+                                 cand_floor + S.slot_base, inserts);
+    __syncthreads();
+  }
+}
+
+template <int CMAX>
+__global__ __launch_bounds__(64) void rescore_batch_kernel(
+    const BatchSample* __restrict__ samples, const uint32_t* __restrict__ row_first, uint32_t nsamples,
+    uint32_t total_rows, uint32_t k, double eps_list, double eps32, const uint32_t* __restrict__ cand_idx,
+    const uint32_t* __restrict__ cand_cnt, const float* __restrict__ cand_floor, uint32_t* __restrict__ fb_rows,
+    double* __restrict__ fb_lo, uint32_t* __restrict__ fb_count, uint32_t* __restrict__ short_count) {
+  __shared__ double s_sc[CMAX];
+  __shared__ uint32_t s_id[CMAX];
+  const uint32_t g = blockIdx.x;
+  if (g >= total_rows) return;
+  const uint32_t sidx = find_sample(row_first, nsamples, g);
+  const BatchSample& S = samples[sidx];
+  const uint32_t i = g - S.row_base;
+  const uint32_t nlists = 2 * S.nsplit;
+  const int all_admitted = (S.nsplit == 1 && (uint32_t)S.kp >= S.n - 1) ? 1 : 0;
+  rescore_row<CMAX>(s_sc, s_id, S.x, S.d, S.norms, k, S.kp, nlists, i, i, cand_idx + S.entry_base,
+                    cand_cnt + S.slot_base, cand_floor + S.slot_base, all_admitted, eps_list, eps32, S.idx, S.cos,
+                    S.wts, g, fb_rows, fb_lo, fb_count, short_count);
+}
+
+template <int CAP>
+__global__ __launch_bounds__(256) void rowscan_batch_kernel(
+    const BatchSample* __restrict__ samples, const uint32_t* __restrict__ row_first, uint32_t nsamples,
+    const float* __restrict__ xs32, uint32_t dp32, uint32_t k, const uint32_t* __restrict__ rows,
+    const uint32_t* __restrict__ nrows_dev, const double* __restrict__ lo_arr, double eps32) {
+  __shared__ RowscanSmem<CAP> Sm;
+  const uint32_t nrows = *nrows_dev;
+  for (uint32_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+    const uint32_t g = rows[r];
+    const uint32_t sidx = find_sample(row_first, nsamples, g);
+    const BatchSample& S = samples[sidx];
+    rowscan_row<CAP, true>(Sm, S.x, S.d, xs32 + (size_t)S.row_pad * dp32, dp32, S.norms, S.n, k,
+                           COSTAL_SELECT_COSINE, g - S.row_base, lo_arr[r], eps32, S.idx, S.cos, S.wts);
   }
 }
 
@@ -1095,6 +1286,306 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
 
 hipStream_t pick_stream(Device& D, void* stream) { return stream ? (hipStream_t)stream : D.stream; }
 
+// ------------------------------------------------------------------------------------------------
+// Batched samples (host side)
+
+typedef void (*CoarseBatchFn)(const BatchSample*, const uint32_t*, uint32_t, uint32_t*, const uint8_t*,
+                              const float*, uint32_t, float, uint32_t*, float*, uint32_t*, float*,
+                              unsigned long long*);
+
+template <int PREC, int KP>
+CoarseBatchFn coarse_batch_by_j(uint32_t J) {
+  switch (J) {
+    case 1: return coarse_batch_kernel<PREC, 1, KP, 4>;
+    case 2: return coarse_batch_kernel<PREC, 2, KP, 4>;
+    case 3: return coarse_batch_kernel<PREC, 3, KP, 4>;
+    case 4: return coarse_batch_kernel<PREC, 4, KP, 4>;
+    case 5: return coarse_batch_kernel<PREC, 5, KP, 4>;
+    case 6: return coarse_batch_kernel<PREC, 6, KP, 4>;
+    case 7: return coarse_batch_kernel<PREC, 7, KP, 4>;
+    case 8: return coarse_batch_kernel<PREC, 8, KP, 4>;
+  }
+  return nullptr;
+}
+
+CoarseBatchFn coarse_batch_select(int prec, uint32_t J, int KP) {
+  if (prec == PREC_F32) return KP == 32 ? coarse_batch_by_j<PREC_F32, 32>(J) : coarse_batch_by_j<PREC_F32, 64>(J);
+  if (prec == PREC_F16) return KP == 32 ? coarse_batch_by_j<PREC_F16, 32>(J) : coarse_batch_by_j<PREC_F16, 64>(J);
+  if (J % 2) return nullptr;
+  return KP == 32 ? coarse_batch_by_j<PREC_F16X3, 32>(J) : coarse_batch_by_j<PREC_F16X3, 64>(J);
+}
+
+struct BatchGraph {
+  std::vector<uint8_t> key;  // samples + params the graph was captured for
+  void* ws = nullptr;        // the batch path's own workspace (the graph points into it)
+  size_t ws_cap = 0;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t stream = nullptr;
+  bool use_graph = true;
+  bool ready = false;
+  std::function<hipError_t(hipStream_t)> launch;  // the batch's launch sequence
+  // host copies of what the stats need
+  uint32_t QB = 0, TC = 0, max_split = 0, max_kp = 0;
+  uint64_t blocks_total = 0;
+  size_t off_counters = 0;
+};
+BatchGraph g_batch;
+
+int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const costal_knn_params& p,
+                       costal_knn_stats* st, hipStream_t s) {
+  const uint64_t k = p.k;
+  if (p.select != COSTAL_SELECT_COSINE) return fail(COSTAL_E_UNSUPPORTED, "batched samples use cosine selection");
+  if (p.precision > COSTAL_PREC_FP16X3) return fail(COSTAL_E_INVALID, "unknown precision %u", p.precision);
+  if (p.row_threshold || p.mlp_params) return fail(COSTAL_E_UNSUPPORTED, "batched samples take no admission thresholds");
+  uint32_t dmax = 1;
+  uint64_t total_rows = 0;
+  for (uint32_t i = 0; i < S; ++i) {
+    const costal_sample& a = smp[i];
+    if (!a.x || !a.idx || !a.cos || !a.norms) return fail(COSTAL_E_INVALID, "sample %u: null pointer", i);
+    if (a.n > 0xffffffffull) return fail(COSTAL_E_TOO_MANY_ROWS, "sample %u: n=%llu exceeds the u32 index range", i,
+                                         (unsigned long long)a.n);
+    if (k == 0 || k >= a.n)
+      return fail(COSTAL_E_BAD_K, "sample %u: invalid k=%llu: need 1 <= k <= n-1 (n=%llu)", i, (unsigned long long)k,
+                  (unsigned long long)a.n);
+    if (a.d == 0 || a.d > 64) return fail(COSTAL_E_UNSUPPORTED, "sample %u: batched samples need 1 <= d <= 64", i);
+    dmax = std::max<uint32_t>(dmax, (uint32_t)a.d);
+    total_rows += a.n;
+  }
+  if (total_rows > 0xffffffffull) return fail(COSTAL_E_TOO_MANY_ROWS, "batch exceeds the u32 row range");
+  // reuse the captured graph when the batch layout is unchanged
+  std::vector<uint8_t> key(sizeof(costal_sample) * S + sizeof(costal_knn_params) + sizeof(void*));
+  memcpy(key.data(), smp, sizeof(costal_sample) * S);
+  memcpy(key.data() + sizeof(costal_sample) * S, &p, sizeof p);
+  memcpy(key.data() + sizeof(costal_sample) * S + sizeof p, &s, sizeof(void*));
+
+  const int prec = (int)p.precision;
+  const uint32_t dp32 = round_up(dmax, 8), dp16 = round_up(dmax, 16);
+  const uint32_t J = prec == PREC_F16 ? dp16 / 16 : prec == PREC_F16X3 ? dp16 / 8 : dp32 / 8;
+  const uint32_t TC = J <= 2 ? 256 : J <= 4 ? 128 : 64;
+  const uint32_t QB = 128;
+  const bool need16 = prec != PREC_F32;
+  const int split16 = prec == PREC_F16X3 ? 1 : 0;
+  const size_t row16 = (size_t)dp16 * (split16 ? 2 : 1);
+
+  if (g_batch.ready && g_batch.key == key) {
+    // replay
+  } else {
+    if (g_batch.exec) {
+      (void)hipGraphExecDestroy(g_batch.exec);
+      g_batch.exec = nullptr;
+    }
+    g_batch.ready = false;
+    std::vector<BatchSample> bs(S);
+    std::vector<uint32_t> pad_first(S), row_first(S), items;
+    uint32_t pad = 0, base = 0, max_split = 1, max_kp = 1;
+    uint64_t slots = 0, entries = 0, blocks_total = 0;
+    const uint32_t target_cols = 8192;  // candidates per work item: items of similar cost
+    for (uint32_t i = 0; i < S; ++i) {
+      const costal_sample& a = smp[i];
+      const uint32_t n = (uint32_t)a.n;
+      const int64_t mg = a.margin >= 0 ? a.margin : (p.margin >= 0 ? p.margin : (int64_t)k);
+      const uint32_t kp = (uint32_t)std::min<uint64_t>(k + (uint64_t)mg, n - 1);
+      if (kp > FUSED_KP_MAX) return fail(COSTAL_E_UNSUPPORTED, "sample %u: k + margin above %u", i, FUSED_KP_MAX);
+      const uint32_t ntile = (n + TC - 1) / TC;
+      uint32_t ns = a.col_splits ? a.col_splits : (n + target_cols - 1) / target_cols;
+      ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, ntile));
+      ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, 2048 / (2 * kp)));
+      const uint32_t tiles_per = (ntile + ns - 1) / ns;
+      ns = (ntile + tiles_per - 1) / tiles_per;
+      BatchSample& b = bs[i];
+      b.x = a.x;
+      b.n = n;
+      b.d = (uint32_t)a.d;
+      b.row_pad = pad;
+      b.row_base = base;
+      b.kp = (int)kp;
+      b.nsplit = ns;
+      b.cols_per_split = tiles_per * TC;
+      b.slot_base = slots;
+      b.entry_base = entries;
+      b.idx = a.idx;
+      b.cos = a.cos;
+      b.norms = a.norms;
+      b.wts = a.weights;
+      pad_first[i] = pad;
+      row_first[i] = base;
+      const uint32_t nqb = (n + QB - 1) / QB;
+      for (uint32_t qb = 0; qb < nqb; ++qb)
+        for (uint32_t sp = 0; sp < ns; ++sp) {
+          items.push_back(i);
+          items.push_back(qb);
+          items.push_back(sp);
+        }
+      blocks_total += (uint64_t)nqb * ntile;
+      pad += round_up(n, 256);
+      base += n;
+      slots += (uint64_t)n * ns * 2;
+      entries += (uint64_t)n * ns * 2 * kp;
+      max_split = std::max(max_split, ns);
+      max_kp = std::max(max_kp, kp);
+    }
+    // largest work first (an item's cost is its candidate count)
+    const uint32_t nitems = (uint32_t)(items.size() / 3);
+    std::vector<uint32_t> order(nitems);
+    for (uint32_t e = 0; e < nitems; ++e) order[e] = e;
+    auto cost = [&](uint32_t e) {
+      const BatchSample& b = bs[items[3 * e]];
+      const uint32_t c0 = items[3 * e + 2] * b.cols_per_split;
+      return std::min<uint32_t>(b.cols_per_split, b.n > c0 ? b.n - c0 : 0);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
+    std::vector<uint32_t> sorted(items.size());
+    for (uint32_t e = 0; e < nitems; ++e)
+      for (int c = 0; c < 3; ++c) sorted[3 * e + c] = items[3 * order[e] + c];
+
+    const int KP = max_kp <= 32 ? 32 : 64;
+    CoarseBatchFn fn = coarse_batch_select(prec, J, KP);
+    if (!fn) return fail(COSTAL_E_UNSUPPORTED, "no batched scan kernel for d=%u", dmax);
+
+    size_t bytes;
+    {
+      Carve c(nullptr);
+      c.take<float>((size_t)pad * dp32);
+      if (need16) c.take<_Float16>((size_t)pad * row16);
+      c.take<uint32_t>(entries);
+      c.take<float>(entries);
+      c.take<uint32_t>(slots);
+      c.take<float>(slots);
+      c.take<uint32_t>(total_rows);
+      c.take<double>(total_rows);
+      c.take<uint32_t>(64);
+      c.take<BatchSample>(S);
+      c.take<uint32_t>(S);
+      c.take<uint32_t>(S);
+      c.take<uint32_t>(items.size());
+      bytes = c.off + 256;
+    }
+    if (bytes > g_batch.ws_cap) {
+      if (g_batch.ws) {
+        HIP_TRY(hipStreamSynchronize(s));
+        HIP_TRY(hipFree(g_batch.ws));
+        g_batch.ws = nullptr;
+        g_batch.ws_cap = 0;
+      }
+      const size_t cap = bytes + bytes / 8 + (1u << 20);
+      HIP_TRY(hipMalloc(&g_batch.ws, cap));
+      g_batch.ws_cap = cap;
+    }
+    Carve c(g_batch.ws);
+    float* xs32 = c.take<float>((size_t)pad * dp32);
+    _Float16* xs16 = need16 ? c.take<_Float16>((size_t)pad * row16) : nullptr;
+    uint32_t* cand_idx = c.take<uint32_t>(entries);
+    float* cand_sc = c.take<float>(entries);
+    uint32_t* cand_cnt = c.take<uint32_t>(slots);
+    float* cand_floor = c.take<float>(slots);
+    uint32_t* fb_rows = c.take<uint32_t>(total_rows);
+    double* fb_lo = c.take<double>(total_rows);
+    uint32_t* counters = c.take<uint32_t>(64);
+    const size_t off_counters = (size_t)(reinterpret_cast<uint8_t*>(counters) - static_cast<uint8_t*>(g_batch.ws));
+    BatchSample* d_samples = c.take<BatchSample>(S);
+    uint32_t* d_pad_first = c.take<uint32_t>(S);
+    uint32_t* d_row_first = c.take<uint32_t>(S);
+    uint32_t* d_items = c.take<uint32_t>(items.size());
+    HIP_TRY(hipMemcpyAsync(d_samples, bs.data(), sizeof(BatchSample) * S, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_pad_first, pad_first.data(), sizeof(uint32_t) * S, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_row_first, row_first.data(), sizeof(uint32_t) * S, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_items, sorted.data(), sizeof(uint32_t) * sorted.size(), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+
+    const double eps32 = fp32_eps(dmax);
+    const double eps_list = prec == PREC_F16X3 ? fp16x3_eps(dmax) : eps32;
+    const float eps_pre = prec == PREC_F16 ? (float)(fp16_eps(dmax) + eps32 + std::ldexp(1.0, -22)) : 0.0f;
+    const uint8_t* xsq = need16 ? (const uint8_t*)xs16 : (const uint8_t*)xs32;
+    const uint32_t max_slots = 2 * max_split * max_kp;
+    const uint32_t grid_scan = 2u * (uint32_t)std::max(D.cus, 1);
+    const uint32_t grid_fb = 4u * (uint32_t)std::max(D.cus, 1);
+
+    // the whole batch as one sequence of launches (no host round trip, no memset nodes)
+    g_batch.launch = [=](hipStream_t st) {
+      batch_init_kernel<<<1, 64, 0, st>>>(counters, 64);
+      prep_batch_kernel<<<(pad + 255) / 256, 256, 0, st>>>(d_samples, d_pad_first, S, pad, dp32, dp16, split16, xs32,
+                                                           xs16, reinterpret_cast<unsigned long long*>(counters));
+      fn<<<grid_scan, 256, 0, st>>>(d_samples, d_items, nitems, counters + 6, xsq, xs32, dp32, eps_pre, cand_idx,
+                                    cand_sc, cand_cnt, cand_floor, reinterpret_cast<unsigned long long*>(counters + 4));
+#define RESCORE_B(CM)                                                                                             \
+  rescore_batch_kernel<CM><<<(uint32_t)total_rows, 64, 0, st>>>(d_samples, d_row_first, S, (uint32_t)total_rows, \
+                                                                (uint32_t)k, eps_list, eps32, cand_idx, cand_cnt, \
+                                                                cand_floor, fb_rows, fb_lo, counters + 2,          \
+                                                                counters + 3)
+      if (max_slots <= 64) RESCORE_B(64);
+      else if (max_slots <= 128) RESCORE_B(128);
+      else if (max_slots <= 256) RESCORE_B(256);
+      else if (max_slots <= 512) RESCORE_B(512);
+      else if (max_slots <= 1024) RESCORE_B(1024);
+      else RESCORE_B(2048);
+#undef RESCORE_B
+      if (k <= 512)
+        rowscan_batch_kernel<1024><<<grid_fb, 256, 0, st>>>(d_samples, d_row_first, S, xs32, dp32, (uint32_t)k,
+                                                            fb_rows, counters + 2, fb_lo, eps32);
+      else
+        rowscan_batch_kernel<4096><<<grid_fb, 256, 0, st>>>(d_samples, d_row_first, S, xs32, dp32, (uint32_t)k,
+                                                            fb_rows, counters + 2, fb_lo, eps32);
+      return hipGetLastError();
+    };
+    g_batch.use_graph = getenv("COSTAL_NO_GRAPH") == nullptr;
+    if (g_batch.use_graph) {
+      hipGraph_t graph = nullptr;
+      HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      hipError_t le = g_batch.launch(s);
+      hipError_t ce = hipStreamEndCapture(s, &graph);
+      if (le != hipSuccess) return fail(COSTAL_E_DEVICE, "batched launch failed: %s", hipGetErrorString(le));
+      if (ce != hipSuccess) return fail(COSTAL_E_DEVICE, "graph capture failed: %s", hipGetErrorString(ce));
+      hipError_t ie = hipGraphInstantiate(&g_batch.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        g_batch.exec = nullptr;
+        return fail(COSTAL_E_DEVICE, "graph instantiation failed: %s", hipGetErrorString(ie));
+      }
+    }
+    g_batch.key = key;
+    g_batch.ready = true;
+    g_batch.stream = s;
+    g_batch.QB = QB;
+    g_batch.TC = TC;
+    g_batch.max_split = max_split;
+    g_batch.max_kp = max_kp;
+    g_batch.blocks_total = blocks_total;
+    g_batch.off_counters = off_counters;
+  }
+
+  uint32_t* counters = reinterpret_cast<uint32_t*>(static_cast<uint8_t*>(g_batch.ws) + g_batch.off_counters);
+  HIP_TRY(hipEventRecord(D.ev0, s));
+  if (g_batch.use_graph) {
+    HIP_TRY(hipGraphLaunch(g_batch.exec, s));
+  } else {
+    HIP_TRY(g_batch.launch(s));
+  }
+  HIP_TRY(hipEventRecord(D.ev1, s));
+  uint32_t cnt[16] = {};
+  HIP_TRY(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (cnt[0] != NONE || cnt[1] != NONE)
+    return fail(COSTAL_E_NON_FINITE, "sample %u: row %u contains NaN or inf", cnt[1], cnt[0]);
+  if (st) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, D.ev0, D.ev1) != hipSuccess) ms = 0.0f;
+    memset(st, 0, sizeof *st);
+    st->fallback_rows = cnt[2];
+    st->short_rows = cnt[3];
+    st->rowscan_rows = cnt[2];
+    st->row_block = g_batch.QB;
+    st->col_block = g_batch.TC;
+    st->blocks_total = g_batch.blocks_total;
+    st->candidates = g_batch.max_kp;
+    st->col_splits = g_batch.max_split;
+    st->path = 2;
+    st->inserts = (uint64_t)cnt[4] | ((uint64_t)cnt[5] << 32);
+    st->device_ms = ms;
+    st->coarse_ms = ms;
+  }
+  return 0;
+}
+
 }  // namespace
 
 // ==================================================================================================
@@ -1200,6 +1691,16 @@ int costal_knn_exact(const float* x, uint64_t n, uint64_t d, const costal_knn_pa
                      uint32_t* idx, double* cos, double* norms, double* weights,
                      costal_knn_stats* stats) {
   return knn_host(x, n, d, params, idx, cos, norms, weights, stats, false);
+}
+
+int costal_knn_batched_device(const costal_sample* samples, uint32_t nsamples, const costal_knn_params* params,
+                              costal_knn_stats* stats, void* stream) {
+  if (!params || (!samples && nsamples)) return fail(COSTAL_E_INVALID, "null pointer argument");
+  if (nsamples == 0) return 0;
+  std::lock_guard<std::mutex> lk(g_dev.mu);
+  int rc = device_init(g_dev);
+  if (rc) return rc;
+  return knn_batched_locked(g_dev, samples, nsamples, *params, stats, pick_stream(g_dev, stream));
 }
 
 int costal_knn_bruteforce(const float* x, uint64_t n, uint64_t d, uint64_t k, uint32_t select,

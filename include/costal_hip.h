@@ -118,6 +118,28 @@ int costal_knn_exact_device(const float* x, uint64_t n, uint64_t d,
                             const costal_knn_params* params, uint32_t* idx, double* cos,
                             double* norms, double* weights, costal_knn_stats* stats, void* stream);
 
+/* ---- batched samples -----------------------------------------------------------------------
+ * No single counterpart in the reference (it runs one knn() per sample, knn.rs:360-379): many
+ * independent samples of uneven size in one pass, each with exactly the result costal_knn_exact
+ * gives for it alone.  One persistent scan launch pulls (sample, query block, candidate split)
+ * work items from a queue; the whole call is captured once as a hipGraph and replayed while the
+ * batch layout (pointers, sizes, parameters) stays the same.  Cosine selection, d <= 64. */
+typedef struct costal_sample {
+  const float* x;        /* device, n x d row-major */
+  uint64_t n, d;
+  uint32_t* idx;         /* device outputs: n*k, n*k, n, n*k (weights may be NULL) */
+  double* cos;
+  double* norms;
+  double* weights;
+  int64_t margin;        /* candidate list min(k + margin, n - 1); negative = params->margin */
+  uint32_t col_splits;   /* candidate-range splits of this sample; 0 = planned */
+  uint32_t reserved;
+} costal_sample;
+
+/* params->k, ->precision and ->margin apply to every sample (select must be cosine). */
+int costal_knn_batched_device(const costal_sample* samples, uint32_t nsamples,
+                              const costal_knn_params* params, costal_knn_stats* stats, void* stream);
+
 /* Brute force by the definition: every j != i scored in f64, ranked by (score desc, index asc).
  * Replaces knn_bruteforce_select (knn.rs:884-924). */
 int costal_knn_bruteforce(const float* x, uint64_t n, uint64_t d, uint64_t k, uint32_t select,

@@ -1,19 +1,24 @@
-"""Benchmark: exact k=10 cosine kNN + Tanimoto weights on one MI355X, in cells/s.
+"""Benchmark: exact cosine kNN + Tanimoto weights on one MI355X, in cells/s.
 
-Workload: 396,460 cells x 15 features (arcsinh Gaussian mixture, 20 components plus 3 rare ones,
-0.5% exact duplicate rows), k = 10; synthetic, seed-pinned (costal_amd.synth.rare_396460x15).
-One step = one complete drop-in call on inputs already resident in HBM: norms, fused MFMA tiles
-with per-row candidate lists, f64 rescoring with the exactness certificate, full-row fallback and
-fused Tanimoto weights, ending with idx/cos/norms/weights in HBM.
+Default workload (the headline): 396,460 cells x 15 features (arcsinh Gaussian mixture, 20
+components plus 3 rare ones, 0.5% exact duplicate rows), k = 10; synthetic, seed-pinned
+(costal_amd.synth.rare_396460x15).  One step = one complete drop-in call on inputs already
+resident in HBM: norms, fused MFMA tiles with per-row candidate lists, f64 rescoring with the
+exactness certificate, fallbacks and fused Tanimoto weights, ending with idx/cos/norms/weights in
+HBM.
 
-The headline ``value`` is the mixed-precision form (FP16 MFMA candidates re-scored in f32, then
-f64); the plain FP32 MFMA form is timed the same way and reported beside it, with a check that
-both give identical neighbors and weights.  ``roofline`` is for the dominant kernel (the fused
-tile + scan kernel), timed live with HIP events on the stream it runs on.  ``cpu_baseline`` times
-the CPU oracle's blocked form (a C restatement of the reference's run_blocked, OpenMP) on a
-bounded sample of query rows of the same input, on this host's cores.
+Forms timed side by side: plain FP32 MFMA tiles; split-FP16 MFMA tiles (hi.hi + hi.lo + lo.hi);
+split-FP16 tiles with learned BF16 admission thresholds (the headline ``value``; the threshold
+model is trained before timing on 200,000 other cells of the same mixture).  The line reports that
+all forms give identical neighbors and weights.  ``roofline`` is for the dominant kernel (the fused
+tile + scan kernel), its time measured live with HIP events on the stream it runs on.
+``cpu_baseline`` times the CPU oracle's blocked form (a C restatement of the reference's run_blocked,
+OpenMP) on a bounded sample of query rows of the same input, on this host's cores.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--no-cpu]
+Other workloads (--workload): cytometry_86864x39, learned_1000000x40 (k=30), batched_64 (64 uneven
+samples in one batched hipGraph call, k=15), counts_3605x2000 (wide rows: the exact row-scan path).
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--workload NAME] [--no-cpu]
 """
 
 from __future__ import annotations
@@ -31,7 +36,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 PEAK_TFLOPS = {"fp32": 157.3, "fp16": 2500.0, "fp16x3": 2500.0}  # MI355X dense MFMA peaks
-PEAK_HBM_GBS = 8000.0
+METRIC = "cells/s exact k=10 cosine kNN+Tanimoto; mixed-prec vs FP32 GPU at 100% match"
 FORM_DESC = {
     "fp32": "fp32 MFMA candidates + f64 rescoring",
     "fp16": "fp16 MFMA candidates + f32 re-score + f64 rescoring",
@@ -39,6 +44,14 @@ FORM_DESC = {
     "fp16x3+learned": "split-fp16 MFMA candidates + learned BF16 admission thresholds + f64 rescoring",
 }
 PRECISION_OF = {"fp32": "fp32", "fp16": "fp16", "fp16x3": "fp16x3", "fp16x3+learned": "fp16x3"}
+WORKLOADS = {
+    # name: (k, forms, train the threshold model on this many other cells)
+    "rare_396460x15": (10, "fp32,fp16x3,fp16x3+learned", 200_000),
+    "cytometry_86864x39": (10, "fp32,fp16,fp16x3", 0),
+    "learned_1000000x40": (30, "fp32,fp16x3,fp16x3+learned", 200_000),
+    "batched_64": (15, "fp32,fp16x3", 0),
+    "counts_3605x2000": (10, "fp32", 0),
+}
 
 
 def parse():
@@ -46,12 +59,29 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--k", type=int, default=10)
+    ap.add_argument("--workload", default="rare_396460x15", choices=sorted(WORKLOADS))
+    ap.add_argument("--k", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline duration")
-    ap.add_argument("--forms", default="fp32,fp16x3,fp16x3+learned", help="forms to time (headline = last)")
-    ap.add_argument("--train-cells", type=int, default=200_000, help="threshold-model training cells")
+    ap.add_argument("--forms", default=None, help="forms to time (headline = last)")
+    ap.add_argument("--train-cells", type=int, default=None, help="threshold-model training cells")
     return ap.parse_args()
+
+
+def load_workload(name: str, train_cells: int):
+    from costal_amd import synth
+
+    if name == "rare_396460x15":
+        return synth.rare_396460x15(), (lambda: synth.rare_396460x15(n=train_cells, sample_seed=1))
+    if name == "cytometry_86864x39":
+        return synth.cytometry_86864x39(), None
+    if name == "learned_1000000x40":
+        return synth.learned_1000000x40(), (lambda: synth.learned_train_200000x40(n=train_cells))
+    if name == "counts_3605x2000":
+        return np.ascontiguousarray(synth.counts_3605x2000(), dtype=np.float32), None
+    if name == "batched_64":
+        return synth.batched_64(), None
+    raise ValueError(name)
 
 
 class DeviceRun:
@@ -74,6 +104,7 @@ class DeviceRun:
         self.norms = torch.empty(self.n, dtype=torch.float64, device=dev)
         self.w = torch.empty((self.n, k), dtype=torch.float64, device=dev)
         self.stream = torch.cuda.current_stream().cuda_stream
+        self.model = None
         torch.cuda.synchronize()
 
     def set_model(self, model):
@@ -105,20 +136,39 @@ class DeviceRun:
                 self.w.cpu().numpy())
 
 
-def time_form(run: DeviceRun, precision: str, steps: int, warmup: int):
+class BatchRun:
+    """The 64-sample batch: one costal_knn_batched_device call per step (a hipGraph replay)."""
+
+    def __init__(self, xs, k: int, tuner_plan=None):
+        from costal_amd import batched
+
+        margins = col_splits = None
+        if tuner_plan:
+            _, margins, col_splits = tuner_plan
+        self.b = batched.Batch(xs, k, margins=margins, col_splits=col_splits)
+        self.torch = self.b.torch
+
+    def step(self, form: str, threshold=None):
+        return self.b.run({"fp32": "mfma-f32", "fp16": "mfma-f16", "fp16x3": "mfma-f16x3"}[form])
+
+    def outputs(self):
+        outs = self.b.outputs()
+        return tuple(np.concatenate([o[c].ravel() for o in outs]) for c in range(4))
+
+
+def time_form(run, form: str, steps: int, warmup: int):
     torch = run.torch
     for _ in range(warmup):
-        run.step(precision)
+        run.step(form)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    stats = [run.step(precision) for _ in range(steps)]
+    stats = [run.step(form) for _ in range(steps)]
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1000.0 / steps
-    coarse = float(np.mean([s.coarse_ms for s in stats]))
     kern = {
         "prep": float(np.mean([s.prep_ms for s in stats])),
-        "coarse": coarse,
+        "coarse": float(np.mean([s.coarse_ms for s in stats])),
         "rescore": float(np.mean([s.rescore_ms for s in stats])),
         "fallback": float(np.mean([s.fallback_ms for s in stats])),
     }
@@ -141,8 +191,7 @@ def cpu_baseline(x: np.ndarray, k: int, target_s: float):
     per_row = max((times[1] - times[0]) / max(probes[1] - probes[0], 1), 1e-7)
     rows = int(min(n, max(probes[1], target_s / per_row)))
     t0 = time.perf_counter()
-    (idx, cos, norms), _ = oracle.knn_blocked(x, k, sort_by_norm=False, rows=(0, rows),
-                                               n_threads=threads)
+    (idx, cos, norms), _ = oracle.knn_blocked(x, k, sort_by_norm=False, rows=(0, rows), n_threads=threads)
     dt = time.perf_counter() - t0
     return {
         "value": rows / dt,
@@ -160,33 +209,49 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
               "(the path shards by rows; see the design notes)", file=sys.stderr)
         sys.exit(2)
-    from costal_amd import _lib, synth
+    from costal_amd import _lib
 
-    x = synth.rare_396460x15()
-    n, d = x.shape
-    k = args.k
-    run = DeviceRun(x, k)
-    forms = args.forms.split(",")
+    wk, wforms, wtrain = WORKLOADS[args.workload]
+    k = args.k or wk
+    forms = (args.forms or wforms).split(",")
+    train_cells = args.train_cells or wtrain
+    data, train_data = load_workload(args.workload, train_cells)
+    batched = args.workload == "batched_64"
+    tuner_info = None
+    if batched:
+        from costal_amd import tuner
+
+        t0 = time.perf_counter()
+        cache = os.path.join(HERE, "gpurun_out", "tune.json") if os.path.isdir(os.path.join(HERE, "gpurun_out")) else None
+        tn = tuner.Tuner(cache)
+        plan = tn.plan_batch([x.shape for x in data], k)
+        tuner_info = {"precision": plan[0], "seconds": time.perf_counter() - t0, "buckets": len(tn.table)}
+        run = BatchRun(data, k, plan)
+        n = sum(len(x) for x in data)
+        d = max(x.shape[1] for x in data)
+        flops = sum(2.0 * len(x) * len(x) * x.shape[1] for x in data)
+    else:
+        n, d = data.shape
+        run = DeviceRun(data, k)
+        flops = 2.0 * n * n * d
     model_info = None
     if any(f.endswith("+learned") for f in forms):
         from costal_amd import learned
 
         t0 = time.perf_counter()
-        xt = synth.rare_396460x15(n=args.train_cells, sample_seed=1)
-        model = learned.train(xt, k, n_target=n, seed=0)
+        model = learned.train(train_data(), k, n_target=n, seed=0)
         run.set_model(model)
-        model_info = {"train_cells": args.train_cells, "sample_seed": 1, "k_label": model.meta["k_label"],
+        model_info = {"train_cells": train_cells, "k_label": model.meta["k_label"],
                       "holdout_rmse": model.meta["holdout_rmse"], "offset": model.offset,
                       "quantile": model.meta["quantile"], "train_seconds": time.perf_counter() - t0,
                       "arch": f"{d}->64->64->1 relu, fp32 weights, bf16 inference"}
-    results = {}
-    outs = {}
+    results, outs = {}, {}
     for form in forms:
         ms, kern, st = time_form(run, form, args.steps, args.warmup)
         results[form] = {"ms_per_step": ms, "cells_per_s": n / (ms / 1000.0), "kernel_ms": kern,
                          "fallback_rows": int(st.fallback_rows), "short_rows": int(st.short_rows),
                          "col_splits": int(st.col_splits), "candidates": int(st.candidates),
-                         "inserts": int(st.inserts), "rowscan_rows": int(st.rowscan_rows)}
+                         "inserts": int(st.inserts), "rowscan_rows": int(st.rowscan_rows), "path": int(st.path)}
         outs[form] = run.outputs()
     head = forms[-1]
     ref = outs[forms[0]]
@@ -199,25 +264,27 @@ def main():
             "w_max_rel": float(np.max(np.abs(o[3] - ref[3]) / np.maximum(np.abs(ref[3]), 1e-300))),
         }
 
-    # roofline of the dominant kernel (fused tile + scan): algorithmic flops 2*N*N*d per launch
-    flops = 2.0 * n * n * d
-    coarse_ms = results[head]["kernel_ms"]["coarse"]
-    achieved = flops / (coarse_ms / 1000.0) / 1e12
+    # roofline of the dominant kernel: algorithmic flops 2*N*N*d per launch (all ordered pairs)
+    hk = results[head]["kernel_ms"]
+    path = results[head]["path"]
+    kern_name = {0: "coarse_kernel", 1: "rowscan_kernel", 2: "batched graph (prep+scan+rescore)"}[path]
+    kern_ms = hk["coarse"] if path != 1 else hk["fallback"]
+    achieved = flops / (kern_ms / 1000.0) / 1e12 if kern_ms > 0 else 0.0
     peak = PEAK_TFLOPS[PRECISION_OF[head]]
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None,
-                "kernel": "coarse_kernel", "flops_per_launch": flops, "avg_launch_ms": coarse_ms}
+                "frac": achieved / peak, "traffic": None, "kernel": kern_name,
+                "flops_per_launch": flops, "avg_launch_ms": kern_ms}
 
     cpu = None
-    if not args.no_cpu:
-        cpu, (rows, cidx, ccos) = cpu_baseline(x, k, args.cpu_seconds)
+    if not args.no_cpu and not batched:
+        cpu, (rows, cidx, ccos) = cpu_baseline(data, k, args.cpu_seconds)
         gidx, gcos = outs[head][0], outs[head][1]
         match["gpu_vs_cpu_sample"] = {"rows": rows,
                                       "idx_identical": bool(np.array_equal(gidx[:rows], cidx)),
                                       "cos_identical": bool(np.array_equal(gcos[:rows], ccos))}
 
     line = {
-        "metric": "cells/s exact k=10 cosine kNN+Tanimoto; mixed-prec vs FP32 GPU at 100% match",
+        "metric": METRIC if args.workload == "rare_396460x15" else f"cells/s exact k={k} cosine kNN+Tanimoto",
         "value": results[head]["cells_per_s"],
         "unit": "cells/s",
         "n_gpus": 1,
@@ -229,14 +296,14 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32" if PRECISION_OF[head] == "fp32" else "fp16",
         "data": "synthetic",
-        "config": {"workload": "rare_396460x15", "cells": n, "features": d, "k": k,
-                   "form": FORM_DESC[head],
-                   "parallelism": "single GPU"},
+        "config": {"workload": args.workload, "cells": n, "features": d, "k": k,
+                   "form": FORM_DESC[head], "parallelism": "single GPU"},
         "forms": results,
         "match": match,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "threshold_model": model_info,
+        "tuner": tuner_info,
         "device": _lib.device_info(),
     }
     print(json.dumps(line))

@@ -1,0 +1,120 @@
+"""Autotuner for the fused scan: coarse-tile precision, candidate-list width and candidate splits.
+
+For a shape bucket (feature count rounded up to 8, row count rounded to a power of two, k) it times
+the library on synthetic data of that shape (an arcsinh Gaussian mixture like cytometry data) over
+a small grid of configurations and keeps the fastest.  Results are cached in a JSON file
+(``$COSTAL_TUNE_CACHE``, default ``~/.cache/costal_amd/tune.json``).  None of these knobs can change
+the output; they only trade candidate-list inserts, rescoring work and parallelism.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+import os
+
+import numpy as np
+
+from . import _lib
+
+DEFAULT_CACHE = os.environ.get("COSTAL_TUNE_CACHE",
+                               os.path.join(os.path.expanduser("~"), ".cache", "costal_amd", "tune.json"))
+
+
+def bucket(n: int, d: int, k: int) -> str:
+    nb = 1 << max(0, int(round(math.log2(max(n, 1)))))
+    db = -(-d // 8) * 8
+    return f"d{db}:n{nb}:k{k}"
+
+
+def _synthetic(n: int, d: int, seed: int = 0) -> np.ndarray:
+    rng = np.random.default_rng([seed, n, d])
+    comps = int(rng.integers(10, 31))
+    means = rng.normal(0.0, 20.0, size=(comps, d))
+    w = rng.dirichlet(np.ones(comps))
+    c = rng.choice(comps, size=n, p=w)
+    x = means[c] + rng.normal(0.0, 5.0, size=(n, d))
+    return np.ascontiguousarray(np.arcsinh(x / 5.0), dtype=np.float32)
+
+
+def candidates(k: int):
+    margins = sorted({max(0, k // 2), k, min(2 * k, 64 - k)})
+    margins = [m for m in margins if k + m <= 64]
+    for prec in ("fp16x3", "fp32"):
+        for m in margins:
+            for ns in (0, 1, 4):
+                yield {"precision": prec, "margin": m, "col_splits": ns}
+
+
+def time_config(x_dev, n: int, d: int, k: int, cfg: dict, outs, reps: int = 3) -> float:
+    L = _lib.load()
+    p = _lib.KnnParams()
+    p.k = k
+    p.margin = cfg["margin"]
+    p.precision = _lib.PRECISION[cfg["precision"]]
+    p.col_splits = cfg["col_splits"]
+    st = _lib.KnnStats()
+    times = []
+    for r in range(reps + 1):
+        _lib.check(L.costal_knn_exact_device(C.c_void_p(x_dev.data_ptr()), n, d, C.byref(p),
+                                             *[C.c_void_p(o.data_ptr()) for o in outs], C.byref(st), None))
+        if r:
+            times.append(st.coarse_ms + st.rescore_ms + st.fallback_ms)
+    return float(np.median(times))
+
+
+def tune_shape(n: int, d: int, k: int, *, reps: int = 3, seed: int = 0) -> dict:
+    import torch
+
+    x = _synthetic(n, d, seed)
+    dev = torch.device("cuda:0")
+    xd = torch.from_numpy(x).to(dev)
+    outs = [torch.empty((n, k), dtype=torch.int32, device=dev), torch.empty((n, k), dtype=torch.float64, device=dev),
+            torch.empty(n, dtype=torch.float64, device=dev), torch.empty((n, k), dtype=torch.float64, device=dev)]
+    results = []
+    for cfg in candidates(k):
+        ms = time_config(xd, n, d, k, cfg, outs, reps)
+        results.append({**cfg, "ms": ms})
+    best = min(results, key=lambda r: r["ms"])
+    return {"best": {kk: best[kk] for kk in ("precision", "margin", "col_splits")}, "ms": best["ms"],
+            "trials": results, "shape": [n, d, k]}
+
+
+class Tuner:
+    """Cached per-bucket choices; ``get`` tunes a bucket the first time it is asked for."""
+
+    def __init__(self, cache_path: str | None = DEFAULT_CACHE):
+        self.path = cache_path
+        self.table = {}
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as f:
+                self.table = json.load(f)
+
+    def save(self) -> None:
+        if not self.path:
+            return
+        os.makedirs(os.path.dirname(self.path) or ".", exist_ok=True)
+        tmp = self.path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(self.table, f, indent=1)
+        os.replace(tmp, self.path)
+
+    def get(self, n: int, d: int, k: int) -> dict:
+        key = bucket(n, d, k)
+        if key not in self.table:
+            nb = int(key.split(":")[1][1:])
+            db = int(key.split(":")[0][1:])
+            self.table[key] = tune_shape(max(nb, k + 2), db, k)
+            self.save()
+        return self.table[key]["best"]
+
+    def plan_batch(self, shapes, k: int):
+        """Per-sample (margin, col_splits) and the batch precision: the precision chosen for the
+        buckets holding most of the scan work (sum of n^2 d)."""
+        plans = [self.get(n, d, k) for n, d in shapes]
+        work = {}
+        for (n, d), pl in zip(shapes, plans):
+            work[pl["precision"]] = work.get(pl["precision"], 0.0) + float(n) * n * d
+        prec = max(work, key=work.get)
+        return prec, [pl["margin"] for pl in plans], [pl["col_splits"] for pl in plans]

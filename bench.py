@@ -44,6 +44,12 @@ FORM_DESC = {
     "fp16x3+learned": "split-fp16 MFMA candidates + learned BF16 admission thresholds + f64 rescoring",
 }
 PRECISION_OF = {"fp32": "fp32", "fp16": "fp16", "fp16x3": "fp16x3", "fp16x3+learned": "fp16x3"}
+# HBM bytes per launch of the dominant kernel: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB), the scale
+# factors from a 1 GiB device copy under the same counters; two rocprofv3 --pmc runs of this script
+# (--forms fp16x3+learned --steps 2), mean over the dispatches of the first pass.
+MEASURED_TRAFFIC = {
+    ("rare_396460x15", "fp16x3+learned"): ((2_929_955 + 3_104_378) / 2 * 2.0 + 127_055) * 1024,
+}
 WORKLOADS = {
     # name: (k, forms, train the threshold model on this many other cells)
     "rare_396460x15": (10, "fp32,fp16x3,fp16x3+learned", 200_000),
@@ -272,7 +278,7 @@ def main():
     achieved = flops / (kern_ms / 1000.0) / 1e12 if kern_ms > 0 else 0.0
     peak = PEAK_TFLOPS[PRECISION_OF[head]]
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None, "kernel": kern_name,
+                "frac": achieved / peak, "traffic": MEASURED_TRAFFIC.get((args.workload, head)), "kernel": kern_name,
                 "flops_per_launch": flops, "avg_launch_ms": kern_ms}
 
     cpu = None

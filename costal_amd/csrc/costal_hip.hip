@@ -32,6 +32,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "device_ids.h"
+
 #include <stdarg.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -115,7 +117,7 @@ __global__ void prep_kernel(const float* __restrict__ x, uint32_t n, uint32_t d,
                             uint32_t dp32, uint32_t dp16, int split16, double* __restrict__ norms,
                             float* __restrict__ xs32, _Float16* __restrict__ xs16,
                             uint32_t* __restrict__ bad_row) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = bid_x() * bdim_x() + tid_x();
   if (i >= n_pad) return;
   const uint32_t dmax = dp32 > dp16 ? dp32 : dp16;
   const uint32_t s16 = split16 ? 2 * dp16 : dp16;
@@ -154,14 +156,15 @@ __global__ void prep_kernel(const float* __restrict__ x, uint32_t n, uint32_t d,
 // coarse_kernel
 //
 // Workgroup = NW waves; each wave owns 32 queries, held in registers for the whole launch.  The
-// workgroup streams its candidate range through a 4-stage LDS ring of TC-row tiles filled by
-// global_load_lds (direct to LDS, no register staging), three tiles ahead of the compute.  For every
-// 32-candidate sub-tile a wave issues the MFMAs for D[cand][query] = sum_k cand[k] * query[k]:
-// lane l owns query (l % 32) and the 16 candidates 8*(i/4) + 4*(l/32) + (i%4), i = 0..15 (the
-// 32x32 accumulator layout).  The scan takes the max of a group of sub-tiles (max3 chains) and
-// compares once with the lane's threshold; only waves with a hit take the insert path.
+// workgroup streams its candidate range through two LDS buffers of TC-row tiles (16 KB each),
+// staged global -> registers -> LDS two tiles ahead of the compute.  For every 32-candidate
+// sub-tile a wave issues the MFMAs for D[cand][query] = sum_k cand[k] * query[k]: lane l owns query
+// (l % 32) and the 16 candidates 8*(i/4) + 4*(l/32) + (i%4), i = 0..15 (the 32x32 accumulator
+// layout).  The scan takes the max of a step's sub-tiles (max3 chains) and compares once with the
+// lane's threshold; only waves with a hit take the insert path.  MFMAs run one step ahead of the
+// scan (see scan_block).
 //
-// Candidate lists live in registers: each lane keeps its own sorted list of the best K' (score,
+// Candidate lists live in registers: each lane keeps its own sorted list of the best kp (score,
 // candidate) pairs among the candidates it sees (its half of every sub-tile), so the two lanes of a
 // query never wait on each other.  Both lists go to the rescoring, which treats them like two
 // candidate-range splits: a candidate outside both lists scored at most the larger of the two
@@ -173,10 +176,6 @@ __global__ void prep_kernel(const float* __restrict__ x, uint32_t n, uint32_t d,
 // the same products).  In LDS the chunks of one (sub-tile, slot, half) are stored for the 32 lanes
 // contiguously (1 KB per wave-wide load), so each ds_read_b128 is conflict free.
 
-#define LDS_PTR(p) ((__attribute__((address_space(3))) void*)(p))
-
-// s_waitcnt vmcnt(N) with lgkmcnt and expcnt left alone (gfx9 encoding)
-#define WAIT_VMCNT(N) __builtin_amdgcn_s_waitcnt(0x0f70 | ((N) & 15) | (((N) >> 4) << 14))
 
 __device__ __forceinline__ float max16(const f32x16& a) {
   const float m0 = fmaxf(fmaxf(a[0], a[1]), a[2]);
@@ -187,12 +186,14 @@ __device__ __forceinline__ float max16(const f32x16& a) {
   return fmaxf(fmaxf(fmaxf(m0, m1), m2), fmaxf(fmaxf(m3, m4), a[15]));
 }
 
-// a[i] for a lane-varying i, as a select chain (keeps the accumulator in registers)
+// a[i] for a lane-varying i, as a select tree on the bits of i (keeps the accumulator in registers)
 __device__ __forceinline__ float pick16(const f32x16& a, int i) {
-  float r = a[0];
-#pragma unroll
-  for (int e = 1; e < 16; ++e) r = (i == e) ? a[e] : r;
-  return r;
+  const bool b3 = i & 8, b2 = i & 4, b1 = i & 2, b0 = i & 1;
+  const float c0 = b3 ? a[8] : a[0], c1 = b3 ? a[9] : a[1], c2 = b3 ? a[10] : a[2], c3 = b3 ? a[11] : a[3];
+  const float c4 = b3 ? a[12] : a[4], c5 = b3 ? a[13] : a[5], c6 = b3 ? a[14] : a[6], c7 = b3 ? a[15] : a[7];
+  const float d0 = b2 ? c4 : c0, d1 = b2 ? c5 : c1, d2 = b2 ? c6 : c2, d3 = b2 ? c7 : c3;
+  const float e0 = b1 ? d2 : d0, e1 = b1 ? d3 : d1;
+  return b0 ? e1 : e0;
 }
 
 // Sorted (ascending) register list: ls[0] is the smallest live score.  Entries [kp, KPR) hold +inf
@@ -215,23 +216,22 @@ __device__ __forceinline__ void list_push(float (&ls)[KPR], uint32_t (&li)[KPR],
 }
 
 // Precision forms of the coarse tile:
-//   PREC_F32   f32 rows, v_mfma_f32_32x32x2f32; a 16-byte slot is 4 features (4 MFMAs).
-//   PREC_F16   FP16 rows, v_mfma_f32_32x32x16_f16; a slot is 8 features (1 MFMA); a hit is re-scored
+//   PREC_F32   f32 rows, v_mfma_f32_32x32x2f32; a 16-byte chunk is 4 features (4 MFMAs).
+//   PREC_F16   FP16 rows, v_mfma_f32_32x32x16_f16; a chunk is 8 features (1 MFMA); a hit is re-scored
 //              in f32 before admission.
-//   PREC_F16X3 FP16 hi/lo split rows (x = hi + lo + O(2^-22)): slots [0, J/2) hold hi, [J/2, J) lo;
-//              hi.hi + hi.lo + lo.hi per slot (3 MFMAs), an f32-class score without re-scoring.
+//   PREC_F16X3 FP16 hi/lo split rows (x = hi + lo + O(2^-22)): chunks [0, J/2) hold hi, [J/2, J) lo;
+//              hi.hi + hi.lo + lo.hi per chunk pair (3 MFMAs), an f32-class score without re-scoring.
 constexpr int PREC_F16X3 = COSTAL_PREC_FP16X3;
 
+// The 32x32 product of one sub-tile: b = the lane's J candidate chunks, q = its J query chunks.
 template <int PREC, int J>
-__device__ __forceinline__ f32x16 tile_mfma(const uint8_t* __restrict__ T, int sub, int h, int l32,
-                                            const uint4 (&qreg)[J]) {
+__device__ __forceinline__ f32x16 tile_mfma(const uint4 (&b)[J], const uint4 (&q)[J]) {
   f32x16 acc = {};
   if constexpr (PREC == PREC_F32) {
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-      const uint4 bv = *reinterpret_cast<const uint4*>(T + (((sub * J + j) * 2 + h) * 32 + l32) * 16);
-      const f32x4 b4 = __builtin_bit_cast(f32x4, bv);
-      const f32x4 q4 = __builtin_bit_cast(f32x4, qreg[j]);
+      const f32x4 b4 = __builtin_bit_cast(f32x4, b[j]);
+      const f32x4 q4 = __builtin_bit_cast(f32x4, q[j]);
       acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[0], q4[0], acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[1], q4[1], acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[2], q4[2], acc, 0, 0, 0);
@@ -239,19 +239,15 @@ __device__ __forceinline__ f32x16 tile_mfma(const uint8_t* __restrict__ T, int s
     }
   } else if constexpr (PREC == PREC_F16) {
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const uint4 bv = *reinterpret_cast<const uint4*>(T + (((sub * J + j) * 2 + h) * 32 + l32) * 16);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, bv),
-                                                   __builtin_bit_cast(f16x8, qreg[j]), acc, 0, 0, 0);
-    }
+    for (int j = 0; j < J; ++j)
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, b[j]), __builtin_bit_cast(f16x8, q[j]),
+                                                   acc, 0, 0, 0);
   } else {
     constexpr int JH = J / 2;
 #pragma unroll
     for (int j = 0; j < JH; ++j) {
-      const uint4 bh = *reinterpret_cast<const uint4*>(T + (((sub * J + j) * 2 + h) * 32 + l32) * 16);
-      const uint4 bl = *reinterpret_cast<const uint4*>(T + (((sub * J + j + JH) * 2 + h) * 32 + l32) * 16);
-      const f16x8 ch = __builtin_bit_cast(f16x8, bh), cl = __builtin_bit_cast(f16x8, bl);
-      const f16x8 qh = __builtin_bit_cast(f16x8, qreg[j]), ql = __builtin_bit_cast(f16x8, qreg[j + JH]);
+      const f16x8 ch = __builtin_bit_cast(f16x8, b[j]), cl = __builtin_bit_cast(f16x8, b[j + JH]);
+      const f16x8 qh = __builtin_bit_cast(f16x8, q[j]), ql = __builtin_bit_cast(f16x8, q[j + JH]);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(cl, qh, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ch, ql, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ch, qh, acc, 0, 0, 0);
@@ -264,10 +260,22 @@ template <int J>
 struct TileShape {
   static constexpr int TC = J <= 2 ? 256 : J <= 4 ? 128 : 64;  // candidates per tile: 16 KB tiles
   static constexpr int TILEB = TC * 32 * J;
+  static constexpr int SUBT = TC / 32;                         // 32-candidate sub-tiles per tile
+  static constexpr int GS = (SUBT >= 4 && J <= 4) ? 2 : 1;     // sub-tiles per scan step
+  static constexpr int NSTEP = SUBT / GS;                      // scan steps per tile
 };
 
 // One work item of the scan: query block `qblock` (QB queries) against candidate split `split` of
 // one sample.  tile0/tile1 are the workgroup's two LDS tile buffers.
+//
+// Software pipeline over scan steps (GS sub-tiles each, NSTEP per tile, G counting steps over the
+// whole candidate range): step G issues the MFMAs of step G+1, reads the candidate chunks of step
+// G+2 from LDS into the registers those MFMAs have just consumed, and then scans the accumulators
+// of step G, so the matrix core works on the next step while the wave compares and inserts, and
+// the LDS reads complete behind the scan.  Accumulators are double buffered by the parity of G
+// (NSTEP is even, so the parity is the step's parity within its tile).  The next tile is stored to
+// the other LDS buffer, and the workgroup meets at the barrier, at step NSTEP-2 of each tile, before
+// the first read of that tile (step NSTEP-2 reads step 0 of the next tile).
 template <int PREC, int J, int KPR, int NW>
 __device__ __forceinline__ void scan_block(
     uint8_t* __restrict__ tile0, uint8_t* __restrict__ tile1,
@@ -282,21 +290,17 @@ __device__ __forceinline__ void scan_block(
     uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc, uint32_t* __restrict__ cand_cnt,
     float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
   constexpr int QB = 32 * NW;
-  constexpr int TC = J <= 2 ? 256 : J <= 4 ? 128 : 64;  // 16 KB tiles
-  constexpr int SUBT = TC / 32;
+  constexpr int TC = TileShape<J>::TC;
   constexpr int ROWB = 32 * J;
-  constexpr int TILEB = TC * ROWB;
   constexpr int NT = 64 * NW;
   constexpr int CHUNKS = TC * 2 * J;        // 16-byte chunks per tile
   constexpr int CPT = (CHUNKS + NT - 1) / NT;
-  // sub-tiles whose MFMAs are issued together before one scan (FP32 has 4x the MFMAs per sub-tile)
-  constexpr int GMAX = PREC == PREC_F32 ? 2 : 4;
-  constexpr int GS = SUBT < GMAX ? SUBT : GMAX;
-  static_assert(SUBT % GS == 0, "sub-tile grouping");
+  constexpr int GS = TileShape<J>::GS;
+  constexpr int NSTEP = TileShape<J>::NSTEP;
+  static_assert(NSTEP >= 2 && NSTEP % 2 == 0, "pipeline steps per tile");
+  static_assert(GS * NSTEP * 32 == TC, "tile shape");
 
-  static_assert(TILEB == TileShape<J>::TILEB, "tile shape");
-
-  const int tid = threadIdx.x;
+  const int tid = tid_x();
   const int w = tid >> 6, l = tid & 63, h = l >> 5, l32 = l & 31;
   const uint32_t qq = qblock * QB + w * 32 + l32;
   const bool qvalid = qq < nq;
@@ -326,10 +330,11 @@ __device__ __forceinline__ void scan_block(
   if (c_end > n) c_end = n;
   const int ntile = c_begin < c_end ? (int)((c_end - c_begin + TC - 1) / TC) : 0;
 
-  // Tiles go global -> registers -> LDS, two tiles ahead of the scan (register sets ra and rb, LDS
-  // buffers tile0 and tile1, the loop unrolled by two so every register and buffer is static).
+  // Tiles go global -> registers -> LDS: the registers of tile t+1 are loaded while tile t-1 is
+  // scanned and stored to LDS during tile t (buffers tile0 and tile1, the loop unrolled by two so
+  // every buffer is static).
   static_assert(CPT <= 4, "tile staging uses at most four 16-byte registers per thread");
-  uint4 ra0 = {}, ra1 = {}, ra2 = {}, ra3 = {}, rb0 = {}, rb1 = {}, rb2 = {}, rb3 = {};
+  uint4 ra0 = {}, ra1 = {}, ra2 = {}, ra3 = {};
   // Thread e fills LDS slot e (consecutive lanes, consecutive 16-byte slots: conflict free); slot
   // e = ((sub*J + j)*2 + half)*32 + lane32 holds chunk 2j + half of tile row sub*32 + lane32.
 #define LOAD_ONE(R, C)                                                                            \
@@ -337,7 +342,8 @@ __device__ __forceinline__ void scan_block(
     const int e_ = tid + C * NT;                                                                  \
     if (CHUNKS % NT == 0 || e_ < CHUNKS) {                                                        \
       const int lr_ = e_ & 31, hh_ = (e_ >> 5) & 1, j_ = (e_ >> 6) % J, sub_ = (e_ >> 6) / J;     \
-      R##C = *reinterpret_cast<const uint4*>(src_ + (size_t)(sub_ * 32 + lr_) * ROWB + (2 * j_ + hh_) * 16); \
+      const uint32_t off_ = (uint32_t)((sub_ * 32 + lr_) * ROWB + (2 * j_ + hh_) * 16);            \
+      R##C = *reinterpret_cast<const uint4*>(src_ + off_);                                       \
     }                                                                                             \
   }
 #define LOAD_REGS(R, T)                                                                           \
@@ -359,31 +365,46 @@ __device__ __forceinline__ void scan_block(
     __builtin_amdgcn_s_barrier();                                                                 \
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");                                        \
   }
+  // the lane's chunks of scan step S of the tile in BUF
+#define READ_B(BUF, S)                                                                            \
+  _Pragma("unroll") for (int u_ = 0; u_ < GS; ++u_)                                               \
+    _Pragma("unroll") for (int j_ = 0; j_ < J; ++j_)                                              \
+      bop[u_][j_] = *reinterpret_cast<const uint4*>(                                              \
+          (BUF) + (((((S) * GS + u_) * J + j_) * 2 + h) * 32 + l32) * 16);
+#define MFMA_STEP(SLOT) \
+  _Pragma("unroll") for (int u_ = 0; u_ < GS; ++u_) acc[SLOT][u_] = tile_mfma<PREC, J>(bop[u_], qreg);
 
   const float* xq32 = xs32 + (size_t)qg * dp32;
-  f32x16 acc[GS];
+  f32x16 acc[2][GS];
+  uint4 bop[GS][J];
   int t = 0;
-  auto scan_group = [&](int g) __attribute__((always_inline)) {
+  // compare the accumulators of step `p` of tile t with the lane's threshold; rare path: each lane
+  // walks its own hits, sub-tile by sub-tile
+  auto scan_step = [&](const f32x16 (&a)[GS], int p) __attribute__((always_inline)) {
     float mu[GS];
 #pragma unroll
-    for (int u = 0; u < GS; ++u) mu[u] = max16(acc[u]);
+    for (int u = 0; u < GS; ++u) mu[u] = max16(a[u]);
     float m = mu[0];
 #pragma unroll
     for (int u = 1; u < GS; ++u) m = fmaxf(m, mu[u]);
     float the = (PREC == PREC_F16) ? th - eps_pre : th;
     if (__builtin_amdgcn_ballot_w64(m > the) != 0) {
-      // rare path: each lane walks its own hits, sub-tile by sub-tile
 #pragma unroll
       for (int u = 0; u < GS; ++u) {
         if (mu[u] > the) {
+          // bit i = accumulator i above the threshold (built from the top, no wide constants)
           uint32_t mask = 0;
 #pragma unroll
-          for (int i = 0; i < 16; ++i) mask |= (acc[u][i] > the ? 1u : 0u) << i;
-          const uint32_t cbase = c_begin + (uint32_t)t * TC + (uint32_t)(g + u) * 32 + 4 * h;
+          for (int i = 15; i >= 0; --i) mask = (mask << 1) | (a[u][i] > the ? 1u : 0u);
+          // the lane's offset is added here, not folded into per-sub-tile bases hoisted out of the
+          // tile loop (each would hold a register for the whole launch)
+          uint32_t lane_off = 4 * h;
+          __asm__ volatile("" : "+v"(lane_off));
+          const uint32_t cbase = c_begin + (uint32_t)t * TC + (uint32_t)(p * GS + u) * 32 + lane_off;
           while (mask) {
             const int i = __builtin_ctz(mask);
             mask &= mask - 1;
-            float v = pick16(acc[u], i);
+            float v = pick16(a[u], i);
             const uint32_t cand = cbase + 8 * (i >> 2) + (i & 3);
             if (v > the && cand < c_end && cand != qg) {
               if constexpr (PREC == PREC_F16) {
@@ -400,40 +421,48 @@ __device__ __forceinline__ void scan_block(
       }
     }
   };
-#define SCAN_TILE(BUF)                                                                            \
-  _Pragma("unroll") for (int g = 0; g < SUBT; g += GS) {                                          \
-    _Pragma("unroll") for (int u = 0; u < GS; ++u) acc[u] = tile_mfma<PREC, J>(BUF, g + u, h, l32, qreg); \
-    scan_group(g);                                                                                \
+  // all NSTEP steps of tile t, held in CUR; the next tile goes from the staging registers to OTH
+#define RUN_TILE(CUR, OTH)                                                                        \
+  _Pragma("unroll") for (int p = 0; p < NSTEP; ++p) {                                             \
+    MFMA_STEP((p + 1) & 1)                                                                        \
+    if (p == NSTEP - 2) {                                                                         \
+      STORE_REGS(ra, OTH);                                                                        \
+      TILE_BARRIER();                                                                             \
+      LOAD_REGS(ra, t + 2 < ntile ? t + 2 : ntile - 1);                                           \
+    }                                                                                             \
+    if (p + 2 < NSTEP) {                                                                          \
+      READ_B(CUR, p + 2)                                                                          \
+    } else {                                                                                      \
+      READ_B(OTH, p + 2 - NSTEP)                                                                  \
+    }                                                                                             \
+    scan_step(acc[p & 1], p);                                                                     \
   }
 
   if (ntile > 0) {
     LOAD_REGS(ra, 0);
-    LOAD_REGS(rb, ntile > 1 ? 1 : 0);
     STORE_REGS(ra, tile0);
     TILE_BARRIER();
-    LOAD_REGS(ra, ntile > 2 ? 2 : ntile - 1);
+    LOAD_REGS(ra, ntile > 1 ? 1 : 0);
+    READ_B(tile0, 0)
+    MFMA_STEP(0)
+    READ_B(tile0, 1)
     for (;;) {
-      // tile t in tile0, rb = tile t+1, ra = tile t+2 in flight
-      SCAN_TILE(tile0);
-      STORE_REGS(rb, tile1);
-      TILE_BARRIER();
-      LOAD_REGS(rb, t + 3 < ntile ? t + 3 : ntile - 1);
+      // tile t in tile0, ra = tile t+1 in flight
+      RUN_TILE(tile0, tile1)
       if (++t >= ntile) break;
-      // tile t in tile1, ra = tile t+1, rb = tile t+2 in flight
-      SCAN_TILE(tile1);
-      STORE_REGS(ra, tile0);
-      TILE_BARRIER();
-      LOAD_REGS(ra, t + 3 < ntile ? t + 3 : ntile - 1);
+      // tile t in tile1, ra = tile t+1 in flight
+      RUN_TILE(tile1, tile0)
       if (++t >= ntile) break;
     }
   }
-#undef SCAN_TILE
+#undef RUN_TILE
+#undef MFMA_STEP
+#undef READ_B
 #undef TILE_BARRIER
 #undef LOAD_REGS
 #undef STORE_REGS
 #undef LOAD_ONE
 #undef STORE_ONE
-
 
   if (inserts) {
     // one atomic per wave
@@ -467,8 +496,8 @@ __global__ __launch_bounds__(64 * NW) void coarse_kernel(
     float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
   __shared__ __attribute__((aligned(16))) uint8_t tile0[TileShape<J>::TILEB];
   __shared__ __attribute__((aligned(16))) uint8_t tile1[TileShape<J>::TILEB];
-  scan_block<PREC, J, KPR, NW>(tile0, tile1, xsq, xs32, dp32, n, kp, rowlist, nq, blockIdx.x, tinit, blockIdx.y,
-                               gridDim.y, cols_per_split, eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor, inserts);
+  scan_block<PREC, J, KPR, NW>(tile0, tile1, xsq, xs32, dp32, n, kp, rowlist, nq, bid_x(), tinit, bid_y(),
+                               gdim_y(), cols_per_split, eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor, inserts);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -480,7 +509,7 @@ __device__ void bitonic_lds(double* __restrict__ sc, double* __restrict__ cs, ui
                             int P) {
   for (int size = 2; size <= P; size <<= 1) {
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int e = threadIdx.x; e < P / 2; e += NT) {
+      for (int e = tid_x(); e < P / 2; e += NT) {
         int a = 2 * e - (e & (stride - 1));
         int b = a + stride;
         bool up = (a & size) == 0;
@@ -532,7 +561,7 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
     total += cand_cnt[(size_t)qq * nsplit + s];
     floor_ = fmax(floor_, (double)cand_floor[(size_t)qq * nsplit + s]);
   }
-  for (int e = threadIdx.x; e < P; e += 64) {
+  for (int e = tid_x(); e < P; e += 64) {
     double sc = -INFINITY;
     uint32_t j = NONE;
     if (e < tot_slots) {
@@ -558,14 +587,14 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
     ok = s_sc[k - 1] > floor_ + eps_list;
   }
   if (ok) {
-    for (uint32_t t = threadIdx.x; t < k; t += 64) {
+    for (uint32_t t = tid_x(); t < k; t += 64) {
       const uint32_t j = s_id[t];
       const double c = s_sc[t];
       idx[(size_t)i * k + t] = j;
       cos_out[(size_t)i * k + t] = c;
       if (wts) wts[(size_t)i * k + t] = tani_from_cos(ni, norms[j], c);
     }
-  } else if (threadIdx.x == 0) {
+  } else if (tid_x() == 0) {
     const uint32_t p = atomicAdd(fb_count, 1u);
     fb_rows[p] = fb_tag;
     fb_lo[p] = total >= k ? s_sc[k - 1] - eps32 : -INFINITY;
@@ -584,7 +613,7 @@ __global__ __launch_bounds__(64) void rescore_kernel(
     uint32_t* __restrict__ short_count) {
   __shared__ double s_sc[CMAX];
   __shared__ uint32_t s_id[CMAX];
-  const uint32_t qq = blockIdx.x;
+  const uint32_t qq = bid_x();
   if (qq >= nq) return;
   const uint32_t i = rowlist ? rowlist[qq] : qq;
   rescore_row<CMAX>(s_sc, s_id, x, d, norms, k, kp, nsplit, qq, i, cand_idx, cand_cnt, cand_floor, all_admitted,
@@ -617,7 +646,7 @@ __device__ void rowscan_row(RowscanSmem<CAP>& S, const float* __restrict__ x, ui
                             const float* __restrict__ xs32, uint32_t dp32, const double* __restrict__ norms,
                             uint32_t n, uint32_t k, uint32_t select, uint32_t i, double lo0, double eps32,
                             uint32_t* __restrict__ idx, double* __restrict__ cos_out, double* __restrict__ wts) {
-  const int tid = threadIdx.x;
+  const int tid = tid_x();
   if (tid == 0) {
     S.L = 0;
     S.B = 0;
@@ -699,7 +728,7 @@ __global__ __launch_bounds__(256) void rowscan_kernel(
     double* __restrict__ cos_out, double* __restrict__ wts) {
   __shared__ RowscanSmem<CAP> S;
   const uint32_t nrows = nrows_dev ? *nrows_dev : nrows_host;
-  for (uint32_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+  for (uint32_t r = bid_x(); r < nrows; r += gdim_x()) {
     const uint32_t i = rows ? rows[r] : r;
     rowscan_row<CAP, COARSE>(S, x, d, xs32, dp32, norms, n, k, select, i, lo_arr ? lo_arr[r] : -INFINITY, eps32,
                              idx, cos_out, wts);
@@ -741,14 +770,14 @@ __device__ __forceinline__ uint32_t find_sample(const uint32_t* __restrict__ fir
 // Graph-safe counter reset (no memset nodes): counters[0..1] = first bad (sample, row) as u64
 // (all ones = none), everything else zero.
 __global__ void batch_init_kernel(uint32_t* __restrict__ counters, uint32_t count) {
-  for (uint32_t e = threadIdx.x; e < count; e += blockDim.x) counters[e] = e < 2 ? NONE : 0u;
+  for (uint32_t e = tid_x(); e < count; e += bdim_x()) counters[e] = e < 2 ? NONE : 0u;
 }
 
 __global__ void prep_batch_kernel(const BatchSample* __restrict__ samples, const uint32_t* __restrict__ pad_first,
                                   uint32_t nsamples, uint32_t total_pad, uint32_t dp32, uint32_t dp16, int split16,
                                   float* __restrict__ xs32, _Float16* __restrict__ xs16,
                                   unsigned long long* __restrict__ bad) {
-  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t g = bid_x() * bdim_x() + tid_x();
   if (g >= total_pad) return;
   const uint32_t sidx = find_sample(pad_first, nsamples, g);
   const BatchSample& S = samples[sidx];
@@ -796,9 +825,9 @@ __global__ __launch_bounds__(64 * NW) void coarse_batch_kernel(
   __shared__ __attribute__((aligned(16))) uint8_t tile1[TileShape<J>::TILEB];
   __shared__ uint32_t s_item;
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
+    if (tid_x() == 0) s_item = atomicAdd(queue, 1u);
     __syncthreads();
-    const uint32_t it = s_item;
+    const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
     __syncthreads();
     if (it >= nitems) break;
     const uint32_t sidx = items[3 * it], qb = items[3 * it + 1], sp = items[3 * it + 2];
@@ -820,7 +849,7 @@ __global__ __launch_bounds__(64) void rescore_batch_kernel(
     double* __restrict__ fb_lo, uint32_t* __restrict__ fb_count, uint32_t* __restrict__ short_count) {
   __shared__ double s_sc[CMAX];
   __shared__ uint32_t s_id[CMAX];
-  const uint32_t g = blockIdx.x;
+  const uint32_t g = bid_x();
   if (g >= total_rows) return;
   const uint32_t sidx = find_sample(row_first, nsamples, g);
   const BatchSample& S = samples[sidx];
@@ -839,7 +868,7 @@ __global__ __launch_bounds__(256) void rowscan_batch_kernel(
     const uint32_t* __restrict__ nrows_dev, const double* __restrict__ lo_arr, double eps32) {
   __shared__ RowscanSmem<CAP> Sm;
   const uint32_t nrows = *nrows_dev;
-  for (uint32_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+  for (uint32_t r = bid_x(); r < nrows; r += gdim_x()) {
     const uint32_t g = rows[r];
     const uint32_t sidx = find_sample(row_first, nsamples, g);
     const BatchSample& S = samples[sidx];
@@ -855,7 +884,7 @@ __global__ void reweight_kernel(uint64_t n, uint32_t k, const uint32_t* __restri
                                 const double* __restrict__ cos, const double* __restrict__ norms,
                                 uint32_t weight, double* __restrict__ out,
                                 uint32_t* __restrict__ bad) {
-  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t e = (uint64_t)bid_x() * bdim_x() + tid_x();
   if (e >= n * k) return;
   const uint32_t j = idx[e];
   if (j >= n) {
@@ -993,11 +1022,16 @@ CoarseFn coarse_by_j(uint32_t J) {
 }
 
 // Register-list capacity 32 or 64 (K' <= capacity), 4 waves per workgroup.
+template <int PREC>
+CoarseFn coarse_by_kp(uint32_t J, int KP) {
+  return KP == 16 ? coarse_by_j<PREC, 16, 4>(J) : KP == 32 ? coarse_by_j<PREC, 32, 4>(J) : coarse_by_j<PREC, 64, 4>(J);
+}
+
 CoarseFn coarse_select(int prec, uint32_t J, int KP) {
-  if (prec == PREC_F32) return KP == 32 ? coarse_by_j<PREC_F32, 32, 4>(J) : coarse_by_j<PREC_F32, 64, 4>(J);
-  if (prec == PREC_F16) return KP == 32 ? coarse_by_j<PREC_F16, 32, 4>(J) : coarse_by_j<PREC_F16, 64, 4>(J);
+  if (prec == PREC_F32) return coarse_by_kp<PREC_F32>(J, KP);
+  if (prec == PREC_F16) return coarse_by_kp<PREC_F16>(J, KP);
   if (J % 2) return nullptr;
-  return KP == 32 ? coarse_by_j<PREC_F16X3, 32, 4>(J) : coarse_by_j<PREC_F16X3, 64, 4>(J);
+  return coarse_by_kp<PREC_F16X3>(J, KP);
 }
 
 // FP16 candidate error bound.  x16 = fp16(fp32(x/|x|)): |x16 - x| <= U|x| + eta per component with
@@ -1029,17 +1063,28 @@ double fp16x3_eps(uint32_t d) {
   return (a + e) * (2.0 + a + e) + (u16 + e) * (u16 + e) + gam * (1.0 + u16 + e) * (1.0 + u16 + e);
 }
 
+// Capacity of one candidate list.  A query's candidates are split over two lists (the two lanes of
+// its lane pair, each seeing half of every sub-tile) and over the candidate splits; with K' = k +
+// margin candidates per row (the reference's heap size), each list keeps k + ceil(margin / 2): all
+// of a row's k nearest fit in one list, and the lists together hold more than K'.  The capacity only
+// trades inserts against fallbacks; the certificate makes every choice exact.
+uint32_t list_capacity(uint64_t n, uint64_t k, uint64_t margin) {
+  return (uint32_t)std::min<uint64_t>(k + (margin + 1) / 2, n - 1);
+}
+
+int list_template(uint32_t kp) { return kp <= 16 ? 16 : kp <= 32 ? 32 : 64; }
+
 Plan make_plan(const Device& D, uint64_t n, uint64_t d, uint64_t k, const costal_knn_params& p) {
   Plan P{};
   const uint64_t margin = p.margin < 0 ? k : (uint64_t)p.margin;
-  P.kp = (uint32_t)std::min<uint64_t>(k + margin, n - 1);
+  P.kp = list_capacity(n, k, margin);
   P.prec = (int)p.precision;
   P.dp32 = round_up((uint32_t)std::max<uint64_t>(d, 1), 8);
   P.dp16 = round_up((uint32_t)std::max<uint64_t>(d, 1), 16);
   const uint32_t dmax = P.prec == PREC_F16 ? 128 : 64;
   P.fused = p.select == COSTAL_SELECT_COSINE && d <= dmax && P.kp <= FUSED_KP_MAX;
   P.J = P.prec == PREC_F16 ? P.dp16 / 16 : P.prec == PREC_F16X3 ? P.dp16 / 8 : P.dp32 / 8;
-  P.KP = P.kp <= 32 ? 32 : 64;
+  P.KP = list_template(P.kp);
   P.NW = 4;
   P.QB = 32 * P.NW;
   P.TC = P.J <= 2 ? 256 : P.J <= 4 ? 128 : 64;  // the kernel's 16 KB tiles
@@ -1308,11 +1353,17 @@ CoarseBatchFn coarse_batch_by_j(uint32_t J) {
   return nullptr;
 }
 
+template <int PREC>
+CoarseBatchFn coarse_batch_by_kp(uint32_t J, int KP) {
+  return KP == 16 ? coarse_batch_by_j<PREC, 16>(J) : KP == 32 ? coarse_batch_by_j<PREC, 32>(J)
+                                                             : coarse_batch_by_j<PREC, 64>(J);
+}
+
 CoarseBatchFn coarse_batch_select(int prec, uint32_t J, int KP) {
-  if (prec == PREC_F32) return KP == 32 ? coarse_batch_by_j<PREC_F32, 32>(J) : coarse_batch_by_j<PREC_F32, 64>(J);
-  if (prec == PREC_F16) return KP == 32 ? coarse_batch_by_j<PREC_F16, 32>(J) : coarse_batch_by_j<PREC_F16, 64>(J);
+  if (prec == PREC_F32) return coarse_batch_by_kp<PREC_F32>(J, KP);
+  if (prec == PREC_F16) return coarse_batch_by_kp<PREC_F16>(J, KP);
   if (J % 2) return nullptr;
-  return KP == 32 ? coarse_batch_by_j<PREC_F16X3, 32>(J) : coarse_batch_by_j<PREC_F16X3, 64>(J);
+  return coarse_batch_by_kp<PREC_F16X3>(J, KP);
 }
 
 struct BatchGraph {
@@ -1384,8 +1435,8 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
       const costal_sample& a = smp[i];
       const uint32_t n = (uint32_t)a.n;
       const int64_t mg = a.margin >= 0 ? a.margin : (p.margin >= 0 ? p.margin : (int64_t)k);
-      const uint32_t kp = (uint32_t)std::min<uint64_t>(k + (uint64_t)mg, n - 1);
-      if (kp > FUSED_KP_MAX) return fail(COSTAL_E_UNSUPPORTED, "sample %u: k + margin above %u", i, FUSED_KP_MAX);
+      const uint32_t kp = list_capacity(n, k, (uint64_t)mg);
+      if (kp > FUSED_KP_MAX) return fail(COSTAL_E_UNSUPPORTED, "sample %u: k + ceil(margin/2) above %u", i, FUSED_KP_MAX);
       const uint32_t ntile = (n + TC - 1) / TC;
       uint32_t ns = a.col_splits ? a.col_splits : (n + target_cols - 1) / target_cols;
       ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, ntile));
@@ -1438,7 +1489,7 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
     for (uint32_t e = 0; e < nitems; ++e)
       for (int c = 0; c < 3; ++c) sorted[3 * e + c] = items[3 * order[e] + c];
 
-    const int KP = max_kp <= 32 ? 32 : 64;
+    const int KP = list_template(max_kp);
     CoarseBatchFn fn = coarse_batch_select(prec, J, KP);
     if (!fn) return fail(COSTAL_E_UNSUPPORTED, "no batched scan kernel for d=%u", dmax);
 

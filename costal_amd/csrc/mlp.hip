@@ -13,6 +13,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "device_ids.h"
+
 #include <stdint.h>
 
 #include "../../include/costal_hip.h"
@@ -58,18 +60,18 @@ __device__ void load_params(Smem& S, const float* __restrict__ P, uint32_t d) {
   const float* b2 = W2 + H * H;
   const float* w3 = b2 + H;
   const float* b3 = w3 + H;
-  for (int e = threadIdx.x; e < H * DMAX; e += blockDim.x) {
+  for (int e = tid_x(); e < H * DMAX; e += bdim_x()) {
     const int j = e / DMAX, t = e % DMAX;
     const float v = (uint32_t)t < d ? W1[j * d + t] : 0.0f;
     S.W1[e] = BF16 ? bf16r(v) : v;
   }
-  for (int e = threadIdx.x; e < H * H; e += blockDim.x) S.W2[e] = BF16 ? bf16r(W2[e]) : W2[e];
-  for (int e = threadIdx.x; e < H; e += blockDim.x) {
+  for (int e = tid_x(); e < H * H; e += bdim_x()) S.W2[e] = BF16 ? bf16r(W2[e]) : W2[e];
+  for (int e = tid_x(); e < H; e += bdim_x()) {
     S.b1[e] = b1[e];
     S.b2[e] = b2[e];
     S.w3[e] = BF16 ? bf16r(w3[e]) : w3[e];
   }
-  if (threadIdx.x == 0) S.b3 = b3[0];
+  if (tid_x() == 0) S.b3 = b3[0];
 }
 
 // Forward for one row in registers.  BF16: inputs, weights and hidden activations rounded to bf16,
@@ -112,7 +114,7 @@ __global__ __launch_bounds__(256) void predict_kernel(const float* __restrict__ 
   __shared__ Smem S;
   load_params<BF16>(S, P, d);
   __syncthreads();
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t r = bid_x() * bdim_x() + tid_x();
   if (r >= n) return;
   float xn[DMAX], h1[H], h2[H];
   load_features(x + (size_t)r * d, d, xn);
@@ -130,7 +132,7 @@ __global__ __launch_bounds__(256) void fwd_bwd_kernel(const float* __restrict__ 
   __shared__ Smem S;
   load_params<false>(S, P, d);
   __syncthreads();
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t r = bid_x() * bdim_x() + tid_x();
   if (r >= n) return;
   float xn[DMAX], h1[H], h2[H];
   load_features(x + (size_t)r * d, d, xn);
@@ -167,7 +169,7 @@ __global__ __launch_bounds__(256) void fwd_bwd_kernel(const float* __restrict__ 
 __global__ __launch_bounds__(256) void wgrad_kernel(const float* __restrict__ act, uint32_t n, uint32_t d,
                                                     float* __restrict__ grad, float* __restrict__ loss) {
   const uint32_t P = (uint32_t)param_count(d);
-  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t p = bid_x() * bdim_x() + tid_x();
   if (p > P) return;
   // offsets inside a row of act
   const int XN = 0, H1 = DMAX, H2 = DMAX + H, DY = DMAX + 2 * H, DH2 = DY + 1, DH1 = DH2 + H, SQ = DH1 + H;
@@ -209,7 +211,7 @@ __global__ __launch_bounds__(256) void wgrad_kernel(const float* __restrict__ ac
 
 __global__ void adam_kernel(uint32_t count, float* __restrict__ P, const float* __restrict__ g, float* __restrict__ m,
                             float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = bid_x() * bdim_x() + tid_x();
   if (i >= count) return;
   const float gi = g[i];
   const float mi = b1 * m[i] + (1.0f - b1) * gi;

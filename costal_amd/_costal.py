@@ -150,7 +150,7 @@ def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None
         "candidates": int(st.candidates),
         "col_splits": int(st.col_splits),
         "short_rows": int(st.short_rows),
-        "path": "fused" if st.path == 0 else "rowscan",
+        "path": {0: "fused", 1: "rowscan", 2: "wide"}.get(int(st.path), str(st.path)),
         "inserts": int(st.inserts),
         "rowscan_rows": int(st.rowscan_rows),
         "device_ms": float(st.device_ms),

@@ -152,12 +152,82 @@ __global__ void prep_kernel(const float* __restrict__ x, uint32_t n, uint32_t d,
   }
 }
 
+// prep for long rows (f32 working copy only): the same norms and copies as prep_kernel, for 32
+// rows per workgroup.  The rows come through LDS in coalesced 64-feature chunks (the next chunk in
+// flight while the current one is summed), and thread r sums row r's squares in feature order, so
+// the norms are bit-identical; the normalized copy is written straight from global, coalesced.
+constexpr int PREP_ROWS = 32;
+constexpr int PREP_CF = 64;
+
+__global__ __launch_bounds__(256) void prep_wide_kernel(const float* __restrict__ x, uint32_t n, uint32_t d,
+                                                        uint32_t n_pad, uint32_t dp32, double* __restrict__ norms,
+                                                        float* __restrict__ xs32, uint32_t* __restrict__ bad_row) {
+  constexpr int PER = PREP_ROWS * PREP_CF / 256;  // elements per thread per chunk
+  __shared__ uint32_t tile[2][PREP_ROWS][PREP_CF + 1];  // the rows' bits (the non-finite test reads bits)
+  __shared__ double nrm_s[PREP_ROWS];
+  const uint32_t t = tid_x(), r0 = bid_x() * PREP_ROWS;
+  const uint32_t nr = r0 < n ? (n - r0 < (uint32_t)PREP_ROWS ? n - r0 : (uint32_t)PREP_ROWS) : 0;
+  const uint32_t* xb = reinterpret_cast<const uint32_t*>(x);
+  const uint32_t nch = (d + PREP_CF - 1) / PREP_CF;
+  uint32_t reg[PER];
+  auto load = [&](uint32_t c0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const uint32_t e = t + 256 * m, r = e / PREP_CF, f = e % PREP_CF;
+      reg[m] = (r < nr && c0 + f < d) ? xb[(size_t)(r0 + r) * d + c0 + f] : 0u;
+    }
+  };
+  auto store = [&](int b) __attribute__((always_inline)) {
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const uint32_t e = t + 256 * m;
+      tile[b][e / PREP_CF][e % PREP_CF] = reg[m];
+    }
+  };
+  double s = 0.0;
+  bool bad = false;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (uint32_t c = 0; c < nch; ++c) {
+    if (c + 1 < nch) load((c + 1) * PREP_CF);
+    if (t < nr) {
+      const uint32_t c0 = c * PREP_CF;
+      const uint32_t m = d - c0 < (uint32_t)PREP_CF ? d - c0 : (uint32_t)PREP_CF;
+      for (uint32_t f = 0; f < m; ++f) {
+        const uint32_t b = tile[c & 1][t][f];
+        bad |= (b & 0x7f800000u) == 0x7f800000u;  // inf or NaN
+        const float v = __uint_as_float(b);
+        s += (double)v * (double)v;
+      }
+    }
+    if (c + 1 < nch) store((c + 1) & 1);
+    __syncthreads();
+  }
+  if (t < PREP_ROWS) {
+    const double nrm = sqrt(s);
+    nrm_s[t] = t < nr ? nrm : 0.0;
+    if (t < nr) {
+      norms[r0 + t] = nrm;
+      if (bad) atomicMin(bad_row, r0 + t);
+    }
+  }
+  __syncthreads();
+  for (uint32_t e = t; e < (uint32_t)PREP_ROWS * dp32; e += 256) {
+    const uint32_t r = e / dp32, f = e % dp32;
+    if (r0 + r >= n_pad) break;
+    const double nr_ = nrm_s[r];
+    xs32[(size_t)(r0 + r) * dp32 + f] =
+        (r < nr && f < d && nr_ > 0.0) ? (float)((double)__uint_as_float(xb[(size_t)(r0 + r) * d + f]) / nr_) : 0.0f;
+  }
+}
+
 // ------------------------------------------------------------------------------------------------
 // coarse_kernel
 //
 // Workgroup = NW waves; each wave owns 32 queries, held in registers for the whole launch.  The
 // workgroup streams its candidate range through two LDS buffers of TC-row tiles (16 KB each),
-// staged global -> registers -> LDS two tiles ahead of the compute.  For every 32-candidate
+// staged global -> registers -> LDS one tile ahead of the compute.  For every 32-candidate
 // sub-tile a wave issues the MFMAs for D[cand][query] = sum_k cand[k] * query[k]: lane l owns query
 // (l % 32) and the 16 candidates 8*(i/4) + 4*(l/32) + (i%4), i = 0..15 (the 32x32 accumulator
 // layout).  The scan takes the max of a step's sub-tiles (max3 chains) and compares once with the
@@ -196,9 +266,16 @@ __device__ __forceinline__ float pick16(const f32x16& a, int i) {
   return b0 ? e1 : e0;
 }
 
-// Sorted (ascending) register list: ls[0] is the smallest live score.  Entries [kp, KPR) hold +inf
-// and never move; empty entries hold -inf.  push() replaces the minimum and bubbles the new entry
-// up to its place.
+// The value held by the lane of the other half of the wave (lane l ^ 32): the other list of the
+// same query.
+__device__ __forceinline__ float partner32(float x) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  return __uint_as_float((__builtin_amdgcn_workitem_id_x() & 32) ? r[0] : r[1]);
+}
+
+// Sorted (ascending) register list: ls[0] is the smallest score.  Entries [kp, KPR) hold +inf and
+// never move; empty entries hold the admission threshold.  push() replaces the minimum and bubbles
+// the new entry up to its place.
 template <int KPR>
 __device__ __forceinline__ void list_push(float (&ls)[KPR], uint32_t (&li)[KPR], float v, uint32_t c) {
   ls[0] = v;
@@ -225,8 +302,7 @@ constexpr int PREC_F16X3 = COSTAL_PREC_FP16X3;
 
 // The 32x32 product of one sub-tile: b = the lane's J candidate chunks, q = its J query chunks.
 template <int PREC, int J>
-__device__ __forceinline__ f32x16 tile_mfma(const uint4 (&b)[J], const uint4 (&q)[J]) {
-  f32x16 acc = {};
+__device__ __forceinline__ f32x16 tile_mfma(const uint4 (&b)[J], const uint4 (&q)[J], f32x16 acc = {}) {
   if constexpr (PREC == PREC_F32) {
 #pragma unroll
     for (int j = 0; j < J; ++j) {
@@ -313,16 +389,19 @@ __device__ __forceinline__ void scan_block(
 #pragma unroll
     for (int j = 0; j < J; ++j) qreg[j] = *reinterpret_cast<const uint4*>(qrow + (2 * j + h) * 16);
   }
-  const float t0 = (tinit && qvalid) ? tinit[qg] : -INFINITY;
+  // Empty list entries hold the lane's admission threshold (-inf without one; +inf for padding
+  // queries, which so never take the insert path): the list minimum ls[0] is then always the
+  // lane's threshold, and real entries (score above it) are the ones with an index.
+  const float t0 = qvalid ? (tinit ? tinit[qg] : -INFINITY) : INFINITY;
   float ls[KPR];
   uint32_t li[KPR];
 #pragma unroll
   for (int e = 0; e < KPR; ++e) {
-    ls[e] = e < kp ? -INFINITY : INFINITY;
+    ls[e] = e < kp ? t0 : INFINITY;
     li[e] = NONE;
   }
-  // padding queries get an infinite threshold so they never take the insert path
-  float th = qvalid ? fmaxf(t0, ls[0]) : INFINITY;
+  float th = ls[0];
+  float thp = -INFINITY;  // the partner list's threshold, refreshed once per tile
   uint32_t my_inserts = 0;
 
   const uint32_t c_begin = split * cols_per_split;
@@ -387,7 +466,7 @@ __device__ __forceinline__ void scan_block(
     float m = mu[0];
 #pragma unroll
     for (int u = 1; u < GS; ++u) m = fmaxf(m, mu[u]);
-    float the = (PREC == PREC_F16) ? th - eps_pre : th;
+    float the = (PREC == PREC_F16) ? fmaxf(th, thp) - eps_pre : fmaxf(th, thp);
     if (__builtin_amdgcn_ballot_w64(m > the) != 0) {
 #pragma unroll
       for (int u = 0; u < GS; ++u) {
@@ -412,8 +491,8 @@ __device__ __forceinline__ void scan_block(
                 if (!(v > th)) continue;
               }
               list_push<KPR>(ls, li, v, cand);
-              th = fmaxf(t0, ls[0]);
-              the = (PREC == PREC_F16) ? th - eps_pre : th;
+              th = ls[0];
+              the = (PREC == PREC_F16) ? fmaxf(th, thp) - eps_pre : fmaxf(th, thp);
               ++my_inserts;
             }
           }
@@ -425,6 +504,7 @@ __device__ __forceinline__ void scan_block(
 #define RUN_TILE(CUR, OTH)                                                                        \
   _Pragma("unroll") for (int p = 0; p < NSTEP; ++p) {                                             \
     MFMA_STEP((p + 1) & 1)                                                                        \
+    if (p == 0) thp = partner32(th);                                                              \
     if (p == NSTEP - 2) {                                                                         \
       STORE_REGS(ra, OTH);                                                                        \
       TILE_BARRIER();                                                                             \
@@ -471,20 +551,21 @@ __device__ __forceinline__ void scan_block(
     for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
     if (l == 0) atomicAdd(inserts, (unsigned long long)tot);
   }
+  const float thp_end = partner32(th);
   if (qvalid) {
     const size_t slot = ((size_t)qq * nsplit + split) * 2 + h;
     const size_t base = slot * (size_t)kp;
     uint32_t c = 0;
 #pragma unroll
     for (int e = 0; e < KPR; ++e) {
-      if (e < kp && ls[e] > -INFINITY) {
+      if (e < kp && li[e] != NONE) {
         cand_idx[base + c] = li[e];
         cand_sc[base + c] = ls[e];
         ++c;
       }
     }
     cand_cnt[slot] = c;
-    cand_floor[slot] = ((int)c == kp) ? th : t0;
+    cand_floor[slot] = fmaxf(th, thp_end);  // everything this list turned away scored at most this
   }
 }
 
@@ -498,6 +579,177 @@ __global__ __launch_bounds__(64 * NW) void coarse_kernel(
   __shared__ __attribute__((aligned(16))) uint8_t tile1[TileShape<J>::TILEB];
   scan_block<PREC, J, KPR, NW>(tile0, tile1, xsq, xs32, dp32, n, kp, rowlist, nq, bid_x(), tinit, bid_y(),
                                gdim_y(), cols_per_split, eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor, inserts);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Wide rows (more features than the fused scan keeps in registers): two kernels per block of query
+// rows.  wide_gemm_kernel computes the f32 scores of the block's queries against every candidate
+// with FP32 MFMAs over 32-feature chunks staged through LDS, and stores each 32 x 32 block of
+// scores in the accumulator layout (lane-major, 4 KB per block); wide_scan_kernel reads the blocks
+// back, one 64-byte run per lane, and keeps the same per-lane candidate lists as the fused scan, so
+// the rescoring, the certificate and the fallbacks are shared.  The f32 working copy has rows of a
+// multiple of 32 features (zero padded) for this path.
+//
+// Score block (qs, cs) = queries qs*32 .. +32 of the block, candidates cs*32 .. +32, at floats
+// ((qs * ncs) + cs) * 1024; lane l holds query l % 32 and candidates 8*(v/4) + 4*(l/32) + (v%4) at
+// l*16 + v, v = 0..15.
+
+constexpr int WIDE_KC = 32;   // features per chunk: J = 4 sixteen-byte slots per lane half
+constexpr int WIDE_QT = 64;   // queries per gemm workgroup (2 sub-tiles)
+constexpr int WIDE_CT = 128;  // candidates per gemm workgroup (4 sub-tiles)
+constexpr int WIDE_QB = 128;  // queries per scan workgroup (4 waves)
+
+__global__ __launch_bounds__(256) void wide_gemm_kernel(const float* __restrict__ xs32, uint32_t dp, uint32_t q0,
+                                                        uint32_t ncs, float* __restrict__ S) {
+  constexpr int J = WIDE_KC / 8;
+  constexpr int ROWB = WIDE_KC * 4;
+  __shared__ __attribute__((aligned(16))) uint8_t lq[2][WIDE_QT * ROWB];
+  __shared__ __attribute__((aligned(16))) uint8_t lc[2][WIDE_CT * ROWB];
+  const int tid = tid_x(), w = tid >> 6, l = tid & 63, h = l >> 5, l32 = l & 31;
+  const int wq = w & 1, wc = w >> 1;  // the wave's query sub-tile and candidate pair of sub-tiles
+  const uint8_t* qsrc = reinterpret_cast<const uint8_t*>(xs32 + (size_t)(q0 + bid_x() * WIDE_QT) * dp);
+  const uint8_t* csrc = reinterpret_cast<const uint8_t*>(xs32 + (size_t)(bid_y() * WIDE_CT) * dp);
+  const uint32_t rowbytes = dp * 4, nchunk = dp / WIDE_KC;
+  // LDS slot e = ((sub*J + j)*2 + half)*32 + r holds 16-byte slot 2j + half of the chunk of row
+  // sub*32 + r; thread t stages slots t + 256 i (conflict-free stores and lane reads)
+  auto src_off = [&](int e, uint32_t c) -> uint32_t {
+    const int r = e & 31, hh = (e >> 5) & 1, j = (e >> 6) % J, sub = (e >> 6) / J;
+    return (uint32_t)(sub * 32 + r) * rowbytes + c * ROWB + (uint32_t)(2 * j + hh) * 16;
+  };
+  uint4 rq0, rq1, rc0, rc1, rc2, rc3;
+  auto load = [&](uint32_t c) __attribute__((always_inline)) {
+    rq0 = *reinterpret_cast<const uint4*>(qsrc + src_off(tid, c));
+    rq1 = *reinterpret_cast<const uint4*>(qsrc + src_off(tid + 256, c));
+    rc0 = *reinterpret_cast<const uint4*>(csrc + src_off(tid, c));
+    rc1 = *reinterpret_cast<const uint4*>(csrc + src_off(tid + 256, c));
+    rc2 = *reinterpret_cast<const uint4*>(csrc + src_off(tid + 512, c));
+    rc3 = *reinterpret_cast<const uint4*>(csrc + src_off(tid + 768, c));
+  };
+  auto store = [&](int b) __attribute__((always_inline)) {
+    reinterpret_cast<uint4*>(lq[b])[tid] = rq0;
+    reinterpret_cast<uint4*>(lq[b])[tid + 256] = rq1;
+    reinterpret_cast<uint4*>(lc[b])[tid] = rc0;
+    reinterpret_cast<uint4*>(lc[b])[tid + 256] = rc1;
+    reinterpret_cast<uint4*>(lc[b])[tid + 512] = rc2;
+    reinterpret_cast<uint4*>(lc[b])[tid + 768] = rc3;
+  };
+  f32x16 acc0 = {}, acc1 = {};
+  auto compute = [&](int b) __attribute__((always_inline)) {
+    const uint4* Q = reinterpret_cast<const uint4*>(lq[b]);
+    const uint4* Cc = reinterpret_cast<const uint4*>(lc[b]);
+    uint4 q[J], b0[J], b1[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      q[j] = Q[((wq * J + j) * 2 + h) * 32 + l32];
+      b0[j] = Cc[(((2 * wc) * J + j) * 2 + h) * 32 + l32];
+      b1[j] = Cc[(((2 * wc + 1) * J + j) * 2 + h) * 32 + l32];
+    }
+    acc0 = tile_mfma<PREC_F32, J>(b0, q, acc0);
+    acc1 = tile_mfma<PREC_F32, J>(b1, q, acc1);
+  };
+  load(0);
+  store(0);
+  __syncthreads();
+  for (uint32_t c = 0; c < nchunk; ++c) {
+    if (c + 1 < nchunk) load(c + 1);
+    compute(c & 1);
+    if (c + 1 < nchunk) store((c + 1) & 1);
+    __syncthreads();
+  }
+  const uint32_t qs = bid_x() * 2 + wq, cs = bid_y() * 4 + 2 * wc;
+  float4* o0 = reinterpret_cast<float4*>(S + ((size_t)qs * ncs + cs) * 1024 + l * 16);
+  float4* o1 = reinterpret_cast<float4*>(S + ((size_t)qs * ncs + cs + 1) * 1024 + l * 16);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    o0[v] = make_float4(acc0[4 * v], acc0[4 * v + 1], acc0[4 * v + 2], acc0[4 * v + 3]);
+    o1[v] = make_float4(acc1[4 * v], acc1[4 * v + 1], acc1[4 * v + 2], acc1[4 * v + 3]);
+  }
+}
+
+// The per-lane lists of the fused scan over stored score blocks: queries q0 + [0, nq) (the block),
+// candidate sub-tiles of split bid_y(); lists go to the slots of the queries' rows.
+template <int KPR>
+__global__ __launch_bounds__(256) void wide_scan_kernel(
+    const float* __restrict__ S, uint32_t q0, uint32_t nq, uint32_t ncs, uint32_t n, int kp, uint32_t nsplit,
+    uint32_t cs_per_split, uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc,
+    uint32_t* __restrict__ cand_cnt, float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
+  const int tid = tid_x(), w = tid >> 6, l = tid & 63, h = l >> 5, l32 = l & 31;
+  const uint32_t qs = bid_x() * 4 + w;
+  const uint32_t ql = qs * 32 + l32;
+  const bool qvalid = ql < nq;
+  const uint32_t qg = q0 + (qvalid ? ql : 0);
+  const uint32_t split = bid_y();
+  const uint32_t cs0 = split * cs_per_split;
+  const uint32_t cs1 = cs0 + cs_per_split < ncs ? cs0 + cs_per_split : ncs;
+  const uint32_t c_end = cs1 * 32 < n ? cs1 * 32 : n;
+  float ls[KPR];
+  uint32_t li[KPR];
+#pragma unroll
+  for (int e = 0; e < KPR; ++e) {
+    ls[e] = e < kp ? (qvalid ? -INFINITY : INFINITY) : INFINITY;
+    li[e] = NONE;
+  }
+  float th = ls[0];
+  uint32_t my_inserts = 0;
+  if (qs * 32 < nq) {  // the whole wave's queries lie beyond the block otherwise
+    const float4* blk = reinterpret_cast<const float4*>(S + (size_t)qs * ncs * 1024 + l * 16);
+    f32x16 nxt;
+    auto fetch = [&](uint32_t cs) __attribute__((always_inline)) {
+      const float4* src = blk + (size_t)cs * 256;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const float4 f = src[v];
+        nxt[4 * v] = f.x;
+        nxt[4 * v + 1] = f.y;
+        nxt[4 * v + 2] = f.z;
+        nxt[4 * v + 3] = f.w;
+      }
+    };
+    if (cs0 < cs1) fetch(cs0);
+    for (uint32_t cs = cs0; cs < cs1; ++cs) {
+      const f32x16 a = nxt;
+      if (cs + 1 < cs1) fetch(cs + 1);
+      const float m = max16(a);
+      if (__builtin_amdgcn_ballot_w64(m > th) != 0 && m > th) {
+        uint32_t mask = 0;
+#pragma unroll
+        for (int i = 15; i >= 0; --i) mask = (mask << 1) | (a[i] > th ? 1u : 0u);
+        const uint32_t cbase = cs * 32 + 4 * h;
+        while (mask) {
+          const int i = __builtin_ctz(mask);
+          mask &= mask - 1;
+          const float v = pick16(a, i);
+          const uint32_t cand = cbase + 8 * (i >> 2) + (i & 3);
+          if (v > th && cand < c_end && cand != qg) {
+            list_push<KPR>(ls, li, v, cand);
+            th = ls[0];
+            ++my_inserts;
+          }
+        }
+      }
+    }
+  }
+  if (inserts) {
+    uint32_t tot = my_inserts;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+    if (l == 0) atomicAdd(inserts, (unsigned long long)tot);
+  }
+  if (qvalid) {
+    const size_t slot = ((size_t)qg * nsplit + split) * 2 + h;
+    const size_t base = slot * (size_t)kp;
+    uint32_t c = 0;
+#pragma unroll
+    for (int e = 0; e < KPR; ++e) {
+      if (e < kp && li[e] != NONE) {
+        cand_idx[base + c] = li[e];
+        cand_sc[base + c] = ls[e];
+        ++c;
+      }
+    }
+    cand_cnt[slot] = c;
+    cand_floor[slot] = th;
+  }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -987,6 +1239,8 @@ constexpr uint32_t ROWSCAN_K_MAX = 2048;
 
 struct Plan {
   bool fused;
+  bool wide;        // wide rows: score blocks (wide_gemm_kernel) + wide_scan_kernel
+  uint32_t wide_qb; // query rows per score block (wide)
   int prec;
   uint32_t J;       // 16-byte chunks per lane half per row
   uint32_t kp;      // K'
@@ -1003,7 +1257,8 @@ typedef void (*CoarseFn)(const uint8_t*, const float*, uint32_t, uint32_t, int, 
 
 template <int PREC, int J, int KP, int NW>
 CoarseFn coarse_ptr() {
-  return coarse_kernel<PREC, J, KP, NW>;
+  if constexpr (PREC == PREC_F16X3 && J % 2) return nullptr;  // split rows hold hi and lo halves
+  else return coarse_kernel<PREC, J, KP, NW>;
 }
 
 template <int PREC, int KP, int NW>
@@ -1065,11 +1320,12 @@ double fp16x3_eps(uint32_t d) {
 
 // Capacity of one candidate list.  A query's candidates are split over two lists (the two lanes of
 // its lane pair, each seeing half of every sub-tile) and over the candidate splits; with K' = k +
-// margin candidates per row (the reference's heap size), each list keeps k + ceil(margin / 2): all
-// of a row's k nearest fit in one list, and the lists together hold more than K'.  The capacity only
-// trades inserts against fallbacks; the certificate makes every choice exact.
+// margin candidates per row (the reference's heap size), each list keeps max(k, ceil(K' / 2)): the
+// two lists of a row together hold at least K', and all of its k nearest fit in either one (the
+// default margin k gives lists of k).  The capacity only trades inserts against fallbacks; the
+// certificate makes every choice exact.
 uint32_t list_capacity(uint64_t n, uint64_t k, uint64_t margin) {
-  return (uint32_t)std::min<uint64_t>(k + (margin + 1) / 2, n - 1);
+  return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(k, (k + margin + 1) / 2), n - 1);
 }
 
 int list_template(uint32_t kp) { return kp <= 16 ? 16 : kp <= 32 ? 32 : 64; }
@@ -1102,6 +1358,27 @@ Plan make_plan(const Device& D, uint64_t n, uint64_t d, uint64_t k, const costal
   P.cols_per_split = tiles_per * P.TC;
   P.nsplit = (ntile + tiles_per - 1) / tiles_per;
   if (P.nsplit < 1) P.nsplit = 1;
+  P.wide = !P.fused && p.select == COSTAL_SELECT_COSINE && P.kp <= FUSED_KP_MAX;
+  if (P.wide) {
+    // rows of a multiple of 32 features; score blocks of up to 512 MB; the scan's candidate
+    // sub-tiles split so that about four workgroups per CU run
+    P.dp32 = round_up((uint32_t)std::max<uint64_t>(d, 1), WIDE_KC);
+    P.QB = WIDE_QB;
+    P.TC = 32;
+    const uint64_t row_bytes = (uint64_t)P.n_pad * 4;
+    const uint64_t fit = std::max<uint64_t>(256, (512ull << 20) / row_bytes / 256 * 256);
+    P.wide_qb = (uint32_t)std::min<uint64_t>(P.n_pad, fit);
+    const uint32_t ncs = P.n_pad / 32;
+    const uint32_t blocks = (std::min<uint32_t>(P.wide_qb, (uint32_t)n) + WIDE_QB - 1) / WIDE_QB;
+    // every split adds 2 kp candidates per row to the f64 rescoring, which costs d multiply-adds
+    // each, while a split's scan only reads stored scores: few splits
+    uint32_t ws = p.col_splits ? p.col_splits : (4u * (uint32_t)std::max(D.cus, 1) + blocks - 1) / blocks;
+    ws = std::min<uint32_t>(ws, p.col_splits ? 2048 / (2 * P.kp) : 2);
+    ws = std::max<uint32_t>(1, std::min<uint32_t>(ws, ncs));
+    const uint32_t per = (ncs + ws - 1) / ws;
+    P.cols_per_split = per;  // candidate sub-tiles per split
+    P.nsplit = (ncs + per - 1) / per;
+  }
   return P;
 }
 
@@ -1150,12 +1427,18 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   if (p.precision > COSTAL_PREC_FP16X3) return fail(COSTAL_E_INVALID, "unknown precision %u", p.precision);
   if (p.select > COSTAL_SELECT_TANIMOTO) return fail(COSTAL_E_INVALID, "unknown select %u", p.select);
   Plan P = make_plan(D, n, d, k, p);
-  if (brute) P.fused = false;
+  if (brute) P.fused = P.wide = false;
   const bool need16 = P.fused && (P.prec == PREC_F16 || P.prec == PREC_F16X3);
   const int split16 = P.prec == PREC_F16X3 ? 1 : 0;
   const size_t row16 = (size_t)P.dp16 * (split16 ? 2 : 1);
   // two candidate lists per query and split (one per lane of the query's lane pair)
-  const size_t nq_slots = P.fused ? (size_t)n * P.nsplit * 2 : 0;
+  const size_t nq_slots = (P.fused || P.wide) ? (size_t)n * P.nsplit * 2 : 0;
+  // list capacity of the rescan of first-pass failures; the list buffers hold both passes' lists
+  // (the rescan keeps at most 2 lists of kp2 per failed row per split, and at least one split fits)
+  const uint32_t kp2 = (uint32_t)std::min<uint64_t>(
+      std::max<uint64_t>(P.kp, std::min<uint64_t>((uint64_t)k + (k + 1) / 2, FUSED_KP_MAX)), n - 1);
+  const size_t nq_entries = P.fused ? std::max<size_t>(nq_slots * P.kp, (size_t)n * 2 * kp2) : nq_slots * P.kp;
+  const size_t score_floats = P.wide ? (size_t)P.wide_qb * P.n_pad : 0;
 
   size_t bytes = 0;
   {
@@ -1163,8 +1446,8 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     c.take<float>((size_t)P.n_pad * P.dp32);
     if (need16) c.take<_Float16>((size_t)P.n_pad * row16);
     c.take<double>(n);
-    c.take<uint32_t>(nq_slots * P.kp);
-    c.take<float>(nq_slots * P.kp);
+    c.take<uint32_t>(nq_entries);
+    c.take<float>(nq_entries);
     c.take<uint32_t>(nq_slots);
     c.take<float>(nq_slots);
     c.take<uint32_t>(n);
@@ -1173,6 +1456,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     c.take<double>(n);
     c.take<uint32_t>(64);
     if (P.fused && p.mlp_params && !p.row_threshold) c.take<float>(n);
+    c.take<float>(score_floats);
     bytes = c.off + 256;
   }
   int rc = ws_reserve(D, bytes);
@@ -1181,8 +1465,8 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   float* xs32 = c.take<float>((size_t)P.n_pad * P.dp32);
   _Float16* xs16 = need16 ? c.take<_Float16>((size_t)P.n_pad * row16) : nullptr;
   double* norms_ws = c.take<double>(n);
-  uint32_t* cand_idx = c.take<uint32_t>(nq_slots * P.kp);
-  float* cand_sc = c.take<float>(nq_slots * P.kp);
+  uint32_t* cand_idx = c.take<uint32_t>(nq_entries);
+  float* cand_sc = c.take<float>(nq_entries);
   uint32_t* cand_cnt = c.take<uint32_t>(nq_slots);
   float* cand_floor = c.take<float>(nq_slots);
   uint32_t* fb_rows = c.take<uint32_t>(n);
@@ -1195,6 +1479,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   uint32_t* counters = c.take<uint32_t>(64);
   const bool learned = P.fused && p.mlp_params && !p.row_threshold;
   float* th_ws = learned ? c.take<float>(n) : nullptr;
+  float* scores = P.wide ? c.take<float>(score_floats) : nullptr;
   double* norms = norms_out ? norms_out : norms_ws;
 
   HIP_TRY(hipEventRecord(D.ev0, s));
@@ -1203,8 +1488,12 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     HIP_TRY(hipMemcpyAsync(counters, init, sizeof init, hipMemcpyHostToDevice, s));
   }
   HIP_TRY(hipEventRecord(D.evk[0], s));
-  prep_kernel<<<(P.n_pad + 255) / 256, 256, 0, s>>>(x, n, d, P.n_pad, P.dp32, P.dp16, split16, norms, xs32,
-                                                    xs16, counters);
+  if (d > 64 && !need16)
+    prep_wide_kernel<<<(P.n_pad + PREP_ROWS - 1) / PREP_ROWS, 256, 0, s>>>(x, n, d, P.n_pad, P.dp32, norms, xs32,
+                                                                           counters);
+  else
+    prep_kernel<<<(P.n_pad + 255) / 256, 256, 0, s>>>(x, n, d, P.n_pad, P.dp32, P.dp16, split16, norms, xs32,
+                                                      xs16, counters);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(D.evk[1], s));
   uint32_t bad = NONE;
@@ -1213,6 +1502,27 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   if (bad != NONE) return fail(COSTAL_E_NON_FINITE, "row %u contains NaN or inf", bad);
 
   const double eps32 = fp32_eps(d);
+  unsigned long long* inserts = reinterpret_cast<unsigned long long*>(counters + 8);
+  // f64 rescoring of the candidate lists of the queries `rows` (null: every row) with the
+  // certificate; rows that fail it are appended to (fbr, fbl) and counted in *fbc
+  auto rescore = [&](uint32_t kp, uint32_t nsplit, const uint32_t* rows, uint32_t nq, int all_admitted,
+                     double eps_list, uint32_t* fbr, double* fbl, uint32_t* fbc, uint32_t* shc) -> int {
+    const uint32_t nlists = 2 * nsplit;
+    const uint32_t slots = nlists * kp;
+#define RESCORE(CM)                                                                                    \
+  rescore_kernel<CM><<<nq, 64, 0, s>>>(x, d, norms, n, k, (int)kp, nlists, rows, nq, cand_idx, cand_cnt,     \
+                                       cand_floor, all_admitted, eps_list, eps32, idx, cos, wts, fbr, fbl,  \
+                                       fbc, shc)
+    if (slots <= 64) RESCORE(64);
+    else if (slots <= 128) RESCORE(128);
+    else if (slots <= 256) RESCORE(256);
+    else if (slots <= 512) RESCORE(512);
+    else if (slots <= 1024) RESCORE(1024);
+    else RESCORE(2048);
+#undef RESCORE
+    HIP_TRY(hipGetLastError());
+    return 0;
+  };
   if (P.fused) {
     CoarseFn fn = coarse_select(P.prec, P.J, P.KP);
     if (!fn) return fail(COSTAL_E_UNSUPPORTED, "no coarse kernel for d=%u", d);
@@ -1228,64 +1538,82 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
       if (rc) return fail(rc, "threshold model: %s", costal_mlp_last_error());
       row_th = th_ws;
     }
-    unsigned long long* inserts = reinterpret_cast<unsigned long long*>(counters + 8);
     const uint32_t ntile = (n + P.TC - 1) / P.TC;
     // one scan pass over the queries `rows` (null: every row), then rescoring; rows that fail the
     // certificate are appended to (fbr, fbl) and counted in *fbc
-    auto scan_pass = [&](const uint32_t* rows, uint32_t nq, const float* th, uint32_t nsplit, uint32_t cols_per,
-                         uint32_t* fbr, double* fbl, uint32_t* fbc, uint32_t* shc) -> int {
+    auto scan_pass = [&](CoarseFn f, uint32_t kp, const uint32_t* rows, uint32_t nq, const float* th,
+                         uint32_t nsplit, uint32_t cols_per, uint32_t* fbr, double* fbl, uint32_t* fbc,
+                         uint32_t* shc) -> int {
       dim3 grid((nq + P.QB - 1) / P.QB, nsplit);
-      fn<<<grid, 64 * P.NW, 0, s>>>(xsq, xs32, P.dp32, n, (int)P.kp, rows, nq, th, cols_per, eps_pre, cand_idx,
-                                    cand_sc, cand_cnt, cand_floor, inserts);
+      f<<<grid, 64 * P.NW, 0, s>>>(xsq, xs32, P.dp32, n, (int)kp, rows, nq, th, cols_per, eps_pre, cand_idx,
+                                   cand_sc, cand_cnt, cand_floor, inserts);
       HIP_TRY(hipGetLastError());
       if (!rows) HIP_TRY(hipEventRecord(D.evk[3], s));
-      const int all_admitted = (nsplit == 1 && P.kp >= n - 1 && !th) ? 1 : 0;
-      const uint32_t nlists = 2 * nsplit;
-      const uint32_t slots = nlists * P.kp;
-#define RESCORE(CM)                                                                                    \
-  rescore_kernel<CM><<<nq, 64, 0, s>>>(x, d, norms, n, k, (int)P.kp, nlists, rows, nq, cand_idx, cand_cnt,  \
-                                       cand_floor, all_admitted, eps_list, eps32, idx, cos, wts, fbr, fbl,  \
-                                       fbc, shc)
-      if (slots <= 64) RESCORE(64);
-      else if (slots <= 128) RESCORE(128);
-      else if (slots <= 256) RESCORE(256);
-      else if (slots <= 512) RESCORE(512);
-      else if (slots <= 1024) RESCORE(1024);
-      else RESCORE(2048);
-#undef RESCORE
-      HIP_TRY(hipGetLastError());
-      return 0;
+      const int all_admitted = (nsplit == 1 && kp >= n - 1 && !th) ? 1 : 0;
+      return rescore(kp, nsplit, rows, nq, all_admitted, eps_list, fbr, fbl, fbc, shc);
     };
     HIP_TRY(hipEventRecord(D.evk[2], s));
-    rc = scan_pass(nullptr, n, row_th, P.nsplit, P.cols_per_split, fb_rows, fb_lo, counters + 1, counters + 2);
+    rc = scan_pass(fn, P.kp, nullptr, n, row_th, P.nsplit, P.cols_per_split, fb_rows, fb_lo, counters + 1,
+                   counters + 2);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(D.evk[4], s));
     const uint32_t grid_fb = std::min<uint32_t>(n, 4u * (uint32_t)std::max(D.cus, 1));
-    if (row_th) {
-      // Rows that failed under admission thresholds are rescanned without them, as one batched
-      // scan over the row list (wide candidate splits keep the few rows' scan parallel); only rows
-      // that fail again (exact ties) go to the full-row kernel.
-      uint32_t nfb = 0;
-      HIP_TRY(hipMemcpyAsync(&nfb, counters + 1, sizeof nfb, hipMemcpyDeviceToHost, s));
-      HIP_TRY(hipStreamSynchronize(s));
-      if (nfb > 0) {
-        const uint32_t blocks = (nfb + P.QB - 1) / P.QB;
-        uint32_t ns = (4u * (uint32_t)std::max(D.cus, 1) + blocks - 1) / blocks;
-        ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, ntile / 2));
-        ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, 2048 / (2 * P.kp)));
-        ns = std::min<uint64_t>(ns, std::max<uint64_t>(1, (uint64_t)n * P.nsplit / nfb));
-        ns = std::max<uint32_t>(ns, 1);
-        const uint32_t tiles_per = (ntile + ns - 1) / ns;
-        ns = (ntile + tiles_per - 1) / tiles_per;
-        rc = scan_pass(fb_rows, nfb, nullptr, ns, tiles_per * P.TC, fb2_rows, fb2_lo, counters + 3, counters + 4);
-        if (rc) return rc;
-      }
-      rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb2_rows,
-                          counters + 3, 0, fb2_lo, eps32, idx, cos, wts, grid_fb);
-    } else {
-      rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb_rows,
-                          counters + 1, 0, fb_lo, eps32, idx, cos, wts, grid_fb);
+    // Rows that failed the first pass (admission thresholds above their k-th similarity, or lists
+    // too short for their ties) are rescanned without thresholds and with longer lists, as one
+    // batched scan over the row list (wide candidate splits keep the few rows' scan parallel); only
+    // rows that fail again (exact ties) go to the full-row kernel.
+    uint32_t nfb = 0;
+    HIP_TRY(hipMemcpyAsync(&nfb, counters + 1, sizeof nfb, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (nfb > 0) {
+      CoarseFn fn2 = coarse_select(P.prec, P.J, list_template(kp2));
+      if (!fn2) return fail(COSTAL_E_UNSUPPORTED, "no coarse kernel for d=%u", d);
+      // about one workgroup per CU: more splits would multiply the candidates each row rescores
+      const uint32_t blocks = (nfb + P.QB - 1) / P.QB;
+      uint32_t ns = ((uint32_t)std::max(D.cus, 1) + blocks - 1) / blocks;
+      ns = std::min<uint32_t>(ns, 16);
+      ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, ntile / 2));
+      ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, 2048 / (2 * kp2)));
+      // the rescan's lists fit in the list buffers (nq_entries entries, nq_slots lists; one split
+      // always fits since nfb <= n)
+      ns = std::min<uint64_t>(ns, nq_entries / ((uint64_t)nfb * 2 * kp2));
+      ns = std::min<uint64_t>(ns, nq_slots / ((uint64_t)nfb * 2));
+      ns = std::max<uint32_t>(ns, 1);
+      const uint32_t tiles_per = (ntile + ns - 1) / ns;
+      ns = (ntile + tiles_per - 1) / tiles_per;
+      rc = scan_pass(fn2, kp2, fb_rows, nfb, nullptr, ns, tiles_per * P.TC, fb2_rows, fb2_lo, counters + 3,
+                     counters + 4);
+      if (rc) return rc;
     }
+    rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb2_rows, counters + 3,
+                        0, fb2_lo, eps32, idx, cos, wts, grid_fb);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(D.evk[5], s));
+  } else if (P.wide) {
+    const uint32_t ncs = P.n_pad / 32;
+    HIP_TRY(hipEventRecord(D.evk[2], s));
+    for (uint32_t q0 = 0; q0 < n; q0 += P.wide_qb) {
+      const uint32_t nqb = std::min<uint32_t>(P.wide_qb, n - q0);
+      const dim3 g1(round_up(nqb, WIDE_QT) / WIDE_QT, P.n_pad / WIDE_CT);
+      wide_gemm_kernel<<<g1, 256, 0, s>>>(xs32, P.dp32, q0, ncs, scores);
+      HIP_TRY(hipGetLastError());
+      const dim3 g2((nqb + WIDE_QB - 1) / WIDE_QB, P.nsplit);
+#define WIDE_SCAN(KPR)                                                                                \
+  wide_scan_kernel<KPR><<<g2, 256, 0, s>>>(scores, q0, nqb, ncs, n, (int)P.kp, P.nsplit, P.cols_per_split, \
+                                           cand_idx, cand_sc, cand_cnt, cand_floor, inserts)
+      if (P.KP == 16) WIDE_SCAN(16);
+      else if (P.KP == 32) WIDE_SCAN(32);
+      else WIDE_SCAN(64);
+#undef WIDE_SCAN
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(D.evk[3], s));
+    const int all_admitted = (P.nsplit == 1 && P.kp >= n - 1) ? 1 : 0;
+    rc = rescore(P.kp, P.nsplit, nullptr, n, all_admitted, eps32, fb_rows, fb_lo, counters + 1, counters + 2);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(D.evk[4], s));
+    rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb_rows, counters + 1,
+                        0, fb_lo, eps32, idx, cos, wts, std::min<uint32_t>(n, 4u * (uint32_t)std::max(D.cus, 1)));
     if (rc) return rc;
     HIP_TRY(hipEventRecord(D.evk[5], s));
   } else {
@@ -1305,17 +1633,17 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   if (st) {
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, D.ev0, D.ev1) != hipSuccess) ms = 0.0f;
-    st->fallback_rows = P.fused ? cnt[1] : 0;
+    st->fallback_rows = (P.fused || P.wide) ? cnt[1] : 0;
     st->short_rows = cnt[2];
-    st->row_block = P.fused ? P.QB : 1;
-    st->col_block = P.fused ? P.TC : 256;
+    st->row_block = (P.fused || P.wide) ? P.QB : 1;
+    st->col_block = (P.fused || P.wide) ? P.TC : 256;
     st->blocks_skipped = 0;
-    st->blocks_total = P.fused ? (uint64_t)((n + P.QB - 1) / P.QB) * ((n + P.TC - 1) / P.TC) : n;
+    st->blocks_total = (P.fused || P.wide) ? (uint64_t)((n + P.QB - 1) / P.QB) * ((n + P.TC - 1) / P.TC) : n;
     st->candidates = P.kp;
-    st->col_splits = P.fused ? P.nsplit : 1;
-    st->path = P.fused ? 0 : 1;
+    st->col_splits = (P.fused || P.wide) ? P.nsplit : 1;
+    st->path = P.fused ? 0 : P.wide ? 2 : 1;
     st->inserts = (uint64_t)cnt[8] | ((uint64_t)cnt[9] << 32);
-    st->rowscan_rows = P.fused ? (p.row_threshold || learned ? cnt[3] : cnt[1]) : n;
+    st->rowscan_rows = P.fused ? cnt[3] : P.wide ? cnt[1] : n;
     st->device_ms = ms;
     auto span = [&](int a, int b) {
       float v = 0.0f;
@@ -1338,17 +1666,23 @@ typedef void (*CoarseBatchFn)(const BatchSample*, const uint32_t*, uint32_t, uin
                               const float*, uint32_t, float, uint32_t*, float*, uint32_t*, float*,
                               unsigned long long*);
 
+template <int PREC, int J, int KP>
+CoarseBatchFn coarse_batch_ptr() {
+  if constexpr (PREC == PREC_F16X3 && J % 2) return nullptr;  // split rows hold hi and lo halves
+  else return coarse_batch_kernel<PREC, J, KP, 4>;
+}
+
 template <int PREC, int KP>
 CoarseBatchFn coarse_batch_by_j(uint32_t J) {
   switch (J) {
-    case 1: return coarse_batch_kernel<PREC, 1, KP, 4>;
-    case 2: return coarse_batch_kernel<PREC, 2, KP, 4>;
-    case 3: return coarse_batch_kernel<PREC, 3, KP, 4>;
-    case 4: return coarse_batch_kernel<PREC, 4, KP, 4>;
-    case 5: return coarse_batch_kernel<PREC, 5, KP, 4>;
-    case 6: return coarse_batch_kernel<PREC, 6, KP, 4>;
-    case 7: return coarse_batch_kernel<PREC, 7, KP, 4>;
-    case 8: return coarse_batch_kernel<PREC, 8, KP, 4>;
+    case 1: return coarse_batch_ptr<PREC, 1, KP>();
+    case 2: return coarse_batch_ptr<PREC, 2, KP>();
+    case 3: return coarse_batch_ptr<PREC, 3, KP>();
+    case 4: return coarse_batch_ptr<PREC, 4, KP>();
+    case 5: return coarse_batch_ptr<PREC, 5, KP>();
+    case 6: return coarse_batch_ptr<PREC, 6, KP>();
+    case 7: return coarse_batch_ptr<PREC, 7, KP>();
+    case 8: return coarse_batch_ptr<PREC, 8, KP>();
   }
   return nullptr;
 }

@@ -23,6 +23,7 @@ namespace mlp {
 
 constexpr int H = COSTAL_MLP_HIDDEN;
 constexpr int DMAX = COSTAL_MLP_MAX_FEATURES;
+static_assert(DMAX == H, "the training kernel writes feature j from thread j");
 
 __host__ __device__ constexpr uint64_t param_count(uint64_t d) { return H * d + H + H * H + H + H + 1; }
 
@@ -126,42 +127,82 @@ __global__ __launch_bounds__(256) void predict_kernel(const float* __restrict__ 
 // act = [xn (DMAX), h1 (H), h2 (H), dy, dh2 (H), dh1 (H), sq_err].
 constexpr int ACT = DMAX + 5 * H + 2;
 
-__global__ __launch_bounds__(256) void fwd_bwd_kernel(const float* __restrict__ x, const float* __restrict__ tgt,
-                                                      uint32_t n, uint32_t d, const float* __restrict__ P,
-                                                      float* __restrict__ act) {
-  __shared__ Smem S;
-  load_params<false>(S, P, d);
+// One workgroup = one wave and FB_ROWS rows; thread j owns hidden unit j of both layers, so every
+// dot product is one thread's fixed-order fmaf chain (the same order as forward_row) over weights
+// held in LDS in the layout that makes the wave's reads consecutive: W1 transposed ([t][j]) and W2
+// both ways ([j][g] for the backward pass, [g][j] for the forward pass).
+constexpr int FB_ROWS = 8;
+
+__global__ __launch_bounds__(64) void fwd_bwd_kernel(const float* __restrict__ x, const float* __restrict__ tgt,
+                                                     uint32_t n, uint32_t d, const float* __restrict__ P,
+                                                     float* __restrict__ act) {
+  __shared__ float W1T[DMAX * H];
+  __shared__ float W2R[H * H];
+  __shared__ float W2T[H * H];
+  __shared__ float xs[FB_ROWS][DMAX], h1s[FB_ROWS][H], h2s[FB_ROWS][H], dh2s[FB_ROWS][H], dys[FB_ROWS];
+  const float* W1 = P;
+  const float* b1 = W1 + H * d;
+  const float* W2 = b1 + H;
+  const float* b2 = W2 + H * H;
+  const float* w3 = b2 + H;
+  const float* b3 = w3 + H;
+  const uint32_t j = tid_x();
+  for (uint32_t e = j; e < H * d; e += H) {
+    const uint32_t jj = e / d, t = e % d;
+    W1T[t * H + jj] = W1[e];
+  }
+  for (uint32_t e = j; e < H * H; e += H) {
+    W2R[e] = W2[e];
+    W2T[(e % H) * H + e / H] = W2[e];
+  }
+  const float b1j = b1[j], b2j = b2[j], w3j = w3[j];
+  const uint32_t r0 = bid_x() * FB_ROWS;
+  const uint32_t nr = n - r0 < (uint32_t)FB_ROWS ? n - r0 : (uint32_t)FB_ROWS;
+  // features: thread rr computes row rr's f64 norm and writes its scaled row
+  if (j < nr) {
+    float xn[DMAX];
+    load_features(x + (size_t)(r0 + j) * d, d, xn);
+#pragma unroll
+    for (int t = 0; t < DMAX; ++t) xs[j][t] = xn[t];
+  }
   __syncthreads();
-  const uint32_t r = bid_x() * bdim_x() + tid_x();
-  if (r >= n) return;
-  float xn[DMAX], h1[H], h2[H];
-  load_features(x + (size_t)r * d, d, xn);
-  const float y = forward_row<false>(S, xn, h1, h2);
-  const float err = y - tgt[r];
-  const float dy = 2.0f * err / (float)n;
-  float* a = act + (size_t)r * ACT;
-#pragma unroll
-  for (int t = 0; t < DMAX; ++t) a[t] = xn[t];
-#pragma unroll
-  for (int j = 0; j < H; ++j) {
-    a[DMAX + j] = h1[j];
-    a[DMAX + H + j] = h2[j];
+  for (uint32_t rr = 0; rr < nr; ++rr) {
+    float a = b1j;
+    for (uint32_t t = 0; t < d; ++t) a = fmaf(W1T[t * H + j], xs[rr][t], a);
+    h1s[rr][j] = fmaxf(a, 0.0f);
   }
-  a[DMAX + 2 * H] = dy;
-  float dh2[H];
-#pragma unroll
-  for (int g = 0; g < H; ++g) {
-    dh2[g] = h2[g] > 0.0f ? S.w3[g] * dy : 0.0f;
-    a[DMAX + 2 * H + 1 + g] = dh2[g];
+  __syncthreads();
+  for (uint32_t rr = 0; rr < nr; ++rr) {
+    float a = b2j;
+#pragma unroll 8
+    for (int g = 0; g < H; ++g) a = fmaf(W2T[g * H + j], h1s[rr][g], a);
+    h2s[rr][j] = fmaxf(a, 0.0f);
   }
-#pragma unroll
-  for (int g = 0; g < H; ++g) {
+  __syncthreads();
+  // output, error and dy: thread rr, a fixed-order sum over the hidden units
+  if (j < nr) {
+    float y = b3[0];
+    for (int g = 0; g < H; ++g) y = fmaf(w3[g], h2s[j][g], y);
+    const float err = y - tgt[r0 + j];
+    dys[j] = 2.0f * err / (float)n;
+    float* a = act + (size_t)(r0 + j) * ACT;
+    a[DMAX + 2 * H] = dys[j];
+    a[DMAX + 4 * H + 1] = err * err;
+  }
+  __syncthreads();
+  for (uint32_t rr = 0; rr < nr; ++rr) dh2s[rr][j] = h2s[rr][j] > 0.0f ? w3j * dys[rr] : 0.0f;
+  __syncthreads();
+  for (uint32_t rr = 0; rr < nr; ++rr) {
+    float* a = act + (size_t)(r0 + rr) * ACT;
     float s = 0.0f;
-#pragma unroll
-    for (int j = 0; j < H; ++j) s = fmaf(S.W2[j * H + g], dh2[j], s);
-    a[DMAX + 3 * H + 1 + g] = h1[g] > 0.0f ? s : 0.0f;
+#pragma unroll 8
+    for (int jj = 0; jj < H; ++jj) s = fmaf(W2R[jj * H + j], dh2s[rr][jj], s);
+    a[j] = xs[rr][j];  // DMAX == H: thread j writes feature j
+    a[DMAX + j] = h1s[rr][j];
+    a[DMAX + H + j] = h2s[rr][j];
+    a[DMAX + 2 * H + 1 + j] = dh2s[rr][j];
+    a[DMAX + 3 * H + 1 + j] = h1s[rr][j] > 0.0f ? s : 0.0f;
   }
-  a[DMAX + 4 * H + 1] = err * err;
 }
 
 // Parameter gradients: thread p sums its parameter's contribution over the rows in order; thread P
@@ -259,7 +300,7 @@ int costal_mlp_grad_device(const float* x, const float* target, uint64_t n, uint
   if (d == 0 || d > COSTAL_MLP_MAX_FEATURES) return mlp::fail(COSTAL_E_UNSUPPORTED, "the threshold model takes 1..64 features");
   if (n == 0 || n > 0xffffffffull) return mlp::fail(COSTAL_E_INVALID, "bad batch size");
   hipStream_t s = (hipStream_t)stream;
-  mlp::fwd_bwd_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(x, target, (uint32_t)n, (uint32_t)d, params, act);
+  mlp::fwd_bwd_kernel<<<(unsigned)((n + mlp::FB_ROWS - 1) / mlp::FB_ROWS), 64, 0, s>>>(x, target, (uint32_t)n, (uint32_t)d, params, act);
   const uint64_t P = mlp::param_count(d);
   mlp::wgrad_kernel<<<(unsigned)((P + 1 + 255) / 256), 256, 0, s>>>(act, (uint32_t)n, (uint32_t)d, grad, loss);
   hipError_t e = hipGetLastError();

@@ -167,7 +167,7 @@ def fit(x: np.ndarray, u: np.ndarray, *, epochs: int = 60, batch: int = 512, lr:
 
 
 def train(x_train: np.ndarray, k: int, *, n_target: int | None = None, epochs: int = 60,
-          batch: int = 512, lr: float = 3e-3, holdout: float = 0.1, quantile: float = 0.99,
+          batch: int = 512, lr: float = 3e-3, holdout: float = 0.1, quantile: float = 0.995,
           seed: int = 0, log=None) -> ThresholdModel:
     """Train a threshold model on a sample of the data it will serve.
 

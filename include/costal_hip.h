@@ -97,7 +97,7 @@ typedef struct costal_knn_stats {
   uint64_t candidates;        /* K' = min(k + margin, n - 1) */
   uint64_t col_splits;        /* candidate-range splits used */
   uint64_t short_rows;        /* rows with fewer than k admitted candidates (thresholds only) */
-  uint64_t path;              /* 0 fused MFMA scan, 1 exact row scan (wide d, large K', tanimoto) */
+  uint64_t path;              /* 0 fused MFMA scan, 1 exact row scan (large K', tanimoto), 2 wide rows */
   uint64_t inserts;           /* candidate-list inserts made by the fused scan */
   uint64_t rowscan_rows;      /* rows that needed the exact full-row kernel */
   double device_ms;           /* device time of the call (hipEvent), 0 when not measured */

@@ -67,6 +67,7 @@ constexpr uint32_t NONE = 0xffffffffu;
 __device__ __forceinline__ double dot64(const float* __restrict__ a, const float* __restrict__ b,
                                         uint32_t d) {
   double s = 0.0;
+#pragma unroll 8
   for (uint32_t t = 0; t < d; ++t) s += (double)a[t] * (double)b[t];
   return s;
 }
@@ -413,7 +414,7 @@ __device__ __forceinline__ void scan_block(
   // scanned and stored to LDS during tile t (buffers tile0 and tile1, the loop unrolled by two so
   // every buffer is static).
   static_assert(CPT <= 4, "tile staging uses at most four 16-byte registers per thread");
-  uint4 ra0 = {}, ra1 = {}, ra2 = {}, ra3 = {};
+  uint4 ra0 = {}, ra1 = {}, ra2 = {}, ra3 = {}, rb0 = {}, rb1 = {}, rb2 = {}, rb3 = {};
   // Thread e fills LDS slot e (consecutive lanes, consecutive 16-byte slots: conflict free); slot
   // e = ((sub*J + j)*2 + half)*32 + lane32 holds chunk 2j + half of tile row sub*32 + lane32.
 #define LOAD_ONE(R, C)                                                                            \
@@ -444,18 +445,19 @@ __device__ __forceinline__ void scan_block(
     __builtin_amdgcn_s_barrier();                                                                 \
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");                                        \
   }
-  // the lane's chunks of scan step S of the tile in BUF
-#define READ_B(BUF, S)                                                                            \
+  // the lane's chunks of scan step S of the tile in BUF, into chunk buffer BS
+#define READ_B(BS, BUF, S)                                                                        \
   _Pragma("unroll") for (int u_ = 0; u_ < GS; ++u_)                                               \
     _Pragma("unroll") for (int j_ = 0; j_ < J; ++j_)                                              \
-      bop[u_][j_] = *reinterpret_cast<const uint4*>(                                              \
+      bop[BS][u_][j_] = *reinterpret_cast<const uint4*>(                                          \
           (BUF) + (((((S) * GS + u_) * J + j_) * 2 + h) * 32 + l32) * 16);
 #define MFMA_STEP(SLOT) \
-  _Pragma("unroll") for (int u_ = 0; u_ < GS; ++u_) acc[SLOT][u_] = tile_mfma<PREC, J>(bop[u_], qreg);
+  _Pragma("unroll") for (int u_ = 0; u_ < GS; ++u_) acc[SLOT][u_] = tile_mfma<PREC, J>(bop[SLOT][u_], qreg);
 
   const float* xq32 = xs32 + (size_t)qg * dp32;
   f32x16 acc[2][GS];
-  uint4 bop[GS][J];
+  uint4 bop[2][GS][J];
+  constexpr int RD = NSTEP >= 4 ? 3 : 2;  // chunks of step G+RD are read at step G
   int t = 0;
   // compare the accumulators of step `p` of tile t with the lane's threshold; rare path: each lane
   // walks its own hits, sub-tile by sub-tile
@@ -501,37 +503,39 @@ __device__ __forceinline__ void scan_block(
     }
   };
   // all NSTEP steps of tile t, held in CUR; the next tile goes from the staging registers to OTH
-#define RUN_TILE(CUR, OTH)                                                                        \
+#define RUN_TILE(CUR, OTH, R)                                                                     \
   _Pragma("unroll") for (int p = 0; p < NSTEP; ++p) {                                             \
     MFMA_STEP((p + 1) & 1)                                                                        \
     if (p == 0) thp = partner32(th);                                                              \
-    if (p == NSTEP - 2) {                                                                         \
-      STORE_REGS(ra, OTH);                                                                        \
+    if (p == NSTEP - RD) {                                                                        \
+      STORE_REGS(R, OTH);                                                                         \
       TILE_BARRIER();                                                                             \
-      LOAD_REGS(ra, t + 2 < ntile ? t + 2 : ntile - 1);                                           \
+      LOAD_REGS(R, t + 3 < ntile ? t + 3 : ntile - 1);                                            \
     }                                                                                             \
-    if (p + 2 < NSTEP) {                                                                          \
-      READ_B(CUR, p + 2)                                                                          \
+    if (p + RD < NSTEP) {                                                                         \
+      READ_B((p + RD) & 1, CUR, p + RD)                                                           \
     } else {                                                                                      \
-      READ_B(OTH, p + 2 - NSTEP)                                                                  \
+      READ_B((p + RD) & 1, OTH, p + RD - NSTEP)                                                   \
     }                                                                                             \
     scan_step(acc[p & 1], p);                                                                     \
   }
 
   if (ntile > 0) {
     LOAD_REGS(ra, 0);
+    LOAD_REGS(rb, ntile > 1 ? 1 : 0);
     STORE_REGS(ra, tile0);
     TILE_BARRIER();
-    LOAD_REGS(ra, ntile > 1 ? 1 : 0);
-    READ_B(tile0, 0)
+    LOAD_REGS(ra, ntile > 2 ? 2 : ntile - 1);
+    READ_B(0, tile0, 0)
+    READ_B(1, tile0, 1)
     MFMA_STEP(0)
-    READ_B(tile0, 1)
+    if constexpr (RD == 3) READ_B(0, tile0, 2)
     for (;;) {
       // tile t in tile0, ra = tile t+1 in flight
-      RUN_TILE(tile0, tile1)
+      RUN_TILE(tile0, tile1, rb)
       if (++t >= ntile) break;
       // tile t in tile1, ra = tile t+1 in flight
-      RUN_TILE(tile1, tile0)
+      RUN_TILE(tile1, tile0, ra)
       if (++t >= ntile) break;
     }
   }
@@ -786,9 +790,10 @@ __device__ void bitonic_lds(double* __restrict__ sc, double* __restrict__ cs, ui
 }
 
 // ------------------------------------------------------------------------------------------------
-// rescore_kernel: one 64-thread workgroup per query.  Gathers the query's nsplit lists, rescores
-// each candidate in f64 over the original rows, sorts, applies the certificate and writes the
-// top k (or queues the row for the full-row path).
+// rescore_kernel: one 64-thread workgroup per query (as many waves in flight as the chip holds:
+// the work is latency-bound).  Gathers the query's nsplit lists, rescores each candidate in f64
+// over the original rows, sorts, applies the certificate and writes the top k (or queues the row
+// for the fallback).
 
 // One query's rescoring (64 threads).  Failed rows are recorded as fb_tag in fb_rows.
 template <int CMAX>
@@ -865,11 +870,12 @@ __global__ __launch_bounds__(64) void rescore_kernel(
     uint32_t* __restrict__ short_count) {
   __shared__ double s_sc[CMAX];
   __shared__ uint32_t s_id[CMAX];
-  const uint32_t qq = bid_x();
-  if (qq >= nq) return;
-  const uint32_t i = rowlist ? rowlist[qq] : qq;
-  rescore_row<CMAX>(s_sc, s_id, x, d, norms, k, kp, nsplit, qq, i, cand_idx, cand_cnt, cand_floor, all_admitted,
-                    eps_list, eps32, idx, cos_out, wts, i, fb_rows, fb_lo, fb_count, short_count);
+  for (uint32_t qq = bid_x(); qq < nq; qq += gdim_x()) {
+    const uint32_t i = rowlist ? rowlist[qq] : qq;
+    rescore_row<CMAX>(s_sc, s_id, x, d, norms, k, kp, nsplit, qq, i, cand_idx, cand_cnt, cand_floor, all_admitted,
+                      eps_list, eps32, idx, cos_out, wts, i, fb_rows, fb_lo, fb_count, short_count);
+    __syncthreads();  // the LDS arrays are refilled by the next row
+  }
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -227,8 +227,8 @@ __global__ __launch_bounds__(256) void prep_wide_kernel(const float* __restrict_
 // coarse_kernel
 //
 // Workgroup = NW waves; each wave owns 32 queries, held in registers for the whole launch.  The
-// workgroup streams its candidate range through two LDS buffers of TC-row tiles (16 KB each),
-// staged global -> registers -> LDS one tile ahead of the compute.  For every 32-candidate
+// workgroup streams its candidate range through two LDS buffers of TC-row tiles (32 KB each),
+// staged global -> registers -> LDS ahead of the compute.  For every 32-candidate
 // sub-tile a wave issues the MFMAs for D[cand][query] = sum_k cand[k] * query[k]: lane l owns query
 // (l % 32) and the 16 candidates 8*(i/4) + 4*(l/32) + (i%4), i = 0..15 (the 32x32 accumulator
 // layout).  The scan takes the max of a step's sub-tiles (max3 chains) and compares once with the
@@ -335,7 +335,7 @@ __device__ __forceinline__ f32x16 tile_mfma(const uint4 (&b)[J], const uint4 (&q
 
 template <int J>
 struct TileShape {
-  static constexpr int TC = J <= 2 ? 256 : J <= 4 ? 128 : 64;  // candidates per tile: 16 KB tiles
+  static constexpr int TC = J <= 2 ? 512 : J <= 4 ? 256 : 128;  // candidates per tile: 32 KB tiles
   static constexpr int TILEB = TC * 32 * J;
   static constexpr int SUBT = TC / 32;                         // 32-candidate sub-tiles per tile
   static constexpr int GS = (SUBT >= 4 && J <= 4) ? 2 : 1;     // sub-tiles per scan step
@@ -347,12 +347,12 @@ struct TileShape {
 //
 // Software pipeline over scan steps (GS sub-tiles each, NSTEP per tile, G counting steps over the
 // whole candidate range): step G issues the MFMAs of step G+1, reads the candidate chunks of step
-// G+2 from LDS into the registers those MFMAs have just consumed, and then scans the accumulators
-// of step G, so the matrix core works on the next step while the wave compares and inserts, and
-// the LDS reads complete behind the scan.  Accumulators are double buffered by the parity of G
-// (NSTEP is even, so the parity is the step's parity within its tile).  The next tile is stored to
-// the other LDS buffer, and the workgroup meets at the barrier, at step NSTEP-2 of each tile, before
-// the first read of that tile (step NSTEP-2 reads step 0 of the next tile).
+// G+3 from LDS, and then scans the accumulators of step G, so the matrix core works on the next
+// step while the wave compares and inserts, and the LDS reads complete behind two steps of work.
+// Accumulators and chunk registers are double buffered by the parity of G (NSTEP is even, so the
+// parity is the step's parity within its tile).  The next tile is stored to the other LDS buffer,
+// and the workgroup meets at the barrier, at step NSTEP-3 of each tile, before the first read of
+// that tile.
 template <int PREC, int J, int KPR, int NW>
 __device__ __forceinline__ void scan_block(
     uint8_t* __restrict__ tile0, uint8_t* __restrict__ tile1,
@@ -410,11 +410,14 @@ __device__ __forceinline__ void scan_block(
   if (c_end > n) c_end = n;
   const int ntile = c_begin < c_end ? (int)((c_end - c_begin + TC - 1) / TC) : 0;
 
-  // Tiles go global -> registers -> LDS: the registers of tile t+1 are loaded while tile t-1 is
-  // scanned and stored to LDS during tile t (buffers tile0 and tile1, the loop unrolled by two so
-  // every buffer is static).
-  static_assert(CPT <= 4, "tile staging uses at most four 16-byte registers per thread");
-  uint4 ra0 = {}, ra1 = {}, ra2 = {}, ra3 = {}, rb0 = {}, rb1 = {}, rb2 = {}, rb3 = {};
+  // Tiles go global -> registers -> LDS: the registers are loaded with tile t+2 while tile t is
+  // scanned (right after the workgroup has stored tile t+1 and met) and stored during tile t+1, so
+  // a load has a whole tile of steps to arrive.  One register set, and a loop body of one tile
+  // whose only exit is at its end, swapping the two LDS buffers: the compiler's wait counts then
+  // stay exact at the loop head (a second staging set, or an exit from the middle of a body
+  // unrolled by two, made it wait there for loads still in flight).
+  static_assert(CPT <= 8, "tile staging uses at most eight 16-byte registers per thread");
+  uint4 ra0 = {}, ra1 = {}, ra2 = {}, ra3 = {}, ra4 = {}, ra5 = {}, ra6 = {}, ra7 = {};
   // Thread e fills LDS slot e (consecutive lanes, consecutive 16-byte slots: conflict free); slot
   // e = ((sub*J + j)*2 + half)*32 + lane32 holds chunk 2j + half of tile row sub*32 + lane32.
 #define LOAD_ONE(R, C)                                                                            \
@@ -430,6 +433,7 @@ __device__ __forceinline__ void scan_block(
   {                                                                                               \
     const uint8_t* src_ = xsq + (size_t)(c_begin + (uint32_t)(T) * TC) * ROWB;                     \
     LOAD_ONE(R, 0) LOAD_ONE(R, 1) LOAD_ONE(R, 2) LOAD_ONE(R, 3)                                   \
+    LOAD_ONE(R, 4) LOAD_ONE(R, 5) LOAD_ONE(R, 6) LOAD_ONE(R, 7)                                   \
   }
 #define STORE_ONE(R, C, BUF)                                                                      \
   if constexpr (C < CPT) {                                                                        \
@@ -437,7 +441,8 @@ __device__ __forceinline__ void scan_block(
     if (CHUNKS % NT == 0 || e_ < CHUNKS) *reinterpret_cast<uint4*>(&BUF[e_ * 16]) = R##C;         \
   }
 #define STORE_REGS(R, BUF) \
-  { STORE_ONE(R, 0, BUF) STORE_ONE(R, 1, BUF) STORE_ONE(R, 2, BUF) STORE_ONE(R, 3, BUF) }
+  { STORE_ONE(R, 0, BUF) STORE_ONE(R, 1, BUF) STORE_ONE(R, 2, BUF) STORE_ONE(R, 3, BUF)         \
+    STORE_ONE(R, 4, BUF) STORE_ONE(R, 5, BUF) STORE_ONE(R, 6, BUF) STORE_ONE(R, 7, BUF) }
   // LDS writes of this wave done, then everyone meets; no wait on the global loads in flight
 #define TILE_BARRIER()                                                                            \
   {                                                                                               \
@@ -457,7 +462,8 @@ __device__ __forceinline__ void scan_block(
   const float* xq32 = xs32 + (size_t)qg * dp32;
   f32x16 acc[2][GS];
   uint4 bop[2][GS][J];
-  constexpr int RD = NSTEP >= 4 ? 3 : 2;  // chunks of step G+RD are read at step G
+  constexpr int RD = 3;  // chunks of step G+RD are read at step G
+  static_assert(NSTEP >= RD, "steps per tile");
   int t = 0;
   // compare the accumulators of step `p` of tile t with the lane's threshold; rare path: each lane
   // walks its own hits, sub-tile by sub-tile
@@ -503,14 +509,14 @@ __device__ __forceinline__ void scan_block(
     }
   };
   // all NSTEP steps of tile t, held in CUR; the next tile goes from the staging registers to OTH
-#define RUN_TILE(CUR, OTH, R)                                                                     \
+#define RUN_TILE(CUR, OTH)                                                                        \
   _Pragma("unroll") for (int p = 0; p < NSTEP; ++p) {                                             \
     MFMA_STEP((p + 1) & 1)                                                                        \
     if (p == 0) thp = partner32(th);                                                              \
     if (p == NSTEP - RD) {                                                                        \
-      STORE_REGS(R, OTH);                                                                         \
+      STORE_REGS(ra, OTH);                                                                        \
       TILE_BARRIER();                                                                             \
-      LOAD_REGS(R, t + 3 < ntile ? t + 3 : ntile - 1);                                            \
+      LOAD_REGS(ra, t + 2 < ntile ? t + 2 : ntile - 1);                                           \
     }                                                                                             \
     if (p + RD < NSTEP) {                                                                         \
       READ_B((p + RD) & 1, CUR, p + RD)                                                           \
@@ -521,22 +527,23 @@ __device__ __forceinline__ void scan_block(
   }
 
   if (ntile > 0) {
+    uint8_t* cur = tile0;
+    uint8_t* oth = tile1;
     LOAD_REGS(ra, 0);
-    LOAD_REGS(rb, ntile > 1 ? 1 : 0);
-    STORE_REGS(ra, tile0);
+    STORE_REGS(ra, cur);
     TILE_BARRIER();
-    LOAD_REGS(ra, ntile > 2 ? 2 : ntile - 1);
-    READ_B(0, tile0, 0)
-    READ_B(1, tile0, 1)
+    LOAD_REGS(ra, ntile > 1 ? 1 : 0);
+    READ_B(0, cur, 0)
+    READ_B(1, cur, 1)
     MFMA_STEP(0)
-    if constexpr (RD == 3) READ_B(0, tile0, 2)
+    READ_B(0, cur, 2)
     for (;;) {
-      // tile t in tile0, ra = tile t+1 in flight
-      RUN_TILE(tile0, tile1, rb)
+      // tile t in cur, ra = tile t+1 in flight
+      RUN_TILE(cur, oth)
       if (++t >= ntile) break;
-      // tile t in tile1, ra = tile t+1 in flight
-      RUN_TILE(tile1, tile0, ra)
-      if (++t >= ntile) break;
+      uint8_t* const tmp = cur;
+      cur = oth;
+      oth = tmp;
     }
   }
 #undef RUN_TILE
@@ -995,7 +1002,7 @@ __global__ __launch_bounds__(256) void rowscan_kernel(
 
 // ------------------------------------------------------------------------------------------------
 // Batched samples: many independent kNN problems (uneven n, d) in one pass of each kernel.  The
-// working copies of all samples are stacked (each sample padded to 256 rows, all rows padded to the
+// working copies of all samples are stacked (each sample padded to 512 rows, all rows padded to the
 // batch's widest feature count, zeros being exact), and one persistent scan kernel pulls
 // (sample, query block, candidate split) work items from a queue ordered by decreasing cost.
 
@@ -1349,8 +1356,8 @@ Plan make_plan(const Device& D, uint64_t n, uint64_t d, uint64_t k, const costal
   P.KP = list_template(P.kp);
   P.NW = 4;
   P.QB = 32 * P.NW;
-  P.TC = P.J <= 2 ? 256 : P.J <= 4 ? 128 : 64;  // the kernel's 16 KB tiles
-  P.n_pad = round_up((uint32_t)n, 256);
+  P.TC = P.J <= 2 ? 512 : P.J <= 4 ? 256 : 128;  // the kernel's 32 KB tiles
+  P.n_pad = round_up((uint32_t)n, 512);  // whole tiles of every kind can be read
   const uint32_t ntile = (uint32_t)((n + P.TC - 1) / P.TC);
   const uint32_t nblk = (uint32_t)((n + P.QB - 1) / P.QB);
   uint32_t ns = p.col_splits;
@@ -1752,7 +1759,7 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
   const int prec = (int)p.precision;
   const uint32_t dp32 = round_up(dmax, 8), dp16 = round_up(dmax, 16);
   const uint32_t J = prec == PREC_F16 ? dp16 / 16 : prec == PREC_F16X3 ? dp16 / 8 : dp32 / 8;
-  const uint32_t TC = J <= 2 ? 256 : J <= 4 ? 128 : 64;
+  const uint32_t TC = J <= 2 ? 512 : J <= 4 ? 256 : 128;
   const uint32_t QB = 128;
   const bool need16 = prec != PREC_F32;
   const int split16 = prec == PREC_F16X3 ? 1 : 0;
@@ -1808,7 +1815,7 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
           items.push_back(sp);
         }
       blocks_total += (uint64_t)nqb * ntile;
-      pad += round_up(n, 256);
+      pad += round_up(n, 512);
       base += n;
       slots += (uint64_t)n * ns * 2;
       entries += (uint64_t)n * ns * 2 * kp;

@@ -342,8 +342,9 @@ struct TileShape {
   static constexpr int NSTEP = SUBT / GS;                      // scan steps per tile
 };
 
-// One work item of the scan: query block `qblock` (QB queries) against candidate split `split` of
-// one sample.  tile0/tile1 are the workgroup's two LDS tile buffers.
+// One work item of the scan: query block `qblock` (QB queries) against candidates [c_begin, c_end)
+// of one sample, whose lists go to split `split` of the queries' nsplit list pairs.  tile0/tile1
+// are the workgroup's two LDS tile buffers.
 //
 // Software pipeline over scan steps (GS sub-tiles each, NSTEP per tile, G counting steps over the
 // whole candidate range): step G issues the MFMAs of step G+1, reads the candidate chunks of step
@@ -362,7 +363,7 @@ __device__ __forceinline__ void scan_block(
     const uint32_t* __restrict__ rowlist, uint32_t nq,  // queries: rowlist[0..nq) or rows 0..nq-1
     uint32_t qblock, const float* __restrict__ tinit,   // per-row admission threshold (by row) or null
     uint32_t split, uint32_t nsplit,
-    uint32_t cols_per_split,           // candidate columns per split, a multiple of the tile
+    uint32_t c_begin, uint32_t c_end,  // candidate range; c_begin a multiple of the tile, c_end <= n
     float eps_pre,                     // FP16 form: bound on |FP16 score - f32 score|
     uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc, uint32_t* __restrict__ cand_cnt,
     float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
@@ -405,9 +406,6 @@ __device__ __forceinline__ void scan_block(
   float thp = -INFINITY;  // the partner list's threshold, refreshed once per tile
   uint32_t my_inserts = 0;
 
-  const uint32_t c_begin = split * cols_per_split;
-  uint32_t c_end = c_begin + cols_per_split;
-  if (c_end > n) c_end = n;
   const int ntile = c_begin < c_end ? (int)((c_end - c_begin + TC - 1) / TC) : 0;
 
   // Tiles go global -> registers -> LDS: the registers are loaded with tile t+2 while tile t is
@@ -580,16 +578,63 @@ __device__ __forceinline__ void scan_block(
   }
 }
 
+// Two launch shapes:
+//   units_per_wg == 0: grid (query blocks, splits); workgroup (x, y) scans query block x against
+//     columns [y * cols_per_split, (y+1) * cols_per_split).
+//   units_per_wg > 0: one workgroup per resident slot, each taking the same number of (query block,
+//     tile) units in query-block-major order, so every slot ends together.  A query block whose
+//     units straddle two workgroups is scanned in two pieces: the one from tile 0 fills list pair 0,
+//     the other list pair 1 (nsplit = 2); a block scanned whole marks pair 1 empty.
 template <int PREC, int J, int KPR, int NW>
 __global__ __launch_bounds__(64 * NW) void coarse_kernel(
     const uint8_t* __restrict__ xsq, const float* __restrict__ xs32, uint32_t dp32, uint32_t n, int kp,
     const uint32_t* __restrict__ rowlist, uint32_t nq, const float* __restrict__ tinit, uint32_t cols_per_split,
-    float eps_pre, uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc, uint32_t* __restrict__ cand_cnt,
-    float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
+    uint32_t units_per_wg, float eps_pre, uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc,
+    uint32_t* __restrict__ cand_cnt, float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
   __shared__ __attribute__((aligned(16))) uint8_t tile0[TileShape<J>::TILEB];
   __shared__ __attribute__((aligned(16))) uint8_t tile1[TileShape<J>::TILEB];
-  scan_block<PREC, J, KPR, NW>(tile0, tile1, xsq, xs32, dp32, n, kp, rowlist, nq, bid_x(), tinit, bid_y(),
-                               gdim_y(), cols_per_split, eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor, inserts);
+  constexpr uint32_t TC = TileShape<J>::TC;
+  constexpr uint32_t QB = 32 * NW;
+  const uint32_t ntile = (n + TC - 1) / TC;
+  const uint64_t units = units_per_wg ? (uint64_t)((nq + QB - 1) / QB) * ntile : 1;
+  uint64_t u = units_per_wg ? (uint64_t)bid_x() * units_per_wg : 0;
+  const uint64_t u_end = units_per_wg ? (u + units_per_wg < units ? u + units_per_wg : units) : 1;
+  // one piece per iteration (the direct shape has exactly one); a single call site keeps one copy
+  // of the scan in the kernel
+  for (bool first = true; u < u_end; first = false) {
+    uint32_t qb, split, nspl, c0, c1, len;
+    bool whole = false;
+    if (units_per_wg) {
+      qb = (uint32_t)(u / ntile);
+      const uint32_t tb = (uint32_t)(u % ntile);
+      const uint32_t te = (uint32_t)(tb + (u_end - u) < ntile ? tb + (u_end - u) : ntile);
+      split = tb == 0 ? 0 : 1;
+      nspl = 2;
+      c0 = tb * TC;
+      c1 = te * TC < n ? te * TC : n;
+      len = te - tb;
+      whole = tb == 0 && te == ntile;
+    } else {
+      qb = bid_x();
+      split = bid_y();
+      nspl = gdim_y();
+      c0 = split * cols_per_split;
+      c1 = c0 + cols_per_split < n ? c0 + cols_per_split : n;
+      len = 1;
+    }
+    if (!first) __syncthreads();  // every wave is done with the previous piece's tiles
+    scan_block<PREC, J, KPR, NW>(tile0, tile1, xsq, xs32, dp32, n, kp, rowlist, nq, qb, tinit, split, nspl, c0, c1,
+                                 eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor, inserts);
+    if (whole) {
+      const uint32_t qq = qb * QB + (tid_x() >> 6) * 32 + (tid_x() & 31);
+      if (qq < nq) {
+        const size_t slot = ((size_t)qq * 2 + 1) * 2 + ((tid_x() >> 5) & 1);
+        cand_cnt[slot] = 0;
+        cand_floor[slot] = -INFINITY;
+      }
+    }
+    u += len;
+  }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -815,9 +860,6 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
   const double ni = norms[i];
   const float* xi = x + (size_t)i * d;
   const int tot_slots = (int)nsplit * kp;
-  int P = 1;
-  while (P < tot_slots) P <<= 1;
-  if (P < 2) P = 2;
 
   uint32_t total = 0;
   double floor_ = -INFINITY;
@@ -825,19 +867,28 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
     total += cand_cnt[(size_t)qq * nsplit + s];
     floor_ = fmax(floor_, (double)cand_floor[(size_t)qq * nsplit + s]);
   }
-  for (int e = tid_x(); e < P; e += 64) {
-    double sc = -INFINITY;
-    uint32_t j = NONE;
-    if (e < tot_slots) {
-      const uint32_t s = (uint32_t)e / (uint32_t)kp, r = (uint32_t)e % (uint32_t)kp;
-      const size_t slot = (size_t)qq * nsplit + s;
-      if (r < cand_cnt[slot]) {
-        j = cand_idx[slot * kp + r];
-        sc = cos64(dot64(xi, x + (size_t)j * d, d), ni, norms[j]);
-      }
+  // the lists' entries, compacted (lists are disjoint, so the sort below fixes the order whatever
+  // the order of arrival), then padded to a power of two for the sort
+  __shared__ uint32_t s_fill;
+  if (tid_x() == 0) s_fill = 0;
+  __syncthreads();
+  for (int e = tid_x(); e < tot_slots; e += 64) {
+    const uint32_t s = (uint32_t)e / (uint32_t)kp, r = (uint32_t)e % (uint32_t)kp;
+    const size_t slot = (size_t)qq * nsplit + s;
+    if (r < cand_cnt[slot]) {
+      const uint32_t j = cand_idx[slot * kp + r];
+      const uint32_t pos = atomicAdd(&s_fill, 1u);
+      s_sc[pos] = cos64(dot64(xi, x + (size_t)j * d, d), ni, norms[j]);
+      s_id[pos] = j;
     }
-    s_sc[e] = sc;
-    s_id[e] = j;
+  }
+  __syncthreads();
+  const uint32_t filled = s_fill;  // = total
+  int P = 2;
+  while ((uint32_t)P < filled) P <<= 1;
+  for (int e = (int)filled + tid_x(); e < P; e += 64) {
+    s_sc[e] = -INFINITY;
+    s_id[e] = NONE;
   }
   __syncthreads();
   bitonic_lds<64>(s_sc, nullptr, s_id, P);
@@ -1098,8 +1149,10 @@ __global__ __launch_bounds__(64 * NW) void coarse_batch_kernel(
     const uint32_t sidx = items[3 * it], qb = items[3 * it + 1], sp = items[3 * it + 2];
     const BatchSample& S = samples[sidx];
     const size_t rowb = (size_t)S.row_pad;
+    const uint32_t c0 = sp * S.cols_per_split;
+    const uint32_t c1 = c0 + S.cols_per_split < S.n ? c0 + S.cols_per_split : S.n;
     scan_block<PREC, J, KPR, NW>(tile0, tile1, xsq + rowb * 32 * J, xs32 + rowb * dp32, dp32, S.n, S.kp, nullptr,
-                                 S.n, qb, nullptr, sp, S.nsplit, S.cols_per_split, eps_pre,
+                                 S.n, qb, nullptr, sp, S.nsplit, c0, c1, eps_pre,
                                  cand_idx + S.entry_base, cand_sc + S.entry_base, cand_cnt + S.slot_base,
                                  cand_floor + S.slot_base, inserts);
     __syncthreads();
@@ -1265,7 +1318,7 @@ struct Plan {
 };
 
 typedef void (*CoarseFn)(const uint8_t*, const float*, uint32_t, uint32_t, int, const uint32_t*,
-                         uint32_t, const float*, uint32_t, float, uint32_t*, float*, uint32_t*,
+                         uint32_t, const float*, uint32_t, uint32_t, float, uint32_t*, float*, uint32_t*,
                          float*, unsigned long long*);
 
 template <int PREC, int J, int KP, int NW>
@@ -1441,6 +1494,31 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   if (p.select > COSTAL_SELECT_TANIMOTO) return fail(COSTAL_E_INVALID, "unknown select %u", p.select);
   Plan P = make_plan(D, n, d, k, p);
   if (brute) P.fused = P.wide = false;
+  // Balanced first pass (tuned splits only): with at least one query block per resident workgroup
+  // slot, the slots take equal shares of the (query block, tile) units instead of whole blocks,
+  // so no slot idles while the last blocks finish; a block then has at most two pieces.  A second
+  // piece starts with empty lists (more inserts) and the workgroups no longer stream the same tiles
+  // together, so this pays where the scan itself is long against its inserts: the FP32 form, and
+  // split FP16 with long lists (measured on 86,864 x 39 and 396,460 x 15; FP16 and split FP16 with
+  // k = 10 lists were slower).
+  uint32_t units_per_wg = 0, grid_bal = 0;
+  if (P.fused && p.col_splits == 0 && (P.prec == PREC_F32 || (P.prec == PREC_F16X3 && P.KP >= 32))) {
+    CoarseFn f0 = coarse_select(P.prec, P.J, P.KP);
+    int per_cu = 0;
+    if (f0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(f0),
+                                                           64 * P.NW, 0) == hipSuccess && per_cu > 0) {
+      const uint64_t slots = (uint64_t)per_cu * (uint64_t)std::max(D.cus, 1);
+      const uint64_t nblk = (n + P.QB - 1) / P.QB, nt = (n + P.TC - 1) / P.TC;
+      if (nblk >= slots) {
+        const uint64_t units = nblk * nt;
+        const uint64_t per = std::max<uint64_t>((units + slots - 1) / slots, nt);
+        units_per_wg = (uint32_t)per;
+        grid_bal = (uint32_t)((units + per - 1) / per);
+        P.nsplit = 2;
+      }
+    }
+    (void)hipGetLastError();
+  }
   const bool need16 = P.fused && (P.prec == PREC_F16 || P.prec == PREC_F16X3);
   const int split16 = P.prec == PREC_F16X3 ? 1 : 0;
   const size_t row16 = (size_t)P.dp16 * (split16 ? 2 : 1);
@@ -1555,19 +1633,19 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     // one scan pass over the queries `rows` (null: every row), then rescoring; rows that fail the
     // certificate are appended to (fbr, fbl) and counted in *fbc
     auto scan_pass = [&](CoarseFn f, uint32_t kp, const uint32_t* rows, uint32_t nq, const float* th,
-                         uint32_t nsplit, uint32_t cols_per, uint32_t* fbr, double* fbl, uint32_t* fbc,
-                         uint32_t* shc) -> int {
-      dim3 grid((nq + P.QB - 1) / P.QB, nsplit);
-      f<<<grid, 64 * P.NW, 0, s>>>(xsq, xs32, P.dp32, n, (int)kp, rows, nq, th, cols_per, eps_pre, cand_idx,
-                                   cand_sc, cand_cnt, cand_floor, inserts);
+                         uint32_t nsplit, uint32_t cols_per, uint32_t units, uint32_t* fbr, double* fbl,
+                         uint32_t* fbc, uint32_t* shc) -> int {
+      const dim3 grid = units ? dim3(grid_bal, 1) : dim3((nq + P.QB - 1) / P.QB, nsplit);
+      f<<<grid, 64 * P.NW, 0, s>>>(xsq, xs32, P.dp32, n, (int)kp, rows, nq, th, cols_per, units, eps_pre,
+                                   cand_idx, cand_sc, cand_cnt, cand_floor, inserts);
       HIP_TRY(hipGetLastError());
       if (!rows) HIP_TRY(hipEventRecord(D.evk[3], s));
       const int all_admitted = (nsplit == 1 && kp >= n - 1 && !th) ? 1 : 0;
       return rescore(kp, nsplit, rows, nq, all_admitted, eps_list, fbr, fbl, fbc, shc);
     };
     HIP_TRY(hipEventRecord(D.evk[2], s));
-    rc = scan_pass(fn, P.kp, nullptr, n, row_th, P.nsplit, P.cols_per_split, fb_rows, fb_lo, counters + 1,
-                   counters + 2);
+    rc = scan_pass(fn, P.kp, nullptr, n, row_th, P.nsplit, P.cols_per_split, units_per_wg, fb_rows, fb_lo,
+                   counters + 1, counters + 2);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(D.evk[4], s));
     const uint32_t grid_fb = std::min<uint32_t>(n, 4u * (uint32_t)std::max(D.cus, 1));
@@ -1594,7 +1672,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
       ns = std::max<uint32_t>(ns, 1);
       const uint32_t tiles_per = (ntile + ns - 1) / ns;
       ns = (ntile + tiles_per - 1) / tiles_per;
-      rc = scan_pass(fn2, kp2, fb_rows, nfb, nullptr, ns, tiles_per * P.TC, fb2_rows, fb2_lo, counters + 3,
+      rc = scan_pass(fn2, kp2, fb_rows, nfb, nullptr, ns, tiles_per * P.TC, 0, fb2_rows, fb2_lo, counters + 3,
                      counters + 4);
       if (rc) return rc;
     }

@@ -196,6 +196,19 @@ def test_cytometry_workload_sampled_rows(matmul, oracle):
     assert_same((idx[:2000], cos[:2000], norms), (widx[:2000], wcos[:2000], wnorms))
 
 
+@pytest.mark.parametrize("matmul,k", [("mfma-f32", 10), ("mfma-f16x3", 30)])
+def test_balanced_launch_equals_direct_launch(matmul, k):
+    """With tuned splits a large input takes the balanced launch for these forms (equal shares of
+    (query block, tile) units per resident workgroup, so some blocks are scanned in two pieces by
+    two workgroups); every row equals the one-split direct launch."""
+    x = synth.cytometry_86864x39()
+    a, sa = ext._knn_exact(x, k, matmul=matmul)
+    b, sb = ext._knn_exact(x, k, matmul=matmul, col_splits=1)
+    assert sa["col_splits"] == 2 and sb["col_splits"] == 1
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
+
+
 def test_truncate_prefix_property():
     # tests/test_knn.py:97-108
     import costal_amd as cp

@@ -1429,6 +1429,7 @@ Plan make_plan(const Device& D, uint64_t n, uint64_t d, uint64_t k, const costal
     // rows of a multiple of 32 features; score blocks of up to 512 MB; the scan's candidate
     // sub-tiles split so that about four workgroups per CU run
     P.dp32 = round_up((uint32_t)std::max<uint64_t>(d, 1), WIDE_KC);
+    P.n_pad = round_up((uint32_t)n, 256);  // whole gemm tiles (128 candidates) and scan blocks
     P.QB = WIDE_QB;
     P.TC = 32;
     const uint64_t row_bytes = (uint64_t)P.n_pad * 4;

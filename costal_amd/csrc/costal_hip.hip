@@ -120,7 +120,6 @@ __global__ void prep_kernel(const float* __restrict__ x, uint32_t n, uint32_t d,
                             uint32_t* __restrict__ bad_row) {
   uint32_t i = bid_x() * bdim_x() + tid_x();
   if (i >= n_pad) return;
-  const uint32_t dmax = dp32 > dp16 ? dp32 : dp16;
   const uint32_t s16 = split16 ? 2 * dp16 : dp16;
   if (i >= n) {
     for (uint32_t t = 0; t < dp32; ++t) xs32[(size_t)i * dp32 + t] = 0.0f;
@@ -142,13 +141,23 @@ __global__ void prep_kernel(const float* __restrict__ x, uint32_t n, uint32_t d,
   const double nrm = sqrt(s);
   norms[i] = nrm;
   if (bad) atomicMin(bad_row, i);
-  for (uint32_t t = 0; t < dmax; ++t) {
-    float v = (t < d && nrm > 0.0) ? (float)((double)row[t] / nrm) : 0.0f;
-    if (t < dp32) xs32[(size_t)i * dp32 + t] = v;
-    if (xs16 && t < dp16) {
-      const _Float16 hi = (_Float16)v;
-      xs16[(size_t)i * s16 + t] = hi;
-      if (split16) xs16[(size_t)i * s16 + dp16 + t] = (_Float16)(v - (float)hi);
+  // the copies in 16-byte stores (dp32 is a multiple of 8 and dp16 of 16, so rows stay aligned)
+  auto val = [&](uint32_t t) -> float { return (t < d && nrm > 0.0) ? (float)((double)row[t] / nrm) : 0.0f; };
+  for (uint32_t t = 0; t < dp32; t += 4) {
+    const f32x4 v = {val(t), val(t + 1), val(t + 2), val(t + 3)};
+    *reinterpret_cast<f32x4*>(xs32 + (size_t)i * dp32 + t) = v;
+  }
+  if (xs16) {
+    for (uint32_t t = 0; t < dp16; t += 8) {
+      f16x8 hi, lo;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const float v = val(t + c);
+        hi[c] = (_Float16)v;
+        lo[c] = (_Float16)(v - (float)hi[c]);
+      }
+      *reinterpret_cast<f16x8*>(xs16 + (size_t)i * s16 + t) = hi;
+      if (split16) *reinterpret_cast<f16x8*>(xs16 + (size_t)i * s16 + dp16 + t) = lo;
     }
   }
 }
@@ -884,14 +893,33 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
   }
   __syncthreads();
   const uint32_t filled = s_fill;  // = total
-  int P = 2;
-  while ((uint32_t)P < filled) P <<= 1;
-  for (int e = (int)filled + tid_x(); e < P; e += 64) {
-    s_sc[e] = -INFINITY;
-    s_id[e] = NONE;
+  if (filled <= 64) {
+    // one entry per thread: its rank is the number of entries ranking before it (the candidates
+    // are distinct, so the ranks are a permutation), and it moves straight to that place
+    const uint32_t e = tid_x();
+    double sc_e = 0.0;
+    uint32_t id_e = 0, rank = 0;
+    if (e < filled) {
+      sc_e = s_sc[e];
+      id_e = s_id[e];
+      for (uint32_t f = 0; f < filled; ++f) rank += ranks_before(s_sc[f], s_id[f], sc_e, id_e) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (e < filled) {
+      s_sc[rank] = sc_e;
+      s_id[rank] = id_e;
+    }
+    __syncthreads();
+  } else {
+    int P = 2;
+    while ((uint32_t)P < filled) P <<= 1;
+    for (int e = (int)filled + tid_x(); e < P; e += 64) {
+      s_sc[e] = -INFINITY;
+      s_id[e] = NONE;
+    }
+    __syncthreads();
+    bitonic_lds<64>(s_sc, nullptr, s_id, P);
   }
-  __syncthreads();
-  bitonic_lds<64>(s_sc, nullptr, s_id, P);
 
   bool ok;
   if (all_admitted) {
@@ -1653,17 +1681,20 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     // Rows that failed the first pass (admission thresholds above their k-th similarity, or lists
     // too short for their ties) are rescanned without thresholds and with longer lists, as one
     // batched scan over the row list (wide candidate splits keep the few rows' scan parallel); only
-    // rows that fail again (exact ties) go to the full-row kernel.
+    // rows that fail again (exact ties) go to the full-row kernel.  (Learned thresholds lowered by
+    // the model's offset once more were measured slower: the rows the model mispredicts are the
+    // ones that fail again, and the full-row kernel is slow.)
     uint32_t nfb = 0;
     HIP_TRY(hipMemcpyAsync(&nfb, counters + 1, sizeof nfb, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (nfb > 0) {
       CoarseFn fn2 = coarse_select(P.prec, P.J, list_template(kp2));
       if (!fn2) return fail(COSTAL_E_UNSUPPORTED, "no coarse kernel for d=%u", d);
-      // about one workgroup per CU: more splits would multiply the candidates each row rescores
+      // about one workgroup per CU: the rows are few, so splits are what fills the GPU, but every
+      // split fills its lists from scratch (threshold-free), so more than one per CU was slower
       const uint32_t blocks = (nfb + P.QB - 1) / P.QB;
       uint32_t ns = ((uint32_t)std::max(D.cus, 1) + blocks - 1) / blocks;
-      ns = std::min<uint32_t>(ns, 16);
+      ns = std::min<uint32_t>(ns, 32);
       ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, ntile / 2));
       ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, 2048 / (2 * kp2)));
       // the rescan's lists fit in the list buffers (nq_entries entries, nq_slots lists; one split

@@ -95,7 +95,8 @@ typedef struct costal_knn_stats {
   uint64_t blocks_skipped;    /* always 0 for cosine selection */
   uint64_t blocks_total;      /* row blocks x column tiles */
   uint64_t candidates;        /* K' = min(k + margin, n - 1) */
-  uint64_t col_splits;        /* candidate-range splits used */
+  uint64_t col_splits;        /* candidate-range splits used (2 for the balanced launch: a row's
+                                 range in at most two pieces) */
   uint64_t short_rows;        /* rows with fewer than k admitted candidates (thresholds only) */
   uint64_t path;              /* 0 fused MFMA scan, 1 exact row scan (large K', tanimoto), 2 wide rows */
   uint64_t inserts;           /* candidate-list inserts made by the fused scan */

@@ -1537,10 +1537,10 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     if (f0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(f0),
                                                            64 * P.NW, 0) == hipSuccess && per_cu > 0) {
       const uint64_t slots = (uint64_t)per_cu * (uint64_t)std::max(D.cus, 1);
-      const uint64_t nblk = (n + P.QB - 1) / P.QB, nt = (n + P.TC - 1) / P.TC;
-      if (nblk >= slots) {
-        const uint64_t units = nblk * nt;
-        const uint64_t per = std::max<uint64_t>((units + slots - 1) / slots, nt);
+      const uint64_t nblk = ((uint64_t)n + P.QB - 1) / P.QB, nt = ((uint64_t)n + P.TC - 1) / P.TC;
+      const uint64_t units = nblk * nt;
+      const uint64_t per = std::max<uint64_t>((units + slots - 1) / slots, nt);
+      if (nblk >= slots && per <= 0xffffffffull) {
         units_per_wg = (uint32_t)per;
         grid_bal = (uint32_t)((units + per - 1) / per);
         P.nsplit = 2;
@@ -1887,7 +1887,12 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
     std::vector<uint32_t> pad_first(S), row_first(S), items;
     uint32_t pad = 0, base = 0, max_split = 1, max_kp = 1;
     uint64_t slots = 0, entries = 0, blocks_total = 0;
-    const uint32_t target_cols = 8192;  // candidates per work item: items of similar cost
+    // Untuned samples are cut into work items of about 8192 candidates (items of similar cost)
+    // only when the batch is small; a batch of at least 2048 query blocks fills the GPU with whole
+    // samples, and every split would fill its candidate lists again.
+    uint64_t batch_blocks = 0;
+    for (uint32_t i = 0; i < S; ++i) batch_blocks += (smp[i].n + QB - 1) / QB;
+    const uint32_t target_cols = batch_blocks >= 2048 ? 0xffffffffu : 8192;
     for (uint32_t i = 0; i < S; ++i) {
       const costal_sample& a = smp[i];
       const uint32_t n = (uint32_t)a.n;
@@ -1895,7 +1900,7 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
       const uint32_t kp = list_capacity(n, k, (uint64_t)mg);
       if (kp > FUSED_KP_MAX) return fail(COSTAL_E_UNSUPPORTED, "sample %u: k + ceil(margin/2) above %u", i, FUSED_KP_MAX);
       const uint32_t ntile = (n + TC - 1) / TC;
-      uint32_t ns = a.col_splits ? a.col_splits : (n + target_cols - 1) / target_cols;
+      uint32_t ns = a.col_splits ? a.col_splits : (uint32_t)(((uint64_t)n + target_cols - 1) / target_cols);
       ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, ntile));
       ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, 2048 / (2 * kp)));
       const uint32_t tiles_per = (ntile + ns - 1) / ns;

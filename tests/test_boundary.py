@@ -157,3 +157,33 @@ def test_lcg_matches_rust_generator():
     for _ in range(5):
         s = (s * 6364136223846793005 + 1442695040888963407) % 2**64
         assert g.next_f32() == np.float32(s >> 40) / np.float32(1 << 24)
+
+
+def test_tuner_choice_is_stable_under_timing_noise():
+    """Configurations within 3% of the fastest are ties, settled by the shortest lists, then split
+    FP16, then tuned splits, so one noisy timing run does not flip the plan."""
+    from costal_amd import tuner
+
+    res = [
+        {"precision": "fp16x3", "margin": 15, "col_splits": 0, "ms": 1.00},
+        {"precision": "fp16x3", "margin": 7, "col_splits": 4, "ms": 1.02},
+        {"precision": "fp16x3", "margin": 7, "col_splits": 0, "ms": 1.025},
+        {"precision": "fp32", "margin": 7, "col_splits": 0, "ms": 1.01},
+        {"precision": "fp16x3", "margin": 0, "col_splits": 0, "ms": 1.20},
+    ]
+    assert tuner.choose(res) == res[2]
+    # a clear winner is kept whatever its knobs
+    res[0]["ms"] = 0.90
+    assert tuner.choose(res) == res[0]
+
+
+def test_batch_plan_drops_splits_when_samples_fill_the_gpu(tmp_path):
+    from costal_amd import tuner
+
+    tn = tuner.Tuner(str(tmp_path / "tune.json"))
+    for n, d in [(300, 32), (200000, 40), (100000, 48)]:
+        tn.table[tuner.bucket(n, d, 15)] = {"best": {"precision": "fp16x3", "margin": 7, "col_splits": 4}}
+    prec, margins, splits = tn.plan_batch([(300, 32)], 15)
+    assert (prec, margins, splits) == ("fp16x3", [7], [4])
+    _, _, splits = tn.plan_batch([(300, 32), (200000, 40), (100000, 48)], 15)  # 2,348 query blocks
+    assert splits == [1, 1, 1]

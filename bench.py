@@ -9,7 +9,7 @@ HBM.
 
 Forms timed side by side: plain FP32 MFMA tiles; split-FP16 MFMA tiles (hi.hi + hi.lo + lo.hi);
 split-FP16 tiles with learned BF16 admission thresholds (the headline ``value``; the threshold
-model is trained before timing on 200,000 other cells of the same mixture).  The line reports that
+model is trained before timing on as many other cells of the same mixture as it serves).  The line reports that
 all forms give identical neighbors and weights.  ``roofline`` is for the dominant kernel (the fused
 tile + scan kernel), its time measured live with HIP events on the stream it runs on.
 ``cpu_baseline`` times the CPU oracle's blocked form (a C restatement of the reference's run_blocked,
@@ -46,15 +46,15 @@ FORM_DESC = {
 PRECISION_OF = {"fp32": "fp32", "fp16": "fp16", "fp16x3": "fp16x3", "fp16x3+learned": "fp16x3"}
 # HBM bytes per launch of the dominant kernel: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB), the scale
 # factors from a 1 GiB device copy under the same counters; two rocprofv3 --pmc runs of this script
-# (--forms fp16x3+learned --steps 2), mean over the dispatches of the first pass.
+# (--forms fp16x3+learned --steps 2), mean over the two steps' first passes.
 MEASURED_TRAFFIC = {
-    ("rare_396460x15", "fp16x3+learned"): ((1_282_474 + 1_283_432) / 2 * 2.0 + 68_487) * 1024,
+    ("rare_396460x15", "fp16x3+learned"): ((1_283_844 + 1_285_254) / 2 * 2.0 + 68_493) * 1024,
 }
 WORKLOADS = {
     # name: (k, forms, train the threshold model on this many other cells)
-    "rare_396460x15": (10, "fp32,fp16x3,fp16x3+learned", 200_000),
+    "rare_396460x15": (10, "fp32,fp16x3,fp16x3+learned", 396_460),
     "cytometry_86864x39": (10, "fp32,fp16,fp16x3", 0),
-    "learned_1000000x40": (30, "fp32,fp16x3,fp16x3+learned", 200_000),
+    "learned_1000000x40": (30, "fp32,fp16x3,fp16x3+learned", 1_000_000),
     "batched_64": (15, "fp32,fp16x3", 0),
     "counts_3605x2000": (10, "fp32", 0),
 }
@@ -82,7 +82,7 @@ def load_workload(name: str, train_cells: int):
     if name == "cytometry_86864x39":
         return synth.cytometry_86864x39(), None
     if name == "learned_1000000x40":
-        return synth.learned_1000000x40(), (lambda: synth.learned_train_200000x40(n=train_cells))
+        return synth.learned_1000000x40(), (lambda: synth.learned_train_x40(n=train_cells))
     if name == "counts_3605x2000":
         return np.ascontiguousarray(synth.counts_3605x2000(), dtype=np.float32), None
     if name == "batched_64":

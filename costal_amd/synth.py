@@ -15,7 +15,7 @@ Workloads (cells x features):
 - ``batched_64``: 64 samples, N ~ floor(exp(U(ln 2000, ln 60000))), d ~ U{32..48}, 10-30
   components each, arcsinh, float32. k = 15.
 - ``learned_1000000x40``: 40 components, arcsinh, float32, k = 30; the threshold model trains on
-  200,000 cells drawn from the same mixture with another sampling seed.
+  cells drawn from the same mixture with another sampling seed (``learned_train_x40``).
 
 ``clustered`` reproduces the data generator of the reference's Python tests
 (ike12th/CosTaL tests/test_knn.py:20-24) and ``lcg_clustered`` / ``lcg_uniform`` the Rust unit-test
@@ -118,7 +118,7 @@ def learned_1000000x40(n: int = 1_000_000) -> np.ndarray:
     return learned_draw(n, sample_seed=40)
 
 
-def learned_train_200000x40(n: int = 200_000) -> np.ndarray:
+def learned_train_x40(n: int = 1_000_000) -> np.ndarray:
     return learned_draw(n, sample_seed=41)
 
 

@@ -122,9 +122,9 @@ __global__ void prep_kernel(const float* __restrict__ x, uint32_t n, uint32_t d,
   if (i >= n_pad) return;
   const uint32_t s16 = split16 ? 2 * dp16 : dp16;
   if (i >= n) {
-    for (uint32_t t = 0; t < dp32; ++t) xs32[(size_t)i * dp32 + t] = 0.0f;
+    for (uint32_t t = 0; t < dp32; t += 4) *reinterpret_cast<f32x4*>(xs32 + (size_t)i * dp32 + t) = f32x4{};
     if (xs16)
-      for (uint32_t t = 0; t < s16; ++t) xs16[(size_t)i * s16 + t] = (_Float16)0.0f;
+      for (uint32_t t = 0; t < s16; t += 8) *reinterpret_cast<f16x8*>(xs16 + (size_t)i * s16 + t) = f16x8{};
     return;
   }
   const float* row = x + (size_t)i * d;
@@ -1127,33 +1127,67 @@ __global__ void prep_batch_kernel(const BatchSample* __restrict__ samples, const
   const BatchSample& S = samples[sidx];
   const uint32_t r = g - S.row_pad;
   const uint32_t s16 = split16 ? 2 * dp16 : dp16;
-  const uint32_t dmax = dp32 > dp16 ? dp32 : dp16;
+  // 16-byte stores throughout (dp32 is a multiple of 8 and dp16 of 16, so rows stay aligned)
   if (r >= S.n) {
-    for (uint32_t t = 0; t < dp32; ++t) xs32[(size_t)g * dp32 + t] = 0.0f;
+    for (uint32_t t = 0; t < dp32; t += 4) *reinterpret_cast<f32x4*>(xs32 + (size_t)g * dp32 + t) = f32x4{};
     if (xs16)
-      for (uint32_t t = 0; t < s16; ++t) xs16[(size_t)g * s16 + t] = (_Float16)0.0f;
+      for (uint32_t t = 0; t < s16; t += 8) *reinterpret_cast<f16x8*>(xs16 + (size_t)g * s16 + t) = f16x8{};
     return;
   }
+  // the row's bits (batched samples have d <= 64) in registers, read once (the non-finite test
+  // reads them as integers): 16-byte loads when the row is aligned to them, the feature loops
+  // unrolled to 64 so every index is static
+  constexpr int DMAX = 64;
   const uint32_t d = S.d;
-  const uint32_t* row_bits = reinterpret_cast<const uint32_t*>(S.x + (size_t)r * d);
+  const uint32_t* row = reinterpret_cast<const uint32_t*>(S.x + (size_t)r * d);
+  uint32_t rb[DMAX];
+  if ((d & 3) == 0 && (reinterpret_cast<uintptr_t>(row) & 15) == 0) {
+#pragma unroll
+    for (int t = 0; t < DMAX; t += 4) {
+      uint4 q = {0u, 0u, 0u, 0u};
+      if ((uint32_t)t < d) q = *reinterpret_cast<const uint4*>(row + t);
+      rb[t] = q.x;
+      rb[t + 1] = q.y;
+      rb[t + 2] = q.z;
+      rb[t + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int t = 0; t < DMAX; ++t) rb[t] = (uint32_t)t < d ? row[t] : 0u;
+  }
   double sum = 0.0;
   bool badrow = false;
-  for (uint32_t t = 0; t < d; ++t) {
-    const uint32_t b = row_bits[t];
-    badrow |= (b & 0x7f800000u) == 0x7f800000u;
-    const float v = __uint_as_float(b);
-    sum += (double)v * (double)v;
+#pragma unroll
+  for (int t = 0; t < DMAX; ++t) {
+    if ((uint32_t)t < d) {
+      badrow |= (rb[t] & 0x7f800000u) == 0x7f800000u;
+      const float v = __uint_as_float(rb[t]);
+      sum += (double)v * (double)v;
+    }
   }
   const double nrm = sqrt(sum);
   S.norms[r] = nrm;
   if (badrow) atomicMin(bad, ((unsigned long long)sidx << 32) | r);
-  for (uint32_t t = 0; t < dmax; ++t) {
-    const float v = (t < d && nrm > 0.0) ? (float)((double)__uint_as_float(row_bits[t]) / nrm) : 0.0f;
-    if (t < dp32) xs32[(size_t)g * dp32 + t] = v;
-    if (xs16 && t < dp16) {
-      const _Float16 hi = (_Float16)v;
-      xs16[(size_t)g * s16 + t] = hi;
-      if (split16) xs16[(size_t)g * s16 + dp16 + t] = (_Float16)(v - (float)hi);
+  float nv[DMAX];
+#pragma unroll
+  for (int t = 0; t < DMAX; ++t)
+    nv[t] = ((uint32_t)t < d && nrm > 0.0) ? (float)((double)__uint_as_float(rb[t]) / nrm) : 0.0f;
+#pragma unroll
+  for (int t = 0; t < DMAX; t += 4)
+    if ((uint32_t)t < dp32) *reinterpret_cast<f32x4*>(xs32 + (size_t)g * dp32 + t) = f32x4{nv[t], nv[t + 1], nv[t + 2], nv[t + 3]};
+  if (xs16) {
+#pragma unroll
+    for (int t = 0; t < DMAX; t += 8) {
+      if ((uint32_t)t < dp16) {
+        f16x8 hi, lo;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          hi[c] = (_Float16)nv[t + c];
+          lo[c] = (_Float16)(nv[t + c] - (float)hi[c]);
+        }
+        *reinterpret_cast<f16x8*>(xs16 + (size_t)g * s16 + t) = hi;
+        if (split16) *reinterpret_cast<f16x8*>(xs16 + (size_t)g * s16 + dp16 + t) = lo;
+      }
     }
   }
 }

@@ -1260,6 +1260,43 @@ __global__ __launch_bounds__(256) void rowscan_batch_kernel(
 // ------------------------------------------------------------------------------------------------
 // weight::reweight (weight.rs:41-60): tanimoto / cosine.  Jaccard runs on the host.
 
+// Admission thresholds of the rescan, by row: fb_lo - eps_list rounded down to f32, or -inf for a
+// row with fewer than k candidates (fb_lo = -inf).
+__global__ void rescan_thresholds_kernel(const uint32_t* __restrict__ rows, const double* __restrict__ lo, uint32_t nrows,
+                                         double eps_list, float* __restrict__ th) {
+  const uint32_t q = bid_x() * bdim_x() + tid_x();
+  if (q >= nrows) return;
+  float t = -INFINITY;
+  if (lo[q] > -INFINITY) {
+    const double want = lo[q] - eps_list;
+    t = (float)want;
+    if ((double)t > want) {  // one f32 step down (want is finite: |fb_lo| <= 1 + eps32)
+      const uint32_t b = __float_as_uint(t);
+      t = t > 0.0f ? __uint_as_float(b - 1) : t < 0.0f ? __uint_as_float(b + 1) : -__uint_as_float(1u);
+    }
+  }
+  th[rows[q]] = t;
+}
+
+// out = the rows with a finite threshold, then the others (one workgroup; the order within each
+// kind is not kept).
+__global__ __launch_bounds__(1024) void partition_rows_kernel(const uint32_t* __restrict__ rows, uint32_t nrows,
+                                                              const float* __restrict__ th, uint32_t* __restrict__ out) {
+  __shared__ uint32_t s_cnt[3];
+  if (tid_x() < 3) s_cnt[tid_x()] = 0;
+  __syncthreads();
+  for (uint32_t q = tid_x(); q < nrows; q += bdim_x())
+    if (th[rows[q]] > -INFINITY) atomicAdd(&s_cnt[0], 1u);
+  __syncthreads();
+  const uint32_t with = s_cnt[0];
+  for (uint32_t q = tid_x(); q < nrows; q += bdim_x()) {
+    const uint32_t r = rows[q];
+    const bool has = th[r] > -INFINITY;
+    const uint32_t pos = has ? atomicAdd(&s_cnt[1], 1u) : with + atomicAdd(&s_cnt[2], 1u);
+    out[pos] = r;
+  }
+}
+
 __global__ void reweight_kernel(uint64_t n, uint32_t k, const uint32_t* __restrict__ idx,
                                 const double* __restrict__ cos, const double* __restrict__ norms,
                                 uint32_t weight, double* __restrict__ out,
@@ -1610,6 +1647,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     c.take<double>(n);
     c.take<uint32_t>(64);
     if (P.fused && p.mlp_params && !p.row_threshold) c.take<float>(n);
+    if (P.fused) c.take<float>(n);
     c.take<float>(score_floats);
     bytes = c.off + 256;
   }
@@ -1633,6 +1671,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   uint32_t* counters = c.take<uint32_t>(64);
   const bool learned = P.fused && p.mlp_params && !p.row_threshold;
   float* th_ws = learned ? c.take<float>(n) : nullptr;
+  float* th_rescan = P.fused ? c.take<float>(n) : nullptr;
   float* scores = P.wide ? c.take<float>(score_floats) : nullptr;
   double* norms = norms_out ? norms_out : norms_ws;
 
@@ -1738,7 +1777,18 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
       ns = std::max<uint32_t>(ns, 1);
       const uint32_t tiles_per = (ntile + ns - 1) / ns;
       ns = (ntile + tiles_per - 1) / tiles_per;
-      rc = scan_pass(fn2, kp2, fb_rows, nfb, nullptr, ns, tiles_per * P.TC, 0, fb2_rows, fb2_lo, counters + 3,
+      // A row that failed only the certificate has its k-th exact score s_k among the first
+      // pass's candidates (fb_lo = s_k - eps32), and its true k-th is at least that: its rescan
+      // admits only scores above fb_lo - eps_list, which every true neighbor's list score exceeds.
+      // Rows with fewer than k candidates rescan without a threshold.
+      rescan_thresholds_kernel<<<(nfb + 255) / 256, 256, 0, s>>>(fb_rows, fb_lo, nfb, eps_list, th_rescan);
+      HIP_TRY(hipGetLastError());
+      // rows with a threshold first, then the rest: a query block then holds rows of one kind, and
+      // the quick ones do not wait for the threshold-free ones (fb2_rows is free until the rescan)
+      partition_rows_kernel<<<1, 1024, 0, s>>>(fb_rows, nfb, th_rescan, fb2_rows);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(fb_rows, fb2_rows, (size_t)nfb * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+      rc = scan_pass(fn2, kp2, fb_rows, nfb, th_rescan, ns, tiles_per * P.TC, 0, fb2_rows, fb2_lo, counters + 3,
                      counters + 4);
       if (rc) return rc;
     }

@@ -945,6 +945,81 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
   }
 }
 
+// Two queries per 64-thread workgroup, one per half-wave, for rows of at most 32 list entries:
+// the same gather, f64 rescoring, rank placement, certificate and outputs as rescore_row, with
+// twice the rows in flight (the work is latency-bound).  Every thread meets every barrier.
+__global__ __launch_bounds__(64) void rescore_pair_kernel(
+    const float* __restrict__ x, uint32_t d, const double* __restrict__ norms, uint32_t k, int kp,
+    uint32_t nsplit, const uint32_t* __restrict__ rowlist, uint32_t nq, const uint32_t* __restrict__ cand_idx,
+    const uint32_t* __restrict__ cand_cnt, const float* __restrict__ cand_floor, int all_admitted,
+    double eps_list, double eps32, uint32_t* __restrict__ idx, double* __restrict__ cos_out,
+    double* __restrict__ wts, uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo,
+    uint32_t* __restrict__ fb_count, uint32_t* __restrict__ short_count) {
+  __shared__ double s_sc[2][32];
+  __shared__ uint32_t s_id[2][32];
+  __shared__ uint32_t s_fill[2];
+  const uint32_t half = tid_x() >> 5, lane = tid_x() & 31;
+  const uint32_t qq = bid_x() * 2 + half;
+  const bool valid = qq < nq;
+  const uint32_t i = valid ? (rowlist ? rowlist[qq] : qq) : 0;
+  const int tot_slots = (int)nsplit * kp;  // <= 32
+  double ni = 0.0, floor_ = -INFINITY;
+  uint32_t total = 0;
+  if (valid) {
+    ni = norms[i];
+    for (uint32_t sp = 0; sp < nsplit; ++sp) {
+      total += cand_cnt[(size_t)qq * nsplit + sp];
+      floor_ = fmax(floor_, (double)cand_floor[(size_t)qq * nsplit + sp]);
+    }
+  }
+  if (lane == 0) s_fill[half] = 0;
+  __syncthreads();
+  if (valid && (int)lane < tot_slots) {
+    const uint32_t sp = lane / (uint32_t)kp, r = lane % (uint32_t)kp;
+    const size_t slot = (size_t)qq * nsplit + sp;
+    if (r < cand_cnt[slot]) {
+      const uint32_t j = cand_idx[slot * kp + r];
+      const uint32_t pos = atomicAdd(&s_fill[half], 1u);
+      s_sc[half][pos] = cos64(dot64(x + (size_t)i * d, x + (size_t)j * d, d), ni, norms[j]);
+      s_id[half][pos] = j;
+    }
+  }
+  __syncthreads();
+  const uint32_t filled = s_fill[half];
+  double sc_e = 0.0;
+  uint32_t id_e = 0, rank = 0;
+  if (lane < filled) {
+    sc_e = s_sc[half][lane];
+    id_e = s_id[half][lane];
+    for (uint32_t f = 0; f < filled; ++f) rank += ranks_before(s_sc[half][f], s_id[half][f], sc_e, id_e) ? 1u : 0u;
+  }
+  __syncthreads();
+  if (lane < filled) {
+    s_sc[half][rank] = sc_e;
+    s_id[half][rank] = id_e;
+  }
+  __syncthreads();
+  if (!valid) return;
+  bool ok;
+  if (all_admitted) ok = true;
+  else if (total < k) ok = false;
+  else ok = s_sc[half][k - 1] > floor_ + eps_list;
+  if (ok) {
+    for (uint32_t t = lane; t < k; t += 32) {
+      const uint32_t j = s_id[half][t];
+      const double c = s_sc[half][t];
+      idx[(size_t)i * k + t] = j;
+      cos_out[(size_t)i * k + t] = c;
+      if (wts) wts[(size_t)i * k + t] = tani_from_cos(ni, norms[j], c);
+    }
+  } else if (lane == 0) {
+    const uint32_t pp = atomicAdd(fb_count, 1u);
+    fb_rows[pp] = i;
+    fb_lo[pp] = total >= k ? s_sc[half][k - 1] - eps32 : -INFINITY;
+    if (total < k) atomicAdd(short_count, 1u);
+  }
+}
+
 template <int CMAX>
 __global__ __launch_bounds__(64) void rescore_kernel(
     const float* __restrict__ x, uint32_t d, const double* __restrict__ norms, uint32_t n,
@@ -1706,7 +1781,11 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   rescore_kernel<CM><<<nq, 64, 0, s>>>(x, d, norms, n, k, (int)kp, nlists, rows, nq, cand_idx, cand_cnt,     \
                                        cand_floor, all_admitted, eps_list, eps32, idx, cos, wts, fbr, fbl,  \
                                        fbc, shc)
-    if (slots <= 64) RESCORE(64);
+    if (slots <= 32) {
+      rescore_pair_kernel<<<(nq + 1) / 2, 64, 0, s>>>(x, d, norms, k, (int)kp, nlists, rows, nq, cand_idx, cand_cnt,
+                                                      cand_floor, all_admitted, eps_list, eps32, idx, cos, wts, fbr,
+                                                      fbl, fbc, shc);
+    } else if (slots <= 64) RESCORE(64);
     else if (slots <= 128) RESCORE(128);
     else if (slots <= 256) RESCORE(256);
     else if (slots <= 512) RESCORE(512);

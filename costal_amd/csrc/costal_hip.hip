@@ -947,7 +947,74 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
 
 // Two queries per 64-thread workgroup, one per half-wave, for rows of at most 32 list entries:
 // the same gather, f64 rescoring, rank placement, certificate and outputs as rescore_row, with
-// twice the rows in flight (the work is latency-bound).  Every thread meets every barrier.
+// twice the rows in flight (the work is latency-bound).  Both halves call this together with their
+// own row (valid = false for a missing one), so every thread meets every barrier.
+__device__ void rescore_half(double* __restrict__ s_sc, uint32_t* __restrict__ s_id, uint32_t* __restrict__ s_fill,
+                             bool valid, uint32_t lane, const float* __restrict__ x, uint32_t d,
+                             const double* __restrict__ norms, uint32_t k, int kp, uint32_t nsplit, uint32_t qq,
+                             uint32_t i, const uint32_t* __restrict__ cand_idx, const uint32_t* __restrict__ cand_cnt,
+                             const float* __restrict__ cand_floor, int all_admitted, double eps_list, double eps32,
+                             uint32_t* __restrict__ idx, double* __restrict__ cos_out, double* __restrict__ wts,
+                             uint32_t fb_tag, uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo,
+                             uint32_t* __restrict__ fb_count, uint32_t* __restrict__ short_count) {
+  const int tot_slots = (int)nsplit * kp;  // <= 32
+  double ni = 0.0, floor_ = -INFINITY;
+  uint32_t total = 0;
+  if (valid) {
+    ni = norms[i];
+    for (uint32_t sp = 0; sp < nsplit; ++sp) {
+      total += cand_cnt[(size_t)qq * nsplit + sp];
+      floor_ = fmax(floor_, (double)cand_floor[(size_t)qq * nsplit + sp]);
+    }
+  }
+  if (lane == 0) *s_fill = 0;
+  __syncthreads();
+  if (valid && (int)lane < tot_slots) {
+    const uint32_t sp = lane / (uint32_t)kp, r = lane % (uint32_t)kp;
+    const size_t slot = (size_t)qq * nsplit + sp;
+    if (r < cand_cnt[slot]) {
+      const uint32_t j = cand_idx[slot * kp + r];
+      const uint32_t pos = atomicAdd(s_fill, 1u);
+      s_sc[pos] = cos64(dot64(x + (size_t)i * d, x + (size_t)j * d, d), ni, norms[j]);
+      s_id[pos] = j;
+    }
+  }
+  __syncthreads();
+  const uint32_t filled = *s_fill;
+  double sc_e = 0.0;
+  uint32_t id_e = 0, rank = 0;
+  if (lane < filled) {
+    sc_e = s_sc[lane];
+    id_e = s_id[lane];
+    for (uint32_t f = 0; f < filled; ++f) rank += ranks_before(s_sc[f], s_id[f], sc_e, id_e) ? 1u : 0u;
+  }
+  __syncthreads();
+  if (lane < filled) {
+    s_sc[rank] = sc_e;
+    s_id[rank] = id_e;
+  }
+  __syncthreads();
+  if (!valid) return;
+  bool ok;
+  if (all_admitted) ok = true;
+  else if (total < k) ok = false;
+  else ok = s_sc[k - 1] > floor_ + eps_list;
+  if (ok) {
+    for (uint32_t t = lane; t < k; t += 32) {
+      const uint32_t j = s_id[t];
+      const double c = s_sc[t];
+      idx[(size_t)i * k + t] = j;
+      cos_out[(size_t)i * k + t] = c;
+      if (wts) wts[(size_t)i * k + t] = tani_from_cos(ni, norms[j], c);
+    }
+  } else if (lane == 0) {
+    const uint32_t pp = atomicAdd(fb_count, 1u);
+    fb_rows[pp] = fb_tag;
+    fb_lo[pp] = total >= k ? s_sc[k - 1] - eps32 : -INFINITY;
+    if (total < k) atomicAdd(short_count, 1u);
+  }
+}
+
 __global__ __launch_bounds__(64) void rescore_pair_kernel(
     const float* __restrict__ x, uint32_t d, const double* __restrict__ norms, uint32_t k, int kp,
     uint32_t nsplit, const uint32_t* __restrict__ rowlist, uint32_t nq, const uint32_t* __restrict__ cand_idx,
@@ -962,62 +1029,9 @@ __global__ __launch_bounds__(64) void rescore_pair_kernel(
   const uint32_t qq = bid_x() * 2 + half;
   const bool valid = qq < nq;
   const uint32_t i = valid ? (rowlist ? rowlist[qq] : qq) : 0;
-  const int tot_slots = (int)nsplit * kp;  // <= 32
-  double ni = 0.0, floor_ = -INFINITY;
-  uint32_t total = 0;
-  if (valid) {
-    ni = norms[i];
-    for (uint32_t sp = 0; sp < nsplit; ++sp) {
-      total += cand_cnt[(size_t)qq * nsplit + sp];
-      floor_ = fmax(floor_, (double)cand_floor[(size_t)qq * nsplit + sp]);
-    }
-  }
-  if (lane == 0) s_fill[half] = 0;
-  __syncthreads();
-  if (valid && (int)lane < tot_slots) {
-    const uint32_t sp = lane / (uint32_t)kp, r = lane % (uint32_t)kp;
-    const size_t slot = (size_t)qq * nsplit + sp;
-    if (r < cand_cnt[slot]) {
-      const uint32_t j = cand_idx[slot * kp + r];
-      const uint32_t pos = atomicAdd(&s_fill[half], 1u);
-      s_sc[half][pos] = cos64(dot64(x + (size_t)i * d, x + (size_t)j * d, d), ni, norms[j]);
-      s_id[half][pos] = j;
-    }
-  }
-  __syncthreads();
-  const uint32_t filled = s_fill[half];
-  double sc_e = 0.0;
-  uint32_t id_e = 0, rank = 0;
-  if (lane < filled) {
-    sc_e = s_sc[half][lane];
-    id_e = s_id[half][lane];
-    for (uint32_t f = 0; f < filled; ++f) rank += ranks_before(s_sc[half][f], s_id[half][f], sc_e, id_e) ? 1u : 0u;
-  }
-  __syncthreads();
-  if (lane < filled) {
-    s_sc[half][rank] = sc_e;
-    s_id[half][rank] = id_e;
-  }
-  __syncthreads();
-  if (!valid) return;
-  bool ok;
-  if (all_admitted) ok = true;
-  else if (total < k) ok = false;
-  else ok = s_sc[half][k - 1] > floor_ + eps_list;
-  if (ok) {
-    for (uint32_t t = lane; t < k; t += 32) {
-      const uint32_t j = s_id[half][t];
-      const double c = s_sc[half][t];
-      idx[(size_t)i * k + t] = j;
-      cos_out[(size_t)i * k + t] = c;
-      if (wts) wts[(size_t)i * k + t] = tani_from_cos(ni, norms[j], c);
-    }
-  } else if (lane == 0) {
-    const uint32_t pp = atomicAdd(fb_count, 1u);
-    fb_rows[pp] = i;
-    fb_lo[pp] = total >= k ? s_sc[half][k - 1] - eps32 : -INFINITY;
-    if (total < k) atomicAdd(short_count, 1u);
-  }
+  rescore_half(s_sc[half], s_id[half], &s_fill[half], valid, lane, x, d, norms, k, kp, nsplit, qq, i, cand_idx,
+               cand_cnt, cand_floor, all_admitted, eps_list, eps32, idx, cos_out, wts, i, fb_rows, fb_lo, fb_count,
+               short_count);
 }
 
 template <int CMAX>
@@ -1314,6 +1328,26 @@ __global__ __launch_bounds__(64) void rescore_batch_kernel(
   rescore_row<CMAX>(s_sc, s_id, S.x, S.d, S.norms, k, S.kp, nlists, i, i, cand_idx + S.entry_base,
                     cand_cnt + S.slot_base, cand_floor + S.slot_base, all_admitted, eps_list, eps32, S.idx, S.cos,
                     S.wts, g, fb_rows, fb_lo, fb_count, short_count);
+}
+
+// rescore_batch_kernel for batches whose rows hold at most 32 list entries: two rows per wave
+__global__ __launch_bounds__(64) void rescore_batch_pair_kernel(
+    const BatchSample* __restrict__ samples, const uint32_t* __restrict__ row_first, uint32_t nsamples,
+    uint32_t total_rows, uint32_t k, double eps_list, double eps32, const uint32_t* __restrict__ cand_idx,
+    const uint32_t* __restrict__ cand_cnt, const float* __restrict__ cand_floor, uint32_t* __restrict__ fb_rows,
+    double* __restrict__ fb_lo, uint32_t* __restrict__ fb_count, uint32_t* __restrict__ short_count) {
+  __shared__ double s_sc[2][32];
+  __shared__ uint32_t s_id[2][32];
+  __shared__ uint32_t s_fill[2];
+  const uint32_t half = tid_x() >> 5, lane = tid_x() & 31;
+  const uint32_t g = bid_x() * 2 + half;
+  const bool valid = g < total_rows;
+  const BatchSample& S = samples[valid ? find_sample(row_first, nsamples, g) : 0];
+  const uint32_t i = valid ? g - S.row_base : 0;
+  const int all_admitted = (S.nsplit == 1 && (uint32_t)S.kp >= S.n - 1) ? 1 : 0;
+  rescore_half(s_sc[half], s_id[half], &s_fill[half], valid, lane, S.x, S.d, S.norms, k, S.kp, 2 * S.nsplit, i, i,
+               cand_idx + S.entry_base, cand_cnt + S.slot_base, cand_floor + S.slot_base, all_admitted, eps_list,
+               eps32, S.idx, S.cos, S.wts, g, fb_rows, fb_lo, fb_count, short_count);
 }
 
 template <int CAP>
@@ -2188,7 +2222,11 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
                                                                 (uint32_t)k, eps_list, eps32, cand_idx, cand_cnt, \
                                                                 cand_floor, fb_rows, fb_lo, counters + 2,          \
                                                                 counters + 3)
-      if (max_slots <= 64) RESCORE_B(64);
+      if (max_slots <= 32)
+        rescore_batch_pair_kernel<<<(uint32_t)((total_rows + 1) / 2), 64, 0, st>>>(
+            d_samples, d_row_first, S, (uint32_t)total_rows, (uint32_t)k, eps_list, eps32, cand_idx, cand_cnt,
+            cand_floor, fb_rows, fb_lo, counters + 2, counters + 3);
+      else if (max_slots <= 64) RESCORE_B(64);
       else if (max_slots <= 128) RESCORE_B(128);
       else if (max_slots <= 256) RESCORE_B(256);
       else if (max_slots <= 512) RESCORE_B(512);

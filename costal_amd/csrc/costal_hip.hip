@@ -1870,40 +1870,46 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     // rows that fail again (exact ties) go to the full-row kernel.  (Learned thresholds lowered by
     // the model's offset once more were measured slower: the rows the model mispredicts are the
     // ones that fail again, and the full-row kernel is slow.)
-    uint32_t nfb = 0;
-    HIP_TRY(hipMemcpyAsync(&nfb, counters + 1, sizeof nfb, hipMemcpyDeviceToHost, s));
+    uint32_t nfb_short[2] = {0, 0};  // rows that failed, and those of them with fewer than k candidates
+    HIP_TRY(hipMemcpyAsync(nfb_short, counters + 1, sizeof nfb_short, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    const uint32_t nfb = nfb_short[0], nwith = nfb_short[0] - nfb_short[1];
     if (nfb > 0) {
       CoarseFn fn2 = coarse_select(P.prec, P.J, list_template(kp2));
       if (!fn2) return fail(COSTAL_E_UNSUPPORTED, "no coarse kernel for d=%u", d);
-      // about one workgroup per CU: the rows are few, so splits are what fills the GPU, but every
-      // split fills its lists from scratch (threshold-free), so more than one per CU was slower
-      const uint32_t blocks = (nfb + P.QB - 1) / P.QB;
-      uint32_t ns = ((uint32_t)std::max(D.cus, 1) + blocks - 1) / blocks;
-      ns = std::min<uint32_t>(ns, 32);
-      ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, ntile / 2));
-      ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, 2048 / (2 * kp2)));
-      // the rescan's lists fit in the list buffers (nq_entries entries, nq_slots lists; one split
-      // always fits since nfb <= n)
-      ns = std::min<uint64_t>(ns, nq_entries / ((uint64_t)nfb * 2 * kp2));
-      ns = std::min<uint64_t>(ns, nq_slots / ((uint64_t)nfb * 2));
-      ns = std::max<uint32_t>(ns, 1);
-      const uint32_t tiles_per = (ntile + ns - 1) / ns;
-      ns = (ntile + tiles_per - 1) / tiles_per;
       // A row that failed only the certificate has its k-th exact score s_k among the first
       // pass's candidates (fb_lo = s_k - eps32), and its true k-th is at least that: its rescan
-      // admits only scores above fb_lo - eps_list, which every true neighbor's list score exceeds.
-      // Rows with fewer than k candidates rescan without a threshold.
+      // admits only scores above fb_lo - eps_list, which every true neighbor's list score exceeds,
+      // with lists of kp2 for its near-ties.  A row with fewer than k candidates (its admission
+      // threshold was too high) rescans without a threshold and with the first pass's lists.
       rescan_thresholds_kernel<<<(nfb + 255) / 256, 256, 0, s>>>(fb_rows, fb_lo, nfb, eps_list, th_rescan);
       HIP_TRY(hipGetLastError());
-      // rows with a threshold first, then the rest: a query block then holds rows of one kind, and
-      // the quick ones do not wait for the threshold-free ones (fb2_rows is free until the rescan)
-      partition_rows_kernel<<<1, 1024, 0, s>>>(fb_rows, nfb, th_rescan, fb2_rows);
-      HIP_TRY(hipGetLastError());
-      HIP_TRY(hipMemcpyAsync(fb_rows, fb2_rows, (size_t)nfb * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-      rc = scan_pass(fn2, kp2, fb_rows, nfb, th_rescan, ns, tiles_per * P.TC, 0, fb2_rows, fb2_lo, counters + 3,
-                     counters + 4);
-      if (rc) return rc;
+      // rows with a threshold first, then the rest, as two rescans (fb2_rows is free until then)
+      if (nwith > 0 && nwith < nfb) {
+        partition_rows_kernel<<<1, 1024, 0, s>>>(fb_rows, nfb, th_rescan, fb2_rows);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(fb_rows, fb2_rows, (size_t)nfb * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+      }
+      auto rescan = [&](CoarseFn f, uint32_t kpx, const uint32_t* rows, uint32_t nr, const float* th) -> int {
+        // about one workgroup per CU: the rows are few, so splits are what fills the GPU, but
+        // every split fills its lists from scratch, so more than one per CU was slower
+        const uint32_t blocks = (nr + P.QB - 1) / P.QB;
+        uint32_t ns = ((uint32_t)std::max(D.cus, 1) + blocks - 1) / blocks;
+        ns = std::min<uint32_t>(ns, 32);
+        ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, ntile / 2));
+        ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, 2048 / (2 * kpx)));
+        // the lists fit in the list buffers (nq_entries entries, nq_slots lists; one split always
+        // fits since nr <= n and kpx <= kp2)
+        ns = std::min<uint64_t>(ns, nq_entries / ((uint64_t)nr * 2 * kpx));
+        ns = std::min<uint64_t>(ns, nq_slots / ((uint64_t)nr * 2));
+        ns = std::max<uint32_t>(ns, 1);
+        const uint32_t tiles_per = (ntile + ns - 1) / ns;
+        ns = (ntile + tiles_per - 1) / tiles_per;
+        return scan_pass(f, kpx, rows, nr, th, ns, tiles_per * P.TC, 0, fb2_rows, fb2_lo, counters + 3,
+                         counters + 4);
+      };
+      if (nwith > 0 && (rc = rescan(fn2, kp2, fb_rows, nwith, th_rescan))) return rc;
+      if (nfb > nwith && (rc = rescan(fn, P.kp, fb_rows + nwith, nfb - nwith, nullptr))) return rc;
     }
     rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb2_rows, counters + 3,
                         0, fb2_lo, eps32, idx, cos, wts, grid_fb);

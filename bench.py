@@ -48,7 +48,7 @@ PRECISION_OF = {"fp32": "fp32", "fp16": "fp16", "fp16x3": "fp16x3", "fp16x3+lear
 # factors from a 1 GiB device copy under the same counters; two rocprofv3 --pmc runs of this script
 # (--forms fp16x3+learned --steps 2), mean over the two steps' first passes.
 MEASURED_TRAFFIC = {
-    ("rare_396460x15", "fp16x3+learned"): ((1_285_390 + 1_284_365) / 2 * 2.0 + 68_493) * 1024,
+    ("rare_396460x15", "fp16x3+learned"): ((1_285_230 + 1_278_944) / 2 * 2.0 + 68_493) * 1024,
 }
 WORKLOADS = {
     # name: (k, forms, train the threshold model on this many other cells)

@@ -32,7 +32,7 @@
 
 #include <hip/hip_runtime.h>
 
-#include "device_ids.h"
+#include "scan.h"
 
 #include <stdarg.h>
 #include <stdint.h>
@@ -50,15 +50,9 @@
 
 #define COSTAL_VERSION "costal-mi355x 0.1.0 (gfx950)"
 
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
 namespace {
 
-constexpr int PREC_F32 = COSTAL_PREC_FP32;
-constexpr int PREC_F16 = COSTAL_PREC_FP16;
-constexpr uint32_t NONE = 0xffffffffu;
 
 // ------------------------------------------------------------------------------------------------
 // Scalar f64 primitives (bit-exact with the reference)
@@ -86,21 +80,6 @@ __device__ __forceinline__ double tani_from_cos(double a, double b, double c) {
   return d / (a * a + b * b - d);
 }
 
-// f32 inner product of two zero-padded normalized rows, sequential fma: |err| <= gamma_d.
-__device__ __forceinline__ float dot32(const float* __restrict__ a, const float* __restrict__ b,
-                                       uint32_t dp) {
-  float s = 0.0f;
-  const f32x4* a4 = reinterpret_cast<const f32x4*>(a);
-  const f32x4* b4 = reinterpret_cast<const f32x4*>(b);
-  for (uint32_t t = 0; t < dp / 4; ++t) {
-    f32x4 u = a4[t], v = b4[t];
-    s = __builtin_fmaf(u[0], v[0], s);
-    s = __builtin_fmaf(u[1], v[1], s);
-    s = __builtin_fmaf(u[2], v[2], s);
-    s = __builtin_fmaf(u[3], v[3], s);
-  }
-  return s;
-}
 
 // (score desc, index asc): the order of sort_cands (knn.rs:589-591).  Scores are never NaN and
 // -0.0 never occurs (cos64 normalizes it), so '>' agrees with total_cmp.
@@ -232,419 +211,6 @@ __global__ __launch_bounds__(256) void prep_wide_kernel(const float* __restrict_
   }
 }
 
-// ------------------------------------------------------------------------------------------------
-// coarse_kernel
-//
-// Workgroup = NW waves; each wave owns 32 queries, held in registers for the whole launch.  The
-// workgroup streams its candidate range through two LDS buffers of TC-row tiles (32 KB each),
-// staged global -> registers -> LDS ahead of the compute.  For every 32-candidate
-// sub-tile a wave issues the MFMAs for D[cand][query] = sum_k cand[k] * query[k]: lane l owns query
-// (l % 32) and the 16 candidates 8*(i/4) + 4*(l/32) + (i%4), i = 0..15 (the 32x32 accumulator
-// layout).  The scan takes the max of a step's sub-tiles (max3 chains) and compares once with the
-// lane's threshold; only waves with a hit take the insert path.  MFMAs run one step ahead of the
-// scan (see scan_block).
-//
-// Candidate lists live in registers: each lane keeps its own sorted list of the best kp (score,
-// candidate) pairs among the candidates it sees (its half of every sub-tile), so the two lanes of a
-// query never wait on each other.  Both lists go to the rescoring, which treats them like two
-// candidate-range splits: a candidate outside both lists scored at most the larger of the two
-// floors.
-//
-// Row layout (global and LDS): a row is 2*J 16-byte chunks; chunk q feeds lane half h = q % 2,
-// slot j = q / 2.  Queries and candidates use the same chunk -> (lane half, slot) map, so each
-// MFMA contracts matching feature indices (any consistent permutation of k leaves the sum a sum of
-// the same products).  In LDS the chunks of one (sub-tile, slot, half) are stored for the 32 lanes
-// contiguously (1 KB per wave-wide load), so each ds_read_b128 is conflict free.
-
-
-__device__ __forceinline__ float max16(const f32x16& a) {
-  const float m0 = fmaxf(fmaxf(a[0], a[1]), a[2]);
-  const float m1 = fmaxf(fmaxf(a[3], a[4]), a[5]);
-  const float m2 = fmaxf(fmaxf(a[6], a[7]), a[8]);
-  const float m3 = fmaxf(fmaxf(a[9], a[10]), a[11]);
-  const float m4 = fmaxf(fmaxf(a[12], a[13]), a[14]);
-  return fmaxf(fmaxf(fmaxf(m0, m1), m2), fmaxf(fmaxf(m3, m4), a[15]));
-}
-
-// a[i] for a lane-varying i, as a select tree on the bits of i (keeps the accumulator in registers)
-__device__ __forceinline__ float pick16(const f32x16& a, int i) {
-  const bool b3 = i & 8, b2 = i & 4, b1 = i & 2, b0 = i & 1;
-  const float c0 = b3 ? a[8] : a[0], c1 = b3 ? a[9] : a[1], c2 = b3 ? a[10] : a[2], c3 = b3 ? a[11] : a[3];
-  const float c4 = b3 ? a[12] : a[4], c5 = b3 ? a[13] : a[5], c6 = b3 ? a[14] : a[6], c7 = b3 ? a[15] : a[7];
-  const float d0 = b2 ? c4 : c0, d1 = b2 ? c5 : c1, d2 = b2 ? c6 : c2, d3 = b2 ? c7 : c3;
-  const float e0 = b1 ? d2 : d0, e1 = b1 ? d3 : d1;
-  return b0 ? e1 : e0;
-}
-
-// The value held by the lane of the other half of the wave (lane l ^ 32): the other list of the
-// same query.
-__device__ __forceinline__ float partner32(float x) {
-  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
-  return __uint_as_float((__builtin_amdgcn_workitem_id_x() & 32) ? r[0] : r[1]);
-}
-
-// Sorted (ascending) register list: ls[0] is the smallest score.  Entries [kp, KPR) hold +inf and
-// never move; empty entries hold the admission threshold.  push() replaces the minimum and bubbles
-// the new entry up to its place.
-template <int KPR>
-__device__ __forceinline__ void list_push(float (&ls)[KPR], uint32_t (&li)[KPR], float v, uint32_t c) {
-  ls[0] = v;
-  li[0] = c;
-#pragma unroll
-  for (int e = 0; e < KPR - 1; ++e) {
-    const float a = ls[e], b = ls[e + 1];
-    if (!(a > b)) break;
-    const uint32_t ia = li[e], ib = li[e + 1];
-    ls[e] = b;
-    ls[e + 1] = a;
-    li[e] = ib;
-    li[e + 1] = ia;
-  }
-}
-
-// Precision forms of the coarse tile:
-//   PREC_F32   f32 rows, v_mfma_f32_32x32x2f32; a 16-byte chunk is 4 features (4 MFMAs).
-//   PREC_F16   FP16 rows, v_mfma_f32_32x32x16_f16; a chunk is 8 features (1 MFMA); a hit is re-scored
-//              in f32 before admission.
-//   PREC_F16X3 FP16 hi/lo split rows (x = hi + lo + O(2^-22)): chunks [0, J/2) hold hi, [J/2, J) lo;
-//              hi.hi + hi.lo + lo.hi per chunk pair (3 MFMAs), an f32-class score without re-scoring.
-constexpr int PREC_F16X3 = COSTAL_PREC_FP16X3;
-
-// The 32x32 product of one sub-tile: b = the lane's J candidate chunks, q = its J query chunks.
-template <int PREC, int J>
-__device__ __forceinline__ f32x16 tile_mfma(const uint4 (&b)[J], const uint4 (&q)[J], f32x16 acc = {}) {
-  if constexpr (PREC == PREC_F32) {
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const f32x4 b4 = __builtin_bit_cast(f32x4, b[j]);
-      const f32x4 q4 = __builtin_bit_cast(f32x4, q[j]);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[0], q4[0], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[1], q4[1], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[2], q4[2], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(b4[3], q4[3], acc, 0, 0, 0);
-    }
-  } else if constexpr (PREC == PREC_F16) {
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, b[j]), __builtin_bit_cast(f16x8, q[j]),
-                                                   acc, 0, 0, 0);
-  } else {
-    constexpr int JH = J / 2;
-#pragma unroll
-    for (int j = 0; j < JH; ++j) {
-      const f16x8 ch = __builtin_bit_cast(f16x8, b[j]), cl = __builtin_bit_cast(f16x8, b[j + JH]);
-      const f16x8 qh = __builtin_bit_cast(f16x8, q[j]), ql = __builtin_bit_cast(f16x8, q[j + JH]);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(cl, qh, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ch, ql, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ch, qh, acc, 0, 0, 0);
-    }
-  }
-  return acc;
-}
-
-template <int J>
-struct TileShape {
-  static constexpr int TC = J <= 2 ? 512 : J <= 4 ? 256 : 128;  // candidates per tile: 32 KB tiles
-  static constexpr int TILEB = TC * 32 * J;
-  static constexpr int SUBT = TC / 32;                         // 32-candidate sub-tiles per tile
-  static constexpr int GS = (SUBT >= 4 && J <= 4) ? 2 : 1;     // sub-tiles per scan step
-  static constexpr int NSTEP = SUBT / GS;                      // scan steps per tile
-};
-
-// One work item of the scan: query block `qblock` (QB queries) against candidates [c_begin, c_end)
-// of one sample, whose lists go to split `split` of the queries' nsplit list pairs.  tile0/tile1
-// are the workgroup's two LDS tile buffers.
-//
-// Software pipeline over scan steps (GS sub-tiles each, NSTEP per tile, G counting steps over the
-// whole candidate range): step G issues the MFMAs of step G+1, reads the candidate chunks of step
-// G+3 from LDS, and then scans the accumulators of step G, so the matrix core works on the next
-// step while the wave compares and inserts, and the LDS reads complete behind two steps of work.
-// Accumulators and chunk registers are double buffered by the parity of G (NSTEP is even, so the
-// parity is the step's parity within its tile).  The next tile is stored to the other LDS buffer,
-// and the workgroup meets at the barrier, at step NSTEP-3 of each tile, before the first read of
-// that tile.
-template <int PREC, int J, int KPR, int NW>
-__device__ __forceinline__ void scan_block(
-    uint8_t* __restrict__ tile0, uint8_t* __restrict__ tile1,
-    const uint8_t* __restrict__ xsq,   // n_pad rows of 32*J bytes (the working copy of this form)
-    const float* __restrict__ xs32,    // f32 working copy (dp32 per row), for the FP16 re-score
-    uint32_t dp32, uint32_t n, int kp,
-    const uint32_t* __restrict__ rowlist, uint32_t nq,  // queries: rowlist[0..nq) or rows 0..nq-1
-    uint32_t qblock, const float* __restrict__ tinit,   // per-row admission threshold (by row) or null
-    uint32_t split, uint32_t nsplit,
-    uint32_t c_begin, uint32_t c_end,  // candidate range; c_begin a multiple of the tile, c_end <= n
-    float eps_pre,                     // FP16 form: bound on |FP16 score - f32 score|
-    uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc, uint32_t* __restrict__ cand_cnt,
-    float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
-  constexpr int QB = 32 * NW;
-  constexpr int TC = TileShape<J>::TC;
-  constexpr int ROWB = 32 * J;
-  constexpr int NT = 64 * NW;
-  constexpr int CHUNKS = TC * 2 * J;        // 16-byte chunks per tile
-  constexpr int CPT = (CHUNKS + NT - 1) / NT;
-  constexpr int GS = TileShape<J>::GS;
-  constexpr int NSTEP = TileShape<J>::NSTEP;
-  static_assert(NSTEP >= 2 && NSTEP % 2 == 0, "pipeline steps per tile");
-  static_assert(GS * NSTEP * 32 == TC, "tile shape");
-
-  const int tid = tid_x();
-  const int w = tid >> 6, l = tid & 63, h = l >> 5, l32 = l & 31;
-  const uint32_t qq = qblock * QB + w * 32 + l32;
-  const bool qvalid = qq < nq;
-  const uint32_t qg = qvalid ? (rowlist ? rowlist[qq] : qq) : 0;
-
-  // the wave's queries, in registers for the whole launch
-  uint4 qreg[J];
-  {
-    const uint8_t* qrow = xsq + (size_t)qg * ROWB;
-#pragma unroll
-    for (int j = 0; j < J; ++j) qreg[j] = *reinterpret_cast<const uint4*>(qrow + (2 * j + h) * 16);
-  }
-  // Empty list entries hold the lane's admission threshold (-inf without one; +inf for padding
-  // queries, which so never take the insert path): the list minimum ls[0] is then always the
-  // lane's threshold, and real entries (score above it) are the ones with an index.
-  const float t0 = qvalid ? (tinit ? tinit[qg] : -INFINITY) : INFINITY;
-  float ls[KPR];
-  uint32_t li[KPR];
-#pragma unroll
-  for (int e = 0; e < KPR; ++e) {
-    ls[e] = e < kp ? t0 : INFINITY;
-    li[e] = NONE;
-  }
-  float th = ls[0];
-  float thp = -INFINITY;  // the partner list's threshold, refreshed once per tile
-  uint32_t my_inserts = 0;
-
-  const int ntile = c_begin < c_end ? (int)((c_end - c_begin + TC - 1) / TC) : 0;
-
-  // Tiles go global -> registers -> LDS: the registers are loaded with tile t+2 while tile t is
-  // scanned (right after the workgroup has stored tile t+1 and met) and stored during tile t+1, so
-  // a load has a whole tile of steps to arrive.  One register set, and a loop body of one tile
-  // whose only exit is at its end, swapping the two LDS buffers: the compiler's wait counts then
-  // stay exact at the loop head (a second staging set, or an exit from the middle of a body
-  // unrolled by two, made it wait there for loads still in flight).
-  static_assert(CPT <= 8, "tile staging uses at most eight 16-byte registers per thread");
-  uint4 ra0 = {}, ra1 = {}, ra2 = {}, ra3 = {}, ra4 = {}, ra5 = {}, ra6 = {}, ra7 = {};
-  // Thread e fills LDS slot e (consecutive lanes, consecutive 16-byte slots: conflict free); slot
-  // e = ((sub*J + j)*2 + half)*32 + lane32 holds chunk 2j + half of tile row sub*32 + lane32.
-#define LOAD_ONE(R, C)                                                                            \
-  if constexpr (C < CPT) {                                                                        \
-    const int e_ = tid + C * NT;                                                                  \
-    if (CHUNKS % NT == 0 || e_ < CHUNKS) {                                                        \
-      const int lr_ = e_ & 31, hh_ = (e_ >> 5) & 1, j_ = (e_ >> 6) % J, sub_ = (e_ >> 6) / J;     \
-      const uint32_t off_ = (uint32_t)((sub_ * 32 + lr_) * ROWB + (2 * j_ + hh_) * 16);            \
-      R##C = *reinterpret_cast<const uint4*>(src_ + off_);                                       \
-    }                                                                                             \
-  }
-#define LOAD_REGS(R, T)                                                                           \
-  {                                                                                               \
-    const uint8_t* src_ = xsq + (size_t)(c_begin + (uint32_t)(T) * TC) * ROWB;                     \
-    LOAD_ONE(R, 0) LOAD_ONE(R, 1) LOAD_ONE(R, 2) LOAD_ONE(R, 3)                                   \
-    LOAD_ONE(R, 4) LOAD_ONE(R, 5) LOAD_ONE(R, 6) LOAD_ONE(R, 7)                                   \
-  }
-#define STORE_ONE(R, C, BUF)                                                                      \
-  if constexpr (C < CPT) {                                                                        \
-    const int e_ = tid + C * NT;                                                                  \
-    if (CHUNKS % NT == 0 || e_ < CHUNKS) *reinterpret_cast<uint4*>(&BUF[e_ * 16]) = R##C;         \
-  }
-#define STORE_REGS(R, BUF) \
-  { STORE_ONE(R, 0, BUF) STORE_ONE(R, 1, BUF) STORE_ONE(R, 2, BUF) STORE_ONE(R, 3, BUF)         \
-    STORE_ONE(R, 4, BUF) STORE_ONE(R, 5, BUF) STORE_ONE(R, 6, BUF) STORE_ONE(R, 7, BUF) }
-  // LDS writes of this wave done, then everyone meets; no wait on the global loads in flight
-#define TILE_BARRIER()                                                                            \
-  {                                                                                               \
-    __builtin_amdgcn_s_waitcnt(0xc07f); /* lgkmcnt(0) */                                          \
-    __builtin_amdgcn_s_barrier();                                                                 \
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");                                        \
-  }
-  // the lane's chunks of scan step S of the tile in BUF, into chunk buffer BS
-#define READ_B(BS, BUF, S)                                                                        \
-  _Pragma("unroll") for (int u_ = 0; u_ < GS; ++u_)                                               \
-    _Pragma("unroll") for (int j_ = 0; j_ < J; ++j_)                                              \
-      bop[BS][u_][j_] = *reinterpret_cast<const uint4*>(                                          \
-          (BUF) + (((((S) * GS + u_) * J + j_) * 2 + h) * 32 + l32) * 16);
-#define MFMA_STEP(SLOT) \
-  _Pragma("unroll") for (int u_ = 0; u_ < GS; ++u_) acc[SLOT][u_] = tile_mfma<PREC, J>(bop[SLOT][u_], qreg);
-
-  const float* xq32 = xs32 + (size_t)qg * dp32;
-  f32x16 acc[2][GS];
-  uint4 bop[2][GS][J];
-  constexpr int RD = 3;  // chunks of step G+RD are read at step G
-  static_assert(NSTEP >= RD, "steps per tile");
-  int t = 0;
-  // compare the accumulators of step `p` of tile t with the lane's threshold; rare path: each lane
-  // walks its own hits, sub-tile by sub-tile
-  auto scan_step = [&](const f32x16 (&a)[GS], int p) __attribute__((always_inline)) {
-    float mu[GS];
-#pragma unroll
-    for (int u = 0; u < GS; ++u) mu[u] = max16(a[u]);
-    float m = mu[0];
-#pragma unroll
-    for (int u = 1; u < GS; ++u) m = fmaxf(m, mu[u]);
-    float the = (PREC == PREC_F16) ? fmaxf(th, thp) - eps_pre : fmaxf(th, thp);
-    if (__builtin_amdgcn_ballot_w64(m > the) != 0) {
-#pragma unroll
-      for (int u = 0; u < GS; ++u) {
-        if (mu[u] > the) {
-          // bit i = accumulator i above the threshold (built from the top, no wide constants)
-          uint32_t mask = 0;
-#pragma unroll
-          for (int i = 15; i >= 0; --i) mask = (mask << 1) | (a[u][i] > the ? 1u : 0u);
-          // the lane's offset is added here, not folded into per-sub-tile bases hoisted out of the
-          // tile loop (each would hold a register for the whole launch)
-          uint32_t lane_off = 4 * h;
-          __asm__ volatile("" : "+v"(lane_off));
-          const uint32_t cbase = c_begin + (uint32_t)t * TC + (uint32_t)(p * GS + u) * 32 + lane_off;
-          while (mask) {
-            const int i = __builtin_ctz(mask);
-            mask &= mask - 1;
-            float v = pick16(a[u], i);
-            const uint32_t cand = cbase + 8 * (i >> 2) + (i & 3);
-            if (v > the && cand < c_end && cand != qg) {
-              if constexpr (PREC == PREC_F16) {
-                v = dot32(xq32, xs32 + (size_t)cand * dp32, dp32);
-                if (!(v > th)) continue;
-              }
-              list_push<KPR>(ls, li, v, cand);
-              th = ls[0];
-              the = (PREC == PREC_F16) ? fmaxf(th, thp) - eps_pre : fmaxf(th, thp);
-              ++my_inserts;
-            }
-          }
-        }
-      }
-    }
-  };
-  // all NSTEP steps of tile t, held in CUR; the next tile goes from the staging registers to OTH
-#define RUN_TILE(CUR, OTH)                                                                        \
-  _Pragma("unroll") for (int p = 0; p < NSTEP; ++p) {                                             \
-    MFMA_STEP((p + 1) & 1)                                                                        \
-    if (p == 0) thp = partner32(th);                                                              \
-    if (p == NSTEP - RD) {                                                                        \
-      STORE_REGS(ra, OTH);                                                                        \
-      TILE_BARRIER();                                                                             \
-      LOAD_REGS(ra, t + 2 < ntile ? t + 2 : ntile - 1);                                           \
-    }                                                                                             \
-    if (p + RD < NSTEP) {                                                                         \
-      READ_B((p + RD) & 1, CUR, p + RD)                                                           \
-    } else {                                                                                      \
-      READ_B((p + RD) & 1, OTH, p + RD - NSTEP)                                                   \
-    }                                                                                             \
-    scan_step(acc[p & 1], p);                                                                     \
-  }
-
-  if (ntile > 0) {
-    uint8_t* cur = tile0;
-    uint8_t* oth = tile1;
-    LOAD_REGS(ra, 0);
-    STORE_REGS(ra, cur);
-    TILE_BARRIER();
-    LOAD_REGS(ra, ntile > 1 ? 1 : 0);
-    READ_B(0, cur, 0)
-    READ_B(1, cur, 1)
-    MFMA_STEP(0)
-    READ_B(0, cur, 2)
-    for (;;) {
-      // tile t in cur, ra = tile t+1 in flight
-      RUN_TILE(cur, oth)
-      if (++t >= ntile) break;
-      uint8_t* const tmp = cur;
-      cur = oth;
-      oth = tmp;
-    }
-  }
-#undef RUN_TILE
-#undef MFMA_STEP
-#undef READ_B
-#undef TILE_BARRIER
-#undef LOAD_REGS
-#undef STORE_REGS
-#undef LOAD_ONE
-#undef STORE_ONE
-
-  if (inserts) {
-    // one atomic per wave
-    uint32_t tot = my_inserts;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
-    if (l == 0) atomicAdd(inserts, (unsigned long long)tot);
-  }
-  const float thp_end = partner32(th);
-  if (qvalid) {
-    const size_t slot = ((size_t)qq * nsplit + split) * 2 + h;
-    const size_t base = slot * (size_t)kp;
-    uint32_t c = 0;
-#pragma unroll
-    for (int e = 0; e < KPR; ++e) {
-      if (e < kp && li[e] != NONE) {
-        cand_idx[base + c] = li[e];
-        cand_sc[base + c] = ls[e];
-        ++c;
-      }
-    }
-    cand_cnt[slot] = c;
-    cand_floor[slot] = fmaxf(th, thp_end);  // everything this list turned away scored at most this
-  }
-}
-
-// Two launch shapes:
-//   units_per_wg == 0: grid (query blocks, splits); workgroup (x, y) scans query block x against
-//     columns [y * cols_per_split, (y+1) * cols_per_split).
-//   units_per_wg > 0: one workgroup per resident slot, each taking the same number of (query block,
-//     tile) units in query-block-major order, so every slot ends together.  A query block whose
-//     units straddle two workgroups is scanned in two pieces: the one from tile 0 fills list pair 0,
-//     the other list pair 1 (nsplit = 2); a block scanned whole marks pair 1 empty.
-template <int PREC, int J, int KPR, int NW>
-__global__ __launch_bounds__(64 * NW) void coarse_kernel(
-    const uint8_t* __restrict__ xsq, const float* __restrict__ xs32, uint32_t dp32, uint32_t n, int kp,
-    const uint32_t* __restrict__ rowlist, uint32_t nq, const float* __restrict__ tinit, uint32_t cols_per_split,
-    uint32_t units_per_wg, float eps_pre, uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc,
-    uint32_t* __restrict__ cand_cnt, float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
-  __shared__ __attribute__((aligned(16))) uint8_t tile0[TileShape<J>::TILEB];
-  __shared__ __attribute__((aligned(16))) uint8_t tile1[TileShape<J>::TILEB];
-  constexpr uint32_t TC = TileShape<J>::TC;
-  constexpr uint32_t QB = 32 * NW;
-  const uint32_t ntile = (n + TC - 1) / TC;
-  const uint64_t units = units_per_wg ? (uint64_t)((nq + QB - 1) / QB) * ntile : 1;
-  uint64_t u = units_per_wg ? (uint64_t)bid_x() * units_per_wg : 0;
-  const uint64_t u_end = units_per_wg ? (u + units_per_wg < units ? u + units_per_wg : units) : 1;
-  // one piece per iteration (the direct shape has exactly one); a single call site keeps one copy
-  // of the scan in the kernel
-  for (bool first = true; u < u_end; first = false) {
-    uint32_t qb, split, nspl, c0, c1, len;
-    bool whole = false;
-    if (units_per_wg) {
-      qb = (uint32_t)(u / ntile);
-      const uint32_t tb = (uint32_t)(u % ntile);
-      const uint32_t te = (uint32_t)(tb + (u_end - u) < ntile ? tb + (u_end - u) : ntile);
-      split = tb == 0 ? 0 : 1;
-      nspl = 2;
-      c0 = tb * TC;
-      c1 = te * TC < n ? te * TC : n;
-      len = te - tb;
-      whole = tb == 0 && te == ntile;
-    } else {
-      qb = bid_x();
-      split = bid_y();
-      nspl = gdim_y();
-      c0 = split * cols_per_split;
-      c1 = c0 + cols_per_split < n ? c0 + cols_per_split : n;
-      len = 1;
-    }
-    if (!first) __syncthreads();  // every wave is done with the previous piece's tiles
-    scan_block<PREC, J, KPR, NW>(tile0, tile1, xsq, xs32, dp32, n, kp, rowlist, nq, qb, tinit, split, nspl, c0, c1,
-                                 eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor, inserts);
-    if (whole) {
-      const uint32_t qq = qb * QB + (tid_x() >> 6) * 32 + (tid_x() & 31);
-      if (qq < nq) {
-        const size_t slot = ((size_t)qq * 2 + 1) * 2 + ((tid_x() >> 5) & 1);
-        cand_cnt[slot] = 0;
-        cand_floor[slot] = -INFINITY;
-      }
-    }
-    u += len;
-  }
-}
 
 // ------------------------------------------------------------------------------------------------
 // Wide rows (more features than the fused scan keeps in registers): two kernels per block of query
@@ -1174,20 +740,6 @@ __global__ __launch_bounds__(256) void rowscan_kernel(
 // batch's widest feature count, zeros being exact), and one persistent scan kernel pulls
 // (sample, query block, candidate split) work items from a queue ordered by decreasing cost.
 
-struct BatchSample {
-  const float* x;        // the sample's rows (device), n x d
-  uint32_t n, d;
-  uint32_t row_pad;      // first row of the sample in the stacked working copies
-  uint32_t row_base;     // first row of the sample in the concatenated row numbering
-  int kp;                // candidate-list length K'
-  uint32_t nsplit, cols_per_split;
-  uint64_t slot_base;    // first candidate list (slot) of the sample
-  uint64_t entry_base;   // first list entry of the sample (its lists hold kp entries each)
-  uint32_t* idx;
-  double* cos;
-  double* norms;
-  double* wts;
-};
 
 // the sample a row belongs to, by binary search in an ascending table of first rows
 __device__ __forceinline__ uint32_t find_sample(const uint32_t* __restrict__ first, uint32_t nsamples, uint32_t g) {
@@ -1281,34 +833,6 @@ __global__ void prep_batch_kernel(const BatchSample* __restrict__ samples, const
   }
 }
 
-// items[3e..3e+2] = (sample, query block, split), in decreasing cost order
-template <int PREC, int J, int KPR, int NW>
-__global__ __launch_bounds__(64 * NW) void coarse_batch_kernel(
-    const BatchSample* __restrict__ samples, const uint32_t* __restrict__ items, uint32_t nitems,
-    uint32_t* __restrict__ queue, const uint8_t* __restrict__ xsq, const float* __restrict__ xs32, uint32_t dp32,
-    float eps_pre, uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc, uint32_t* __restrict__ cand_cnt,
-    float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
-  __shared__ __attribute__((aligned(16))) uint8_t tile0[TileShape<J>::TILEB];
-  __shared__ __attribute__((aligned(16))) uint8_t tile1[TileShape<J>::TILEB];
-  __shared__ uint32_t s_item;
-  for (;;) {
-    if (tid_x() == 0) s_item = atomicAdd(queue, 1u);
-    __syncthreads();
-    const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
-    __syncthreads();
-    if (it >= nitems) break;
-    const uint32_t sidx = items[3 * it], qb = items[3 * it + 1], sp = items[3 * it + 2];
-    const BatchSample& S = samples[sidx];
-    const size_t rowb = (size_t)S.row_pad;
-    const uint32_t c0 = sp * S.cols_per_split;
-    const uint32_t c1 = c0 + S.cols_per_split < S.n ? c0 + S.cols_per_split : S.n;
-    scan_block<PREC, J, KPR, NW>(tile0, tile1, xsq + rowb * 32 * J, xs32 + rowb * dp32, dp32, S.n, S.kp, nullptr,
-                                 S.n, qb, nullptr, sp, S.nsplit, c0, c1, eps_pre,
-                                 cand_idx + S.entry_base, cand_sc + S.entry_base, cand_cnt + S.slot_base,
-                                 cand_floor + S.slot_base, inserts);
-    __syncthreads();
-  }
-}
 
 template <int CMAX>
 __global__ __launch_bounds__(64) void rescore_batch_kernel(
@@ -1525,42 +1049,13 @@ struct Plan {
   uint32_t dp32, dp16, n_pad;
 };
 
-typedef void (*CoarseFn)(const uint8_t*, const float*, uint32_t, uint32_t, int, const uint32_t*,
-                         uint32_t, const float*, uint32_t, uint32_t, float, uint32_t*, float*, uint32_t*,
-                         float*, unsigned long long*);
-
-template <int PREC, int J, int KP, int NW>
-CoarseFn coarse_ptr() {
-  if constexpr (PREC == PREC_F16X3 && J % 2) return nullptr;  // split rows hold hi and lo halves
-  else return coarse_kernel<PREC, J, KP, NW>;
-}
-
-template <int PREC, int KP, int NW>
-CoarseFn coarse_by_j(uint32_t J) {
-  switch (J) {
-    case 1: return coarse_ptr<PREC, 1, KP, NW>();
-    case 2: return coarse_ptr<PREC, 2, KP, NW>();
-    case 3: return coarse_ptr<PREC, 3, KP, NW>();
-    case 4: return coarse_ptr<PREC, 4, KP, NW>();
-    case 5: return coarse_ptr<PREC, 5, KP, NW>();
-    case 6: return coarse_ptr<PREC, 6, KP, NW>();
-    case 7: return coarse_ptr<PREC, 7, KP, NW>();
-    case 8: return coarse_ptr<PREC, 8, KP, NW>();
-  }
-  return nullptr;
-}
-
-// Register-list capacity 32 or 64 (K' <= capacity), 4 waves per workgroup.
-template <int PREC>
-CoarseFn coarse_by_kp(uint32_t J, int KP) {
-  return KP == 16 ? coarse_by_j<PREC, 16, 4>(J) : KP == 32 ? coarse_by_j<PREC, 32, 4>(J) : coarse_by_j<PREC, 64, 4>(J);
-}
+using costal::CoarseFn;
 
 CoarseFn coarse_select(int prec, uint32_t J, int KP) {
-  if (prec == PREC_F32) return coarse_by_kp<PREC_F32>(J, KP);
-  if (prec == PREC_F16) return coarse_by_kp<PREC_F16>(J, KP);
-  if (J % 2) return nullptr;
-  return coarse_by_kp<PREC_F16X3>(J, KP);
+  using namespace costal;
+  if (prec == PREC_F32) return KP == 16 ? scan_fn_p0_k16(J) : KP == 32 ? scan_fn_p0_k32(J) : scan_fn_p0_k64(J);
+  if (prec == PREC_F16) return KP == 16 ? scan_fn_p1_k16(J) : KP == 32 ? scan_fn_p1_k32(J) : scan_fn_p1_k64(J);
+  return KP == 16 ? scan_fn_p2_k16(J) : KP == 32 ? scan_fn_p2_k32(J) : scan_fn_p2_k64(J);
 }
 
 // FP16 candidate error bound.  x16 = fp16(fp32(x/|x|)): |x16 - x| <= U|x| + eta per component with
@@ -1988,42 +1483,15 @@ hipStream_t pick_stream(Device& D, void* stream) { return stream ? (hipStream_t)
 // ------------------------------------------------------------------------------------------------
 // Batched samples (host side)
 
-typedef void (*CoarseBatchFn)(const BatchSample*, const uint32_t*, uint32_t, uint32_t*, const uint8_t*,
-                              const float*, uint32_t, float, uint32_t*, float*, uint32_t*, float*,
-                              unsigned long long*);
-
-template <int PREC, int J, int KP>
-CoarseBatchFn coarse_batch_ptr() {
-  if constexpr (PREC == PREC_F16X3 && J % 2) return nullptr;  // split rows hold hi and lo halves
-  else return coarse_batch_kernel<PREC, J, KP, 4>;
-}
-
-template <int PREC, int KP>
-CoarseBatchFn coarse_batch_by_j(uint32_t J) {
-  switch (J) {
-    case 1: return coarse_batch_ptr<PREC, 1, KP>();
-    case 2: return coarse_batch_ptr<PREC, 2, KP>();
-    case 3: return coarse_batch_ptr<PREC, 3, KP>();
-    case 4: return coarse_batch_ptr<PREC, 4, KP>();
-    case 5: return coarse_batch_ptr<PREC, 5, KP>();
-    case 6: return coarse_batch_ptr<PREC, 6, KP>();
-    case 7: return coarse_batch_ptr<PREC, 7, KP>();
-    case 8: return coarse_batch_ptr<PREC, 8, KP>();
-  }
-  return nullptr;
-}
-
-template <int PREC>
-CoarseBatchFn coarse_batch_by_kp(uint32_t J, int KP) {
-  return KP == 16 ? coarse_batch_by_j<PREC, 16>(J) : KP == 32 ? coarse_batch_by_j<PREC, 32>(J)
-                                                             : coarse_batch_by_j<PREC, 64>(J);
-}
+using costal::CoarseBatchFn;
 
 CoarseBatchFn coarse_batch_select(int prec, uint32_t J, int KP) {
-  if (prec == PREC_F32) return coarse_batch_by_kp<PREC_F32>(J, KP);
-  if (prec == PREC_F16) return coarse_batch_by_kp<PREC_F16>(J, KP);
-  if (J % 2) return nullptr;
-  return coarse_batch_by_kp<PREC_F16X3>(J, KP);
+  using namespace costal;
+  if (prec == PREC_F32)
+    return KP == 16 ? scan_batch_fn_p0_k16(J) : KP == 32 ? scan_batch_fn_p0_k32(J) : scan_batch_fn_p0_k64(J);
+  if (prec == PREC_F16)
+    return KP == 16 ? scan_batch_fn_p1_k16(J) : KP == 32 ? scan_batch_fn_p1_k32(J) : scan_batch_fn_p1_k64(J);
+  return KP == 16 ? scan_batch_fn_p2_k16(J) : KP == 32 ? scan_batch_fn_p2_k32(J) : scan_batch_fn_p2_k64(J);
 }
 
 struct BatchGraph {

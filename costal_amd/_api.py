@@ -81,36 +81,48 @@ def _as_matrix(X) -> np.ndarray:
     return np.ascontiguousarray(X, dtype=np.float32)
 
 
-_CHUNK_BYTES = 1 << 23  # stream statistics over ~8 MB of float64 at a time
+_BLOCK_VALUES = 1 << 20  # entries of X converted to float64 at a time by _diagnose
 
 
 def _diagnose(X: np.ndarray, norms: np.ndarray) -> dict[str, Any]:
-    """Record-only input statistics, streamed over row chunks (same keys and tests as the
-    reference: unit-normalized rows, centered columns, sign and sparsity fractions)."""
+    """Record-only input statistics; the data is never modified.
+
+    ``row_norm_cv``: standard deviation over mean of the exact f64 row norms (0 for a zero mean).
+    ``columns_centered``: some column varies, and every varying column has |mean| <= 0.05 std.
+    ``negative_fraction`` / ``nonzero_fraction``: shares of negative and of non-zero entries.
+
+    The column moments are merged block by block with the pairwise update of Chan, Golub and
+    LeVeque (mean and sum of squared deviations per block, then combined), so the extra memory
+    is one block of rows whatever the size of X.  The semantics (keys, thresholds) are those of
+    the reference's diagnostics, ike12th/CosTaL python/costal/_api.py:112-144."""
     n, d = X.shape
-    step = max(1, _CHUNK_BYTES // (8 * max(d, 1)))
-    spans = [(a, min(a + step, n)) for a in range(0, n, step)]
-    col_sum = np.zeros(d)
-    neg = nz = 0
-    for a, b in spans:
-        c = X[a:b]
-        col_sum += c.sum(axis=0, dtype=np.float64)
-        neg += int(np.count_nonzero(c < 0))
-        nz += int(np.count_nonzero(c))
-    mean = col_sum / max(n, 1)
-    ss = np.zeros(d)
-    for a, b in spans:
-        c = X[a:b].astype(np.float64) - mean
-        ss += np.einsum("ij,ij->j", c, c)
-    std = np.sqrt(ss / max(n, 1))
-    mean_norm = norms.mean() if n else 0.0
-    live = std > 0
-    size = max(X.size, 1)
+    block = max(1, _BLOCK_VALUES // max(d, 1))
+    seen = 0
+    mean = np.zeros(d)
+    m2 = np.zeros(d)
+    negative = nonzero = 0
+    for a in range(0, n, block):
+        rows = X[a:a + block]
+        negative += int(np.count_nonzero(rows < 0))
+        nonzero += int(np.count_nonzero(rows))
+        b = rows.astype(np.float64)
+        m = len(b)
+        b_mean = b.mean(axis=0)
+        b_m2 = np.square(b - b_mean).sum(axis=0)
+        delta = b_mean - mean
+        total = seen + m
+        mean = mean + delta * (m / total)
+        m2 = m2 + b_m2 + np.square(delta) * (seen * m / total)
+        seen = total
+    std = np.sqrt(m2 / n) if n else np.zeros(d)
+    varying = std > 0
+    mu = norms.mean() if n else 0.0
+    entries = max(n * d, 1)
     return {
-        "row_norm_cv": float(norms.std() / mean_norm) if mean_norm > 0 else 0.0,
-        "columns_centered": bool(live.any() and np.all(np.abs(mean[live]) <= 0.05 * std[live])),
-        "negative_fraction": neg / size,
-        "nonzero_fraction": nz / size,
+        "row_norm_cv": float(norms.std() / mu) if mu > 0 else 0.0,
+        "columns_centered": bool(varying.any() and np.all(np.abs(mean[varying]) <= 0.05 * std[varying])),
+        "negative_fraction": negative / entries,
+        "nonzero_fraction": nonzero / entries,
     }
 
 
@@ -119,7 +131,8 @@ def knn(X, k: int = 10, *, select: str = "cosine", normalize: str | None = None,
         matmul: str | None = None) -> KnnGraph:
     """Exact kNN of the rows of ``X`` (k excludes the row itself), by cosine or, with
     ``select="tanimoto"``, by Tanimoto on the raw rows. ``matmul`` picks the coarse-tile precision
-    (``"mfma-f16"`` default, ``"mfma-f32"``); the result is identical either way."""
+    (``"mfma-f16x3"`` split FP16, the default; ``"mfma-f16"``; ``"mfma-f32"``); the result is
+    identical either way."""
     normalize = _check_choices(select, normalize, backend, compat)
     effective = select
     if select == "tanimoto" and normalize == "l2":

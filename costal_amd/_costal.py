@@ -78,10 +78,14 @@ def _parse_backend(matmul) -> tuple[str, str]:
     return name, _BACKENDS[name]
 
 
-def _call(fn, *args):
+def _call(fn, *args, stats=None):
     rc = fn(*args)
     if rc != _lib.OK:
-        raise _lib.CostalError(rc, _lib.load().costal_last_error().decode())
+        row = sample = None
+        if rc == _lib.E_NON_FINITE and stats is not None and stats.nonfinite_row != _lib.NO_ROW:
+            row = int(stats.nonfinite_row)
+            sample = None if stats.nonfinite_sample == _lib.NO_ROW else int(stats.nonfinite_sample)
+        raise _lib.CostalError(rc, _lib.load().costal_last_error().decode(), row, sample)
 
 
 def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None, matmul=None,
@@ -135,7 +139,7 @@ def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None
     st = _lib.KnnStats()
     L = _lib.load()
     _call(L.costal_knn_exact, _ptr(x), n, d, C.byref(p), _ptr(idx), _ptr(cos), _ptr(norms),
-          _ptr(w) if w is not None else None, C.byref(st))
+          _ptr(w) if w is not None else None, C.byref(st), stats=st)
     stats = {
         "fallback_rows": int(st.fallback_rows),
         "row_block": int(st.row_block),
@@ -154,6 +158,8 @@ def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None
         "inserts": int(st.inserts),
         "rowscan_rows": int(st.rowscan_rows),
         "device_ms": float(st.device_ms),
+        "coarse_err_max": float(st.coarse_err_max),
+        "coarse_eps": float(st.coarse_eps),
         "kernel_ms": {"prep": st.prep_ms, "coarse": st.coarse_ms, "rescore": st.rescore_ms,
                       "fallback": st.fallback_ms},
     }

@@ -38,6 +38,8 @@ EXPORTS = (
     "costal_mlp_adam_device",
     "costal_mlp_last_error",
     "costal_knn_batched_device",
+    "costal_knn_plan",
+    "costal_mfma_probe",
 )
 
 
@@ -76,6 +78,10 @@ class KnnStats(C.Structure):
         ("coarse_ms", C.c_double),
         ("rescore_ms", C.c_double),
         ("fallback_ms", C.c_double),
+        ("nonfinite_row", C.c_uint64),
+        ("nonfinite_sample", C.c_uint64),
+        ("coarse_err_max", C.c_double),
+        ("coarse_eps", C.c_double),
     ]
 
 
@@ -94,12 +100,18 @@ class Sample(C.Structure):
     ]
 
 
-class CostalError(ValueError):
-    """An error returned by the library; ``code`` is one of the E_* values."""
+NO_ROW = (1 << 64) - 1
 
-    def __init__(self, code: int, msg: str):
+
+class CostalError(ValueError):
+    """An error returned by the library; ``code`` is one of the E_* values.  For E_NON_FINITE,
+    ``row`` (and for a batch ``sample``) name the first row holding a NaN or inf."""
+
+    def __init__(self, code: int, msg: str, row: int | None = None, sample: int | None = None):
         super().__init__(msg)
         self.code = code
+        self.row = row
+        self.sample = sample
 
 
 _lib = None
@@ -124,6 +136,8 @@ def load():
     L.costal_knn_exact_device.argtypes = [P, U64, U64, C.POINTER(KnnParams), P, P, P, P,
                                           C.POINTER(KnnStats), P]
     L.costal_knn_bruteforce.argtypes = [P, U64, U64, U64, U32, P, P, P]
+    L.costal_mfma_probe.argtypes = [P]
+    L.costal_knn_plan.argtypes = [U64, U64, C.POINTER(KnnParams), C.POINTER(KnnStats)]
     L.costal_knn_batched_device.argtypes = [C.POINTER(Sample), U32, C.POINTER(KnnParams), C.POINTER(KnnStats), P]
     L.costal_reweight.argtypes = [U64, U64, P, P, P, U32, P]
     L.costal_reweight_device.argtypes = [U64, U64, P, P, P, U32, P, P]

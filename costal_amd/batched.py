@@ -58,8 +58,13 @@ class Batch:
         p.margin = -1
         p.precision = _lib.PRECISION[_PREC[matmul]]
         st = _lib.KnnStats()
-        _lib.check(_lib.load().costal_knn_batched_device(self.samples, len(self.xs), C.byref(p), C.byref(st),
-                                                         C.c_void_p(self.stream)))
+        rc = _lib.load().costal_knn_batched_device(self.samples, len(self.xs), C.byref(p), C.byref(st),
+                                                   C.c_void_p(self.stream))
+        if rc != _lib.OK:
+            row = sample = None
+            if rc == _lib.E_NON_FINITE and st.nonfinite_row != _lib.NO_ROW:
+                row, sample = int(st.nonfinite_row), int(st.nonfinite_sample)
+            raise _lib.CostalError(rc, _lib.load().costal_last_error().decode(), row, sample)
         return st
 
     def outputs(self):
@@ -77,4 +82,5 @@ def knn_batched(xs: list[np.ndarray], k: int, *, matmul: str = "mfma-f16x3", mar
     st = b.run(matmul)
     return b.outputs(), {"fallback_rows": int(st.fallback_rows), "inserts": int(st.inserts),
                          "col_splits": int(st.col_splits), "candidates": int(st.candidates),
-                         "device_ms": float(st.device_ms)}
+                         "device_ms": float(st.device_ms), "coarse_err_max": float(st.coarse_err_max),
+                         "coarse_eps": float(st.coarse_eps)}

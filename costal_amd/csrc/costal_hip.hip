@@ -48,7 +48,7 @@
 
 #include "../../include/costal_hip.h"
 
-#define COSTAL_VERSION "costal-mi355x 0.1.0 (gfx950)"
+#define COSTAL_VERSION "costal-mi355x 0.2.0 (gfx950)"
 
 
 namespace {
@@ -422,16 +422,35 @@ __device__ void bitonic_lds(double* __restrict__ sc, double* __restrict__ cs, ui
 // over the original rows, sorts, applies the certificate and writes the top k (or queues the row
 // for the fallback).
 
+// The largest of the wave's err values into one of ERR_SLOTS maxima (err_max[workgroup % ERR_SLOTS],
+// one atomic per wave at most; the host takes the largest slot).  Non-negative doubles order like
+// their bit patterns read as unsigned integers.
+constexpr uint32_t ERR_SLOTS = 256;
+__device__ __forceinline__ void wave_err_max(double err, unsigned long long* __restrict__ err_slots) {
+  unsigned long long* err_max = err_slots + (bid_x() & (ERR_SLOTS - 1));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) err = fmax(err, __shfl_xor(err, off));
+  // only a wave that beats the value already in its slot (read first, without an atomic) takes
+  // the atomic, so the updates stay few and spread over the slots
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(err);
+  if ((tid_x() & 63) == 0 && err > 0.0 &&
+      bits > __hip_atomic_load(err_max, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMax(err_max, bits);
+}
+
 // One query's rescoring (64 threads).  Failed rows are recorded as fb_tag in fb_rows.
+// err_max collects the largest |list score - f64 cosine| over the row's admitted candidates: the
+// measured counterpart of the error bound eps_list that the certificate relies on.
 template <int CMAX>
 __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_id, const float* __restrict__ x,
                             uint32_t d, const double* __restrict__ norms, uint32_t k, int kp, uint32_t nsplit,
                             uint32_t qq, uint32_t i, const uint32_t* __restrict__ cand_idx,
-                            const uint32_t* __restrict__ cand_cnt, const float* __restrict__ cand_floor,
-                            int all_admitted, double eps_list, double eps32, uint32_t* __restrict__ idx,
-                            double* __restrict__ cos_out, double* __restrict__ wts, uint32_t fb_tag,
-                            uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo,
-                            uint32_t* __restrict__ fb_count, uint32_t* __restrict__ short_count) {
+                            const float* __restrict__ cand_sc, const uint32_t* __restrict__ cand_cnt,
+                            const float* __restrict__ cand_floor, int all_admitted, double eps_list, double eps32,
+                            uint32_t* __restrict__ idx, double* __restrict__ cos_out, double* __restrict__ wts,
+                            uint32_t fb_tag, uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo,
+                            uint32_t* __restrict__ fb_count, uint32_t* __restrict__ short_count,
+                            unsigned long long* __restrict__ err_max) {
   const double ni = norms[i];
   const float* xi = x + (size_t)i * d;
   const int tot_slots = (int)nsplit * kp;
@@ -447,16 +466,20 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
   __shared__ uint32_t s_fill;
   if (tid_x() == 0) s_fill = 0;
   __syncthreads();
+  double err = 0.0;  // the largest |list score - f64 cosine| seen by this thread
   for (int e = tid_x(); e < tot_slots; e += 64) {
     const uint32_t s = (uint32_t)e / (uint32_t)kp, r = (uint32_t)e % (uint32_t)kp;
     const size_t slot = (size_t)qq * nsplit + s;
     if (r < cand_cnt[slot]) {
       const uint32_t j = cand_idx[slot * kp + r];
       const uint32_t pos = atomicAdd(&s_fill, 1u);
-      s_sc[pos] = cos64(dot64(xi, x + (size_t)j * d, d), ni, norms[j]);
+      const double c = cos64(dot64(xi, x + (size_t)j * d, d), ni, norms[j]);
+      err = fmax(err, fabs((double)cand_sc[slot * kp + r] - c));
+      s_sc[pos] = c;
       s_id[pos] = j;
     }
   }
+  wave_err_max(err, err_max);
   __syncthreads();
   const uint32_t filled = s_fill;  // = total
   if (filled <= 64) {
@@ -518,11 +541,12 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
 __device__ void rescore_half(double* __restrict__ s_sc, uint32_t* __restrict__ s_id, uint32_t* __restrict__ s_fill,
                              bool valid, uint32_t lane, const float* __restrict__ x, uint32_t d,
                              const double* __restrict__ norms, uint32_t k, int kp, uint32_t nsplit, uint32_t qq,
-                             uint32_t i, const uint32_t* __restrict__ cand_idx, const uint32_t* __restrict__ cand_cnt,
-                             const float* __restrict__ cand_floor, int all_admitted, double eps_list, double eps32,
-                             uint32_t* __restrict__ idx, double* __restrict__ cos_out, double* __restrict__ wts,
-                             uint32_t fb_tag, uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo,
-                             uint32_t* __restrict__ fb_count, uint32_t* __restrict__ short_count) {
+                             uint32_t i, const uint32_t* __restrict__ cand_idx, const float* __restrict__ cand_sc,
+                             const uint32_t* __restrict__ cand_cnt, const float* __restrict__ cand_floor,
+                             int all_admitted, double eps_list, double eps32, uint32_t* __restrict__ idx,
+                             double* __restrict__ cos_out, double* __restrict__ wts, uint32_t fb_tag,
+                             uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo, uint32_t* __restrict__ fb_count,
+                             uint32_t* __restrict__ short_count, unsigned long long* __restrict__ err_max) {
   const int tot_slots = (int)nsplit * kp;  // <= 32
   double ni = 0.0, floor_ = -INFINITY;
   uint32_t total = 0;
@@ -535,16 +559,20 @@ __device__ void rescore_half(double* __restrict__ s_sc, uint32_t* __restrict__ s
   }
   if (lane == 0) *s_fill = 0;
   __syncthreads();
+  double err = 0.0;
   if (valid && (int)lane < tot_slots) {
     const uint32_t sp = lane / (uint32_t)kp, r = lane % (uint32_t)kp;
     const size_t slot = (size_t)qq * nsplit + sp;
     if (r < cand_cnt[slot]) {
       const uint32_t j = cand_idx[slot * kp + r];
       const uint32_t pos = atomicAdd(s_fill, 1u);
-      s_sc[pos] = cos64(dot64(x + (size_t)i * d, x + (size_t)j * d, d), ni, norms[j]);
+      const double c = cos64(dot64(x + (size_t)i * d, x + (size_t)j * d, d), ni, norms[j]);
+      err = fmax(err, fabs((double)cand_sc[slot * kp + r] - c));
+      s_sc[pos] = c;
       s_id[pos] = j;
     }
   }
+  wave_err_max(err, err_max);
   __syncthreads();
   const uint32_t filled = *s_fill;
   double sc_e = 0.0;
@@ -584,10 +612,10 @@ __device__ void rescore_half(double* __restrict__ s_sc, uint32_t* __restrict__ s
 __global__ __launch_bounds__(64) void rescore_pair_kernel(
     const float* __restrict__ x, uint32_t d, const double* __restrict__ norms, uint32_t k, int kp,
     uint32_t nsplit, const uint32_t* __restrict__ rowlist, uint32_t nq, const uint32_t* __restrict__ cand_idx,
-    const uint32_t* __restrict__ cand_cnt, const float* __restrict__ cand_floor, int all_admitted,
-    double eps_list, double eps32, uint32_t* __restrict__ idx, double* __restrict__ cos_out,
+    const float* __restrict__ cand_sc, const uint32_t* __restrict__ cand_cnt, const float* __restrict__ cand_floor,
+    int all_admitted, double eps_list, double eps32, uint32_t* __restrict__ idx, double* __restrict__ cos_out,
     double* __restrict__ wts, uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo,
-    uint32_t* __restrict__ fb_count, uint32_t* __restrict__ short_count) {
+    uint32_t* __restrict__ fb_count, uint32_t* __restrict__ short_count, unsigned long long* __restrict__ err_max) {
   __shared__ double s_sc[2][32];
   __shared__ uint32_t s_id[2][32];
   __shared__ uint32_t s_fill[2];
@@ -596,25 +624,26 @@ __global__ __launch_bounds__(64) void rescore_pair_kernel(
   const bool valid = qq < nq;
   const uint32_t i = valid ? (rowlist ? rowlist[qq] : qq) : 0;
   rescore_half(s_sc[half], s_id[half], &s_fill[half], valid, lane, x, d, norms, k, kp, nsplit, qq, i, cand_idx,
-               cand_cnt, cand_floor, all_admitted, eps_list, eps32, idx, cos_out, wts, i, fb_rows, fb_lo, fb_count,
-               short_count);
+               cand_sc, cand_cnt, cand_floor, all_admitted, eps_list, eps32, idx, cos_out, wts, i, fb_rows, fb_lo,
+               fb_count, short_count, err_max);
 }
 
 template <int CMAX>
 __global__ __launch_bounds__(64) void rescore_kernel(
     const float* __restrict__ x, uint32_t d, const double* __restrict__ norms, uint32_t n,
     uint32_t k, int kp, uint32_t nsplit, const uint32_t* __restrict__ rowlist, uint32_t nq,
-    const uint32_t* __restrict__ cand_idx, const uint32_t* __restrict__ cand_cnt,
+    const uint32_t* __restrict__ cand_idx, const float* __restrict__ cand_sc, const uint32_t* __restrict__ cand_cnt,
     const float* __restrict__ cand_floor, int all_admitted, double eps_list, double eps32,
     uint32_t* __restrict__ idx, double* __restrict__ cos_out, double* __restrict__ wts,
     uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo, uint32_t* __restrict__ fb_count,
-    uint32_t* __restrict__ short_count) {
+    uint32_t* __restrict__ short_count, unsigned long long* __restrict__ err_max) {
   __shared__ double s_sc[CMAX];
   __shared__ uint32_t s_id[CMAX];
   for (uint32_t qq = bid_x(); qq < nq; qq += gdim_x()) {
     const uint32_t i = rowlist ? rowlist[qq] : qq;
-    rescore_row<CMAX>(s_sc, s_id, x, d, norms, k, kp, nsplit, qq, i, cand_idx, cand_cnt, cand_floor, all_admitted,
-                      eps_list, eps32, idx, cos_out, wts, i, fb_rows, fb_lo, fb_count, short_count);
+    rescore_row<CMAX>(s_sc, s_id, x, d, norms, k, kp, nsplit, qq, i, cand_idx, cand_sc, cand_cnt, cand_floor,
+                      all_admitted, eps_list, eps32, idx, cos_out, wts, i, fb_rows, fb_lo, fb_count, short_count,
+                      err_max);
     __syncthreads();  // the LDS arrays are refilled by the next row
   }
 }
@@ -752,6 +781,12 @@ __device__ __forceinline__ uint32_t find_sample(const uint32_t* __restrict__ fir
   return lo;
 }
 
+// counters[0, none) = all ones, counters[none, count) = 0 (a kernel, so that a call needs no host
+// copy and no memset node)
+__global__ void counters_init_kernel(uint32_t* __restrict__ counters, uint32_t count, uint32_t none) {
+  for (uint32_t e = tid_x(); e < count; e += bdim_x()) counters[e] = e < none ? NONE : 0u;
+}
+
 // Graph-safe counter reset (no memset nodes): counters[0..1] = first bad (sample, row) as u64
 // (all ones = none), everything else zero.
 __global__ void batch_init_kernel(uint32_t* __restrict__ counters, uint32_t count) {
@@ -838,31 +873,36 @@ template <int CMAX>
 __global__ __launch_bounds__(64) void rescore_batch_kernel(
     const BatchSample* __restrict__ samples, const uint32_t* __restrict__ row_first, uint32_t nsamples,
     uint32_t total_rows, uint32_t k, double eps_list, double eps32, const uint32_t* __restrict__ cand_idx,
-    const uint32_t* __restrict__ cand_cnt, const float* __restrict__ cand_floor, uint32_t* __restrict__ fb_rows,
-    double* __restrict__ fb_lo, uint32_t* __restrict__ fb_count, uint32_t* __restrict__ short_count) {
+    const float* __restrict__ cand_sc, const uint32_t* __restrict__ cand_cnt, const float* __restrict__ cand_floor,
+    uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo, uint32_t* __restrict__ fb_count,
+    uint32_t* __restrict__ short_count, unsigned long long* __restrict__ err_max,
+    const unsigned long long* __restrict__ bad) {
   __shared__ double s_sc[CMAX];
   __shared__ uint32_t s_id[CMAX];
   const uint32_t g = bid_x();
-  if (g >= total_rows) return;
+  if (g >= total_rows || *bad != ~0ull) return;
   const uint32_t sidx = find_sample(row_first, nsamples, g);
   const BatchSample& S = samples[sidx];
   const uint32_t i = g - S.row_base;
   const uint32_t nlists = 2 * S.nsplit;
   const int all_admitted = (S.nsplit == 1 && (uint32_t)S.kp >= S.n - 1) ? 1 : 0;
   rescore_row<CMAX>(s_sc, s_id, S.x, S.d, S.norms, k, S.kp, nlists, i, i, cand_idx + S.entry_base,
-                    cand_cnt + S.slot_base, cand_floor + S.slot_base, all_admitted, eps_list, eps32, S.idx, S.cos,
-                    S.wts, g, fb_rows, fb_lo, fb_count, short_count);
+                    cand_sc + S.entry_base, cand_cnt + S.slot_base, cand_floor + S.slot_base, all_admitted, eps_list,
+                    eps32, S.idx, S.cos, S.wts, g, fb_rows, fb_lo, fb_count, short_count, err_max);
 }
 
 // rescore_batch_kernel for batches whose rows hold at most 32 list entries: two rows per wave
 __global__ __launch_bounds__(64) void rescore_batch_pair_kernel(
     const BatchSample* __restrict__ samples, const uint32_t* __restrict__ row_first, uint32_t nsamples,
     uint32_t total_rows, uint32_t k, double eps_list, double eps32, const uint32_t* __restrict__ cand_idx,
-    const uint32_t* __restrict__ cand_cnt, const float* __restrict__ cand_floor, uint32_t* __restrict__ fb_rows,
-    double* __restrict__ fb_lo, uint32_t* __restrict__ fb_count, uint32_t* __restrict__ short_count) {
+    const float* __restrict__ cand_sc, const uint32_t* __restrict__ cand_cnt, const float* __restrict__ cand_floor,
+    uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo, uint32_t* __restrict__ fb_count,
+    uint32_t* __restrict__ short_count, unsigned long long* __restrict__ err_max,
+    const unsigned long long* __restrict__ bad) {
   __shared__ double s_sc[2][32];
   __shared__ uint32_t s_id[2][32];
   __shared__ uint32_t s_fill[2];
+  if (*bad != ~0ull) return;  // the whole workgroup leaves together
   const uint32_t half = tid_x() >> 5, lane = tid_x() & 31;
   const uint32_t g = bid_x() * 2 + half;
   const bool valid = g < total_rows;
@@ -870,17 +910,18 @@ __global__ __launch_bounds__(64) void rescore_batch_pair_kernel(
   const uint32_t i = valid ? g - S.row_base : 0;
   const int all_admitted = (S.nsplit == 1 && (uint32_t)S.kp >= S.n - 1) ? 1 : 0;
   rescore_half(s_sc[half], s_id[half], &s_fill[half], valid, lane, S.x, S.d, S.norms, k, S.kp, 2 * S.nsplit, i, i,
-               cand_idx + S.entry_base, cand_cnt + S.slot_base, cand_floor + S.slot_base, all_admitted, eps_list,
-               eps32, S.idx, S.cos, S.wts, g, fb_rows, fb_lo, fb_count, short_count);
+               cand_idx + S.entry_base, cand_sc + S.entry_base, cand_cnt + S.slot_base, cand_floor + S.slot_base,
+               all_admitted, eps_list, eps32, S.idx, S.cos, S.wts, g, fb_rows, fb_lo, fb_count, short_count, err_max);
 }
 
 template <int CAP>
 __global__ __launch_bounds__(256) void rowscan_batch_kernel(
     const BatchSample* __restrict__ samples, const uint32_t* __restrict__ row_first, uint32_t nsamples,
     const float* __restrict__ xs32, uint32_t dp32, uint32_t k, const uint32_t* __restrict__ rows,
-    const uint32_t* __restrict__ nrows_dev, const double* __restrict__ lo_arr, double eps32) {
+    const uint32_t* __restrict__ nrows_dev, const double* __restrict__ lo_arr, double eps32,
+    const unsigned long long* __restrict__ bad) {
   __shared__ RowscanSmem<CAP> Sm;
-  const uint32_t nrows = *nrows_dev;
+  const uint32_t nrows = *bad != ~0ull ? 0u : *nrows_dev;
   for (uint32_t r = bid_x(); r < nrows; r += gdim_x()) {
     const uint32_t g = rows[r];
     const uint32_t sidx = find_sample(row_first, nsamples, g);
@@ -945,6 +986,61 @@ __global__ void reweight_kernel(uint64_t n, uint32_t k, const uint32_t* __restri
 }
 
 // ------------------------------------------------------------------------------------------------
+// Self-check of the two matrix-core properties the split-FP16 error bound relies on: how an MFMA
+// rounds its FP32 accumulation, and whether FP16 subnormal inputs are kept.  One wave; only lane 0
+// holds non-zero operands, so every probe's result is D[0][0] = sum_j a_j b_j + c (lane 0's
+// halves are matching k indices in the A and B layouts), held by lane 0 in accumulator 0.
+//   out[0] c = 1, one product 2^-24 + 2^-30 (just above half an ulp of 1): 1 + 2^-23 when the sum
+//          is rounded to nearest, 1 when it is truncated
+//   out[1] c = 1, three products 2^-25 (a quarter ulp each): 1 + 2^-23 if the sum is rounded once,
+//          1 if each addition is rounded in turn (both round to nearest)
+//   out[2] c = 1, one product -(2^-24 + 2^-30): 1 - 2^-24 to nearest, 1 - 2^-23 truncated
+//   out[3] c = 0, 2^-24 (the smallest FP16 subnormal) x 1: 2^-24 if subnormals are kept, else 0
+//   out[4] c = 0, 2^-20 (subnormal) x 2^-4: 2^-24 if kept, else 0
+//   out[5] FP32 MFMA (32x32x2), c = 1, one product 2^-24 + 2^-30: 1 + 2^-23 to nearest
+//   out[6] FP32 MFMA, c = 1, one product -(2^-24 + 2^-30): 1 - 2^-24 to nearest
+//   out[7] as out[3], the subnormal built from its bits (0x0001) instead of by conversion
+__global__ __launch_bounds__(64) void mfma_probe_kernel(float* __restrict__ out) {
+  const bool l0 = tid_x() == 0;
+  auto run16 = [&](float a0, float a1, float a2, float b0, float b1, float b2, float c) -> float {
+    f16x8 A = {}, B = {};
+    f32x16 C = {};
+    if (l0) {
+      A[0] = (_Float16)a0, A[1] = (_Float16)a1, A[2] = (_Float16)a2;
+      B[0] = (_Float16)b0, B[1] = (_Float16)b1, B[2] = (_Float16)b2;
+      C[0] = c;
+    }
+    const f32x16 D = __builtin_amdgcn_mfma_f32_32x32x16_f16(A, B, C, 0, 0, 0);
+    return D[0];
+  };
+  auto run32 = [&](float a, float b, float c) -> float {
+    f32x16 C = {};
+    if (l0) C[0] = c;
+    const f32x16 D = __builtin_amdgcn_mfma_f32_32x32x2f32(l0 ? a : 0.0f, l0 ? b : 0.0f, C, 0, 0, 0);
+    return D[0];
+  };
+  float r[8];
+  r[0] = run16(0x1.04p-12f, 0.0f, 0.0f, 0x1p-12f, 0.0f, 0.0f, 1.0f);
+  r[1] = run16(0x1p-12f, 0x1p-12f, 0x1p-12f, 0x1p-13f, 0x1p-13f, 0x1p-13f, 1.0f);
+  r[2] = run16(-0x1.04p-12f, 0.0f, 0.0f, 0x1p-12f, 0.0f, 0.0f, 1.0f);
+  r[3] = run16(0x1p-24f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f, 0.0f);
+  r[4] = run16(0x1p-20f, 0.0f, 0.0f, 0x1p-4f, 0.0f, 0.0f, 0.0f);
+  r[5] = run32(0x1.04p-12f, 0x1p-12f, 1.0f);
+  r[6] = run32(-0x1.04p-12f, 0x1p-12f, 1.0f);
+  {
+    f16x8 A = {}, B = {};
+    f32x16 C = {};
+    if (l0) {
+      A[0] = __builtin_bit_cast(_Float16, (uint16_t)0x0001u);
+      B[0] = (_Float16)1.0f;
+    }
+    r[7] = __builtin_amdgcn_mfma_f32_32x32x16_f16(A, B, C, 0, 0, 0)[0];
+  }
+  if (l0)
+    for (int e = 0; e < 8; ++e) out[e] = r[e];
+}
+
+// ------------------------------------------------------------------------------------------------
 // Host side
 
 thread_local std::string g_err;
@@ -975,12 +1071,33 @@ struct Device {
   hipStream_t stream = nullptr;
   void* ws = nullptr;
   size_t ws_cap = 0;
+  // device copies of the host entry points' arguments (grow-only, kept between calls)
+  void* io = nullptr;
+  size_t io_cap = 0;
   // call start/end and per-kernel brackets: prep, coarse, rescore, fallback
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evk[8] = {};
 };
 
-Device g_dev;
+// One state per device, selected by the device current at the call (a process may drive several
+// GPUs, one thread each).
+constexpr int MAX_DEVICES = 16;
+Device g_devs[MAX_DEVICES];
+
+Device* call_device() {
+  int id = 0;
+  hipError_t e = hipGetDevice(&id);
+  if (e != hipSuccess) {
+    fail(COSTAL_E_DEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+    return nullptr;
+  }
+  if (id < 0 || id >= MAX_DEVICES) {
+    fail(COSTAL_E_DEVICE, "device %d is beyond the %d devices this library serves", id, MAX_DEVICES);
+    return nullptr;
+  }
+  g_devs[id].id = id;
+  return &g_devs[id];
+}
 
 int device_init(Device& D) {
   if (D.init) return 0;
@@ -988,7 +1105,6 @@ int device_init(Device& D) {
   hipError_t e = hipGetDeviceCount(&count);
   if (e != hipSuccess || count == 0)
     return fail(COSTAL_E_DEVICE, "no HIP device available (%s)", hipGetErrorString(e));
-  HIP_TRY(hipGetDevice(&D.id));
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, D.id));
   D.cus = prop.multiProcessorCount;
@@ -1014,6 +1130,37 @@ int ws_reserve(Device& D, size_t bytes) {
   return 0;
 }
 
+// The host entry points' device buffers: grow-only like the workspace, so a call of the same or a
+// smaller size allocates nothing.
+int io_reserve(Device& D, size_t bytes) {
+  if (bytes <= D.io_cap) return 0;
+  if (D.io) {
+    HIP_TRY(hipStreamSynchronize(D.stream));
+    HIP_TRY(hipFree(D.io));
+    D.io = nullptr;
+    D.io_cap = 0;
+  }
+  const size_t cap = bytes + bytes / 8 + (1u << 20);
+  HIP_TRY(hipMalloc(&D.io, cap));
+  D.io_cap = cap;
+  return 0;
+}
+
+// Counter block of a call: 64 words of counters, then ERR_SLOTS u64 maxima of the measured error.
+constexpr uint32_t COUNTER_WORDS = 64 + 2 * ERR_SLOTS;
+
+// the largest of the ERR_SLOTS error maxima (f64 bit patterns, two words each)
+double slots_max(const uint32_t* w) {
+  double m = 0.0;
+  for (uint32_t e = 0; e < ERR_SLOTS; ++e) {
+    const uint64_t b = (uint64_t)w[2 * e] | ((uint64_t)w[2 * e + 1] << 32);
+    double v;
+    memcpy(&v, &b, sizeof v);
+    m = std::max(m, v);
+  }
+  return m;
+}
+
 // bump allocator over the workspace
 struct Carve {
   uint8_t* base;
@@ -1033,6 +1180,7 @@ inline uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
 // The fused path handles d <= 64 (FP32) / 128 (FP16) and K' <= 64; everything else runs the
 // exact row scan.
 constexpr uint32_t FUSED_KP_MAX = 64;
+constexpr uint32_t RESCORE_CMAX = 2048;  // list entries one row's rescoring holds (largest rescore_kernel)
 constexpr uint32_t ROWSCAN_K_MAX = 2048;
 
 struct Plan {
@@ -1061,13 +1209,15 @@ CoarseFn coarse_select(int prec, uint32_t J, int KP) {
 // FP16 candidate error bound.  x16 = fp16(fp32(x/|x|)): |x16 - x| <= U|x| + eta per component with
 // U = 2^-11 (1 + 2^-24) + 2^-24 and eta = 2^-25 (half the FP16 subnormal spacing).  With |x| = 1,
 // sum |x16 y16 - x y| <= (U + eta sqrt d)(2 + U + eta sqrt d); the FP32 accumulation of the exact
-// FP16 products adds gamma_d (1 + U + eta sqrt d)^2.  eps_pre also covers the f32 re-score error
-// ((d+4) 2^-23) and the rounding of (threshold - eps_pre) in f32.
+// FP16 products adds gamma_d (1 + U + eta sqrt d)^2, gamma_m = m u / (1 - m u) with u = 2^-24: the
+// matrix core rounds its FP32 accumulation to nearest (costal_mfma_probe shows it, and that FP16
+// subnormal inputs are kept), and gamma_m bounds any order of m rounded additions.  eps_pre also
+// covers the f32 re-score error ((d+4) 2^-23) and the rounding of (threshold - eps_pre) in f32.
 double fp16_eps(uint32_t d) {
-  const double u32 = std::ldexp(1.0, -24);
+  const double u32 = std::ldexp(1.0, -24), uacc = u32;
   const double U = std::ldexp(1.0, -11) * (1.0 + u32) + u32;
   const double e = std::ldexp(1.0, -25) * std::sqrt((double)d);
-  const double gam = d * u32 / (1.0 - d * u32);
+  const double gam = d * uacc / (1.0 - d * uacc);
   return (U + e) * (2.0 + U + e) + gam * (1.0 + U + e) * (1.0 + U + e);
 }
 
@@ -1078,12 +1228,12 @@ double fp32_eps(uint32_t d) { return ((double)d + 4.0) * (double)std::ldexp(1.0,
 // The MFMA sum hi.hi + hi.lo + lo.hi differs from (hi+lo).(hi+lo) by the dropped lo.lo
 // (<= (2^-11 + eta sqrt d)^2), the represented rows differ from the unit rows by
 // (a + eta sqrt d)(2 + a + eta sqrt d), and the FP32 accumulation of 3d exact products adds
-// gamma_3d (1 + 2^-11 + eta sqrt d)^2.
+// gamma_3d (1 + 2^-11 + eta sqrt d)^2 (u = 2^-24: round-to-nearest accumulation, as fp16_eps).
 double fp16x3_eps(uint32_t d) {
-  const double u32 = std::ldexp(1.0, -24), u16 = std::ldexp(1.0, -11);
+  const double u32 = std::ldexp(1.0, -24), u16 = std::ldexp(1.0, -11), uacc = u32;
   const double a = std::ldexp(1.0, -22) * (1.0 + u32) + u32;
   const double e = std::ldexp(1.0, -25) * std::sqrt((double)d);
-  const double gam = 3.0 * d * u32 / (1.0 - 3.0 * d * u32);
+  const double gam = 3.0 * d * uacc / (1.0 - 3.0 * d * uacc);
   return (a + e) * (2.0 + a + e) + (u16 + e) * (u16 + e) + gam * (1.0 + u16 + e) * (1.0 + u16 + e);
 }
 
@@ -1123,6 +1273,8 @@ Plan make_plan(const Device& D, uint64_t n, uint64_t d, uint64_t k, const costal
     ns = std::min<uint32_t>(ns, 8);
   }
   ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, std::max<uint32_t>(1, ntile / 2)));
+  // the rescoring gathers 2 * splits * K' list entries per row into at most 2048 LDS slots
+  ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, RESCORE_CMAX / (2 * P.kp)));
   const uint32_t tiles_per = (ntile + ns - 1) / ns;
   P.cols_per_split = tiles_per * P.TC;
   P.nsplit = (ntile + tiles_per - 1) / tiles_per;
@@ -1189,10 +1341,22 @@ int launch_rowscan(Device& D, hipStream_t s, bool coarse, const float* x, uint32
   return 0;
 }
 
-// The whole path on device pointers.  Serialized by the caller (D.mu held).
+// A caller-fixed workspace (base, capacity) in place of the grow-only one.
+struct FixedWs {
+  void* base;
+  size_t cap;
+};
+
+// The whole path on device pointers.  Serialized by the caller (D.mu held).  With `fixed`, every
+// buffer comes from that region and nothing is allocated: a plan that does not fit is redone with
+// one candidate range per row (and, for wide rows, smaller score blocks), then refused.
 int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, const costal_knn_params& p,
                       uint32_t* idx, double* cos, double* norms_out, double* wts, costal_knn_stats* st,
-                      hipStream_t s, bool brute) {
+                      hipStream_t s, bool brute, const FixedWs* fixed = nullptr) {
+  if (st) {
+    memset(st, 0, sizeof *st);
+    st->nonfinite_row = st->nonfinite_sample = UINT64_MAX;
+  }
   const uint32_t n = (uint32_t)n64, d = (uint32_t)d64, k = (uint32_t)p.k;
   if (p.precision > COSTAL_PREC_FP16X3) return fail(COSTAL_E_INVALID, "unknown precision %u", p.precision);
   if (p.select > COSTAL_SELECT_TANIMOTO) return fail(COSTAL_E_INVALID, "unknown select %u", p.select);
@@ -1226,17 +1390,17 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   const bool need16 = P.fused && (P.prec == PREC_F16 || P.prec == PREC_F16X3);
   const int split16 = P.prec == PREC_F16X3 ? 1 : 0;
   const size_t row16 = (size_t)P.dp16 * (split16 ? 2 : 1);
-  // two candidate lists per query and split (one per lane of the query's lane pair)
-  const size_t nq_slots = (P.fused || P.wide) ? (size_t)n * P.nsplit * 2 : 0;
   // list capacity of the rescan of first-pass failures; the list buffers hold both passes' lists
   // (the rescan keeps at most 2 lists of kp2 per failed row per split, and at least one split fits)
   const uint32_t kp2 = (uint32_t)std::min<uint64_t>(
       std::max<uint64_t>(P.kp, std::min<uint64_t>((uint64_t)k + (k + 1) / 2, FUSED_KP_MAX)), n - 1);
-  const size_t nq_entries = P.fused ? std::max<size_t>(nq_slots * P.kp, (size_t)n * 2 * kp2) : nq_slots * P.kp;
-  const size_t score_floats = P.wide ? (size_t)P.wide_qb * P.n_pad : 0;
-
-  size_t bytes = 0;
-  {
+  size_t nq_slots = 0, nq_entries = 0, score_floats = 0;
+  const bool learned = P.fused && p.mlp_params && !p.row_threshold;
+  auto workspace_bytes = [&]() -> size_t {
+    // two candidate lists per query and split (one per lane of the query's lane pair)
+    nq_slots = (P.fused || P.wide) ? (size_t)n * P.nsplit * 2 : 0;
+    nq_entries = P.fused ? std::max<size_t>(nq_slots * P.kp, (size_t)n * 2 * kp2) : nq_slots * P.kp;
+    score_floats = P.wide ? (size_t)P.wide_qb * P.n_pad : 0;
     Carve c(nullptr);
     c.take<float>((size_t)P.n_pad * P.dp32);
     if (need16) c.take<_Float16>((size_t)P.n_pad * row16);
@@ -1249,15 +1413,36 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     c.take<double>(n);
     c.take<uint32_t>(n);
     c.take<double>(n);
-    c.take<uint32_t>(64);
-    if (P.fused && p.mlp_params && !p.row_threshold) c.take<float>(n);
+    c.take<uint32_t>(COUNTER_WORDS);
+    if (learned) c.take<float>(n);
     if (P.fused) c.take<float>(n);
     c.take<float>(score_floats);
-    bytes = c.off + 256;
+    return c.off + 256;
+  };
+  size_t bytes = workspace_bytes();
+  if (fixed && bytes > fixed->cap && (P.fused || P.wide)) {
+    costal_knn_params p1 = p;
+    p1.col_splits = 1;
+    const Plan P1 = make_plan(D, n, d, k, p1);
+    P.nsplit = P1.nsplit;
+    P.cols_per_split = P1.cols_per_split;
+    units_per_wg = grid_bal = 0;
+    bytes = workspace_bytes();
+    while (P.wide && bytes > fixed->cap && P.wide_qb > 256) {
+      P.wide_qb = std::max<uint32_t>(256, P.wide_qb / 2 / 256 * 256);
+      bytes = workspace_bytes();
+    }
   }
-  int rc = ws_reserve(D, bytes);
-  if (rc) return rc;
-  Carve c(D.ws);
+  int rc;
+  if (fixed) {
+    if (bytes > fixed->cap)
+      return fail(COSTAL_E_UNSUPPORTED,
+                  "n=%u, d=%u, k=%u needs %zu bytes of device workspace; the fixed workspace holds %zu",
+                  n, d, k, bytes, fixed->cap);
+  } else if ((rc = ws_reserve(D, bytes))) {
+    return rc;
+  }
+  Carve c(fixed ? fixed->base : D.ws);
   float* xs32 = c.take<float>((size_t)P.n_pad * P.dp32);
   _Float16* xs16 = need16 ? c.take<_Float16>((size_t)P.n_pad * row16) : nullptr;
   double* norms_ws = c.take<double>(n);
@@ -1272,18 +1457,15 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   // [0] first bad row, [1] rows failing the first scan, [2] of them with < k admitted, [3] rows
   // for the full-row kernel after a threshold-free rescan, [4] short rows of that rescan,
   // [8..9] candidate-list inserts (u64)
-  uint32_t* counters = c.take<uint32_t>(64);
-  const bool learned = P.fused && p.mlp_params && !p.row_threshold;
+  uint32_t* counters = c.take<uint32_t>(COUNTER_WORDS);
   float* th_ws = learned ? c.take<float>(n) : nullptr;
   float* th_rescan = P.fused ? c.take<float>(n) : nullptr;
   float* scores = P.wide ? c.take<float>(score_floats) : nullptr;
   double* norms = norms_out ? norms_out : norms_ws;
 
   HIP_TRY(hipEventRecord(D.ev0, s));
-  {
-    uint32_t init[16] = {NONE, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    HIP_TRY(hipMemcpyAsync(counters, init, sizeof init, hipMemcpyHostToDevice, s));
-  }
+  counters_init_kernel<<<1, 256, 0, s>>>(counters, COUNTER_WORDS, 1);
+  HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(D.evk[0], s));
   if (d > 64 && !need16)
     prep_wide_kernel<<<(P.n_pad + PREP_ROWS - 1) / PREP_ROWS, 256, 0, s>>>(x, n, d, P.n_pad, P.dp32, norms, xs32,
@@ -1296,10 +1478,14 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   uint32_t bad = NONE;
   HIP_TRY(hipMemcpyAsync(&bad, counters, sizeof bad, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  if (bad != NONE) return fail(COSTAL_E_NON_FINITE, "row %u contains NaN or inf", bad);
+  if (bad != NONE) {
+    if (st) st->nonfinite_row = bad;
+    return fail(COSTAL_E_NON_FINITE, "row %u contains NaN or inf", bad);
+  }
 
   const double eps32 = fp32_eps(d);
   unsigned long long* inserts = reinterpret_cast<unsigned long long*>(counters + 8);
+  unsigned long long* err_max = reinterpret_cast<unsigned long long*>(counters + 64);  // ERR_SLOTS
   // f64 rescoring of the candidate lists of the queries `rows` (null: every row) with the
   // certificate; rows that fail it are appended to (fbr, fbl) and counted in *fbc
   auto rescore = [&](uint32_t kp, uint32_t nsplit, const uint32_t* rows, uint32_t nq, int all_admitted,
@@ -1307,13 +1493,13 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     const uint32_t nlists = 2 * nsplit;
     const uint32_t slots = nlists * kp;
 #define RESCORE(CM)                                                                                    \
-  rescore_kernel<CM><<<nq, 64, 0, s>>>(x, d, norms, n, k, (int)kp, nlists, rows, nq, cand_idx, cand_cnt,     \
-                                       cand_floor, all_admitted, eps_list, eps32, idx, cos, wts, fbr, fbl,  \
-                                       fbc, shc)
+  rescore_kernel<CM><<<nq, 64, 0, s>>>(x, d, norms, n, k, (int)kp, nlists, rows, nq, cand_idx, cand_sc,       \
+                                       cand_cnt, cand_floor, all_admitted, eps_list, eps32, idx, cos, wts, fbr, \
+                                       fbl, fbc, shc, err_max)
     if (slots <= 32) {
-      rescore_pair_kernel<<<(nq + 1) / 2, 64, 0, s>>>(x, d, norms, k, (int)kp, nlists, rows, nq, cand_idx, cand_cnt,
-                                                      cand_floor, all_admitted, eps_list, eps32, idx, cos, wts, fbr,
-                                                      fbl, fbc, shc);
+      rescore_pair_kernel<<<(nq + 1) / 2, 64, 0, s>>>(x, d, norms, k, (int)kp, nlists, rows, nq, cand_idx, cand_sc,
+                                                      cand_cnt, cand_floor, all_admitted, eps_list, eps32, idx, cos,
+                                                      wts, fbr, fbl, fbc, shc, err_max);
     } else if (slots <= 64) RESCORE(64);
     else if (slots <= 128) RESCORE(128);
     else if (slots <= 256) RESCORE(256);
@@ -1448,7 +1634,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     HIP_TRY(hipEventRecord(D.evk[5], s));
   }
   HIP_TRY(hipEventRecord(D.ev1, s));
-  uint32_t cnt[16] = {};
+  uint32_t cnt[COUNTER_WORDS] = {};
   HIP_TRY(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   if (st) {
@@ -1474,6 +1660,8 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     st->coarse_ms = span(2, 3);
     st->rescore_ms = span(3, 4);
     st->fallback_ms = span(4, 5);
+    st->coarse_err_max = slots_max(cnt + 64);
+    st->coarse_eps = P.fused ? (P.prec == PREC_F16X3 ? fp16x3_eps(d) : fp32_eps(d)) : P.wide ? fp32_eps(d) : 0.0;
   }
   return 0;
 }
@@ -1505,13 +1693,15 @@ struct BatchGraph {
   std::function<hipError_t(hipStream_t)> launch;  // the batch's launch sequence
   // host copies of what the stats need
   uint32_t QB = 0, TC = 0, max_split = 0, max_kp = 0;
+  double eps_list = 0.0;
   uint64_t blocks_total = 0;
   size_t off_counters = 0;
 };
-BatchGraph g_batch;
+BatchGraph g_batches[MAX_DEVICES];
 
 int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const costal_knn_params& p,
                        costal_knn_stats* st, hipStream_t s) {
+  BatchGraph& g_batch = g_batches[D.id];
   const uint64_t k = p.k;
   if (p.select != COSTAL_SELECT_COSINE) return fail(COSTAL_E_UNSUPPORTED, "batched samples use cosine selection");
   if (p.precision > COSTAL_PREC_FP16X3) return fail(COSTAL_E_INVALID, "unknown precision %u", p.precision);
@@ -1637,7 +1827,7 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
       c.take<float>(slots);
       c.take<uint32_t>(total_rows);
       c.take<double>(total_rows);
-      c.take<uint32_t>(64);
+      c.take<uint32_t>(COUNTER_WORDS);
       c.take<BatchSample>(S);
       c.take<uint32_t>(S);
       c.take<uint32_t>(S);
@@ -1664,7 +1854,7 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
     float* cand_floor = c.take<float>(slots);
     uint32_t* fb_rows = c.take<uint32_t>(total_rows);
     double* fb_lo = c.take<double>(total_rows);
-    uint32_t* counters = c.take<uint32_t>(64);
+    uint32_t* counters = c.take<uint32_t>(COUNTER_WORDS);
     const size_t off_counters = (size_t)(reinterpret_cast<uint8_t*>(counters) - static_cast<uint8_t*>(g_batch.ws));
     BatchSample* d_samples = c.take<BatchSample>(S);
     uint32_t* d_pad_first = c.take<uint32_t>(S);
@@ -1683,23 +1873,28 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
     const uint32_t max_slots = 2 * max_split * max_kp;
     const uint32_t grid_scan = 2u * (uint32_t)std::max(D.cus, 1);
     const uint32_t grid_fb = 4u * (uint32_t)std::max(D.cus, 1);
+    // counters: [0..1] first bad (sample, row), [2] rows failing the certificate, [3] of them short,
+    // [4..5] inserts, [6] work-queue head, [64..) ERR_SLOTS maxima of the measured list-score error
+    const unsigned long long* bad = reinterpret_cast<const unsigned long long*>(counters);
+    unsigned long long* err_max = reinterpret_cast<unsigned long long*>(counters + 64);
 
     // the whole batch as one sequence of launches (no host round trip, no memset nodes)
     g_batch.launch = [=](hipStream_t st) {
-      batch_init_kernel<<<1, 64, 0, st>>>(counters, 64);
+      batch_init_kernel<<<1, 256, 0, st>>>(counters, COUNTER_WORDS);
       prep_batch_kernel<<<(pad + 255) / 256, 256, 0, st>>>(d_samples, d_pad_first, S, pad, dp32, dp16, split16, xs32,
                                                            xs16, reinterpret_cast<unsigned long long*>(counters));
       fn<<<grid_scan, 256, 0, st>>>(d_samples, d_items, nitems, counters + 6, xsq, xs32, dp32, eps_pre, cand_idx,
-                                    cand_sc, cand_cnt, cand_floor, reinterpret_cast<unsigned long long*>(counters + 4));
+                                    cand_sc, cand_cnt, cand_floor, reinterpret_cast<unsigned long long*>(counters + 4),
+                                    bad);
 #define RESCORE_B(CM)                                                                                             \
   rescore_batch_kernel<CM><<<(uint32_t)total_rows, 64, 0, st>>>(d_samples, d_row_first, S, (uint32_t)total_rows, \
-                                                                (uint32_t)k, eps_list, eps32, cand_idx, cand_cnt, \
-                                                                cand_floor, fb_rows, fb_lo, counters + 2,          \
-                                                                counters + 3)
+                                                                (uint32_t)k, eps_list, eps32, cand_idx, cand_sc,  \
+                                                                cand_cnt, cand_floor, fb_rows, fb_lo, counters + 2, \
+                                                                counters + 3, err_max, bad)
       if (max_slots <= 32)
         rescore_batch_pair_kernel<<<(uint32_t)((total_rows + 1) / 2), 64, 0, st>>>(
-            d_samples, d_row_first, S, (uint32_t)total_rows, (uint32_t)k, eps_list, eps32, cand_idx, cand_cnt,
-            cand_floor, fb_rows, fb_lo, counters + 2, counters + 3);
+            d_samples, d_row_first, S, (uint32_t)total_rows, (uint32_t)k, eps_list, eps32, cand_idx, cand_sc,
+            cand_cnt, cand_floor, fb_rows, fb_lo, counters + 2, counters + 3, err_max, bad);
       else if (max_slots <= 64) RESCORE_B(64);
       else if (max_slots <= 128) RESCORE_B(128);
       else if (max_slots <= 256) RESCORE_B(256);
@@ -1709,10 +1904,10 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
 #undef RESCORE_B
       if (k <= 512)
         rowscan_batch_kernel<1024><<<grid_fb, 256, 0, st>>>(d_samples, d_row_first, S, xs32, dp32, (uint32_t)k,
-                                                            fb_rows, counters + 2, fb_lo, eps32);
+                                                            fb_rows, counters + 2, fb_lo, eps32, bad);
       else
         rowscan_batch_kernel<4096><<<grid_fb, 256, 0, st>>>(d_samples, d_row_first, S, xs32, dp32, (uint32_t)k,
-                                                            fb_rows, counters + 2, fb_lo, eps32);
+                                                            fb_rows, counters + 2, fb_lo, eps32, bad);
       return hipGetLastError();
     };
     g_batch.use_graph = getenv("COSTAL_NO_GRAPH") == nullptr;
@@ -1737,6 +1932,7 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
     g_batch.TC = TC;
     g_batch.max_split = max_split;
     g_batch.max_kp = max_kp;
+    g_batch.eps_list = eps_list;
     g_batch.blocks_total = blocks_total;
     g_batch.off_counters = off_counters;
   }
@@ -1749,11 +1945,17 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
     HIP_TRY(g_batch.launch(s));
   }
   HIP_TRY(hipEventRecord(D.ev1, s));
-  uint32_t cnt[16] = {};
+  uint32_t cnt[COUNTER_WORDS] = {};
   HIP_TRY(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  if (cnt[0] != NONE || cnt[1] != NONE)
+  if (cnt[0] != NONE || cnt[1] != NONE) {
+    if (st) {
+      memset(st, 0, sizeof *st);
+      st->nonfinite_row = cnt[0];
+      st->nonfinite_sample = cnt[1];
+    }
     return fail(COSTAL_E_NON_FINITE, "sample %u: row %u contains NaN or inf", cnt[1], cnt[0]);
+  }
   if (st) {
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, D.ev0, D.ev1) != hipSuccess) ms = 0.0f;
@@ -1770,11 +1972,17 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
     st->inserts = (uint64_t)cnt[4] | ((uint64_t)cnt[5] << 32);
     st->device_ms = ms;
     st->coarse_ms = ms;
+    st->nonfinite_row = st->nonfinite_sample = UINT64_MAX;
+    st->coarse_err_max = slots_max(cnt + 64);
+    st->coarse_eps = g_batch.eps_list;
   }
   return 0;
 }
 
 }  // namespace
+
+static int reweight_locked(Device& D, uint64_t n, uint64_t k, const uint32_t* idx, const double* cos,
+                           const double* norms, uint32_t weight, double* out, hipStream_t s);
 
 // ==================================================================================================
 // C ABI
@@ -1786,13 +1994,44 @@ const char* costal_last_error(void) { return g_err.c_str(); }
 const char* costal_version(void) { return COSTAL_VERSION; }
 
 int costal_device_info(char* buf, size_t len) {
-  std::lock_guard<std::mutex> lk(g_dev.mu);
-  int rc = device_init(g_dev);
+  Device* Dp = call_device();
+  if (!Dp) return COSTAL_E_DEVICE;
+  Device& D = *Dp;
+  std::lock_guard<std::mutex> lk(D.mu);
+  int rc = device_init(D);
   if (rc) return rc;
   hipDeviceProp_t prop;
-  HIP_TRY(hipGetDeviceProperties(&prop, g_dev.id));
+  HIP_TRY(hipGetDeviceProperties(&prop, D.id));
   snprintf(buf, len, "%s (%s), %d CUs, %.1f GiB, clock %d kHz", prop.name, prop.gcnArchName,
            prop.multiProcessorCount, prop.totalGlobalMem / 1073741824.0, prop.clockRate);
+  return 0;
+}
+
+int costal_knn_plan(uint64_t n, uint64_t d, const costal_knn_params* params, costal_knn_stats* stats) {
+  if (!params || !stats) return fail(COSTAL_E_INVALID, "null pointer argument");
+  int rc = check_args(n, d, params->k);
+  if (rc) return rc;
+  if (params->precision > COSTAL_PREC_FP16X3) return fail(COSTAL_E_INVALID, "unknown precision %u", params->precision);
+  if (params->select > COSTAL_SELECT_TANIMOTO) return fail(COSTAL_E_INVALID, "unknown select %u", params->select);
+  Device probe;  // the CU count of the current device when one is present, else 256
+  probe.cus = 256;
+  int dev = 0, count = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceCount(&count) == hipSuccess && count > 0 && hipGetDevice(&dev) == hipSuccess &&
+      hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    probe.cus = prop.multiProcessorCount;
+  (void)hipGetLastError();
+  const Plan P = make_plan(probe, n, d, params->k, *params);
+  memset(stats, 0, sizeof *stats);
+  stats->nonfinite_row = stats->nonfinite_sample = UINT64_MAX;
+  stats->path = P.fused ? 0 : P.wide ? 2 : 1;
+  stats->row_block = (P.fused || P.wide) ? P.QB : 1;
+  stats->col_block = (P.fused || P.wide) ? P.TC : 256;
+  stats->candidates = P.kp;
+  stats->col_splits = (P.fused || P.wide) ? P.nsplit : 1;
+  stats->blocks_total = (P.fused || P.wide) ? (uint64_t)((n + P.QB - 1) / P.QB) * ((n + P.TC - 1) / P.TC) : n;
+  stats->coarse_eps = P.fused ? (P.prec == PREC_F16X3 ? fp16x3_eps((uint32_t)d) : fp32_eps((uint32_t)d))
+                              : P.wide ? fp32_eps((uint32_t)d) : 0.0;
   return 0;
 }
 
@@ -1802,11 +2041,31 @@ int costal_knn_exact_device(const float* x, uint64_t n, uint64_t d, const costal
   if (!params || !x || !idx || !cos) return fail(COSTAL_E_INVALID, "null pointer argument");
   int rc = check_args(n, d, params->k);
   if (rc) return rc;
-  std::lock_guard<std::mutex> lk(g_dev.mu);
-  rc = device_init(g_dev);
+  Device* Dp = call_device();
+  if (!Dp) return COSTAL_E_DEVICE;
+  Device& D = *Dp;
+  std::lock_guard<std::mutex> lk(D.mu);
+  rc = device_init(D);
   if (rc) return rc;
-  return knn_device_locked(g_dev, x, n, d, *params, idx, cos, norms, weights, stats,
-                           pick_stream(g_dev, stream), false);
+  return knn_device_locked(D, x, n, d, *params, idx, cos, norms, weights, stats,
+                           pick_stream(D, stream), false);
+}
+
+int costal_mfma_probe(float* out) {
+  if (!out) return fail(COSTAL_E_INVALID, "null pointer argument");
+  Device* Dp = call_device();
+  if (!Dp) return COSTAL_E_DEVICE;
+  Device& D = *Dp;
+  std::lock_guard<std::mutex> lk(D.mu);
+  int rc = device_init(D);
+  if (rc) return rc;
+  if ((rc = ws_reserve(D, 256))) return rc;
+  float* dout = reinterpret_cast<float*>(D.ws);
+  mfma_probe_kernel<<<1, 64, 0, D.stream>>>(dout);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, dout, 8 * sizeof(float), hipMemcpyDeviceToHost, D.stream));
+  HIP_TRY(hipStreamSynchronize(D.stream));
+  return 0;
 }
 
 static int knn_host(const float* x, uint64_t n, uint64_t d, const costal_knn_params* params, uint32_t* idx,
@@ -1814,64 +2073,58 @@ static int knn_host(const float* x, uint64_t n, uint64_t d, const costal_knn_par
   if (!params || !x || !idx || !cos) return fail(COSTAL_E_INVALID, "null pointer argument");
   int rc = check_args(n, d, params->k);
   if (rc) return rc;
-  std::lock_guard<std::mutex> lk(g_dev.mu);
-  rc = device_init(g_dev);
+  Device* Dp = call_device();
+  if (!Dp) return COSTAL_E_DEVICE;
+  Device& D = *Dp;
+  std::lock_guard<std::mutex> lk(D.mu);
+  rc = device_init(D);
   if (rc) return rc;
   const uint64_t k = params->k;
-  hipStream_t s = g_dev.stream;
-  float* dx = nullptr;
-  uint32_t* didx = nullptr;
-  double *dcos = nullptr, *dnorms = nullptr, *dw = nullptr;
-  float* dth = nullptr;
-  float* dmlp = nullptr;
-  auto cleanup = [&]() {
-    (void)hipFree(dx);
-    (void)hipFree(didx);
-    (void)hipFree(dcos);
-    (void)hipFree(dnorms);
-    (void)hipFree(dw);
-    (void)hipFree(dth);
-    (void)hipFree(dmlp);
+  hipStream_t s = D.stream;
+  const bool mlp = params->mlp_params && !params->row_threshold;
+  const size_t mlp_count = mlp ? costal_mlp_param_count(d) : 0;
+  // the call's device copies, carved from the grow-only buffer
+  auto carve = [&](Carve& c, float*& dx, uint32_t*& didx, double*& dcos, double*& dnorms, double*& dw, float*& dth,
+                   float*& dmlp) {
+    dx = c.take<float>(std::max<size_t>(n * d, 1));
+    didx = c.take<uint32_t>(n * k);
+    dcos = c.take<double>(n * k);
+    dnorms = c.take<double>(n);
+    dw = weights ? c.take<double>(n * k) : nullptr;
+    dth = params->row_threshold ? c.take<float>(n) : nullptr;
+    dmlp = mlp ? c.take<float>(mlp_count) : nullptr;
   };
-#define HT(expr)                                                                      \
-  do {                                                                                \
-    hipError_t e_ = (expr);                                                           \
-    if (e_ != hipSuccess) {                                                           \
-      cleanup();                                                                      \
-      return fail(COSTAL_E_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));    \
-    }                                                                                 \
-  } while (0)
-  HT(hipMalloc(&dx, std::max<size_t>(n * d, 1) * sizeof(float)));
-  HT(hipMalloc(&didx, n * k * sizeof(uint32_t)));
-  HT(hipMalloc(&dcos, n * k * sizeof(double)));
-  HT(hipMalloc(&dnorms, n * sizeof(double)));
-  if (weights) HT(hipMalloc(&dw, n * k * sizeof(double)));
+  float *dx, *dth, *dmlp;
+  uint32_t* didx;
+  double *dcos, *dnorms, *dw;
+  {
+    Carve c(nullptr);
+    carve(c, dx, didx, dcos, dnorms, dw, dth, dmlp);
+    rc = io_reserve(D, c.off + 256);
+    if (rc) return rc;
+  }
+  Carve c(D.io);
+  carve(c, dx, didx, dcos, dnorms, dw, dth, dmlp);
   costal_knn_params p = *params;
-  if (params->mlp_params && !params->row_threshold) {
-    const size_t cnt = costal_mlp_param_count(d);
-    HT(hipMalloc(&dmlp, cnt * sizeof(float)));
-    HT(hipMemcpyAsync(dmlp, params->mlp_params, cnt * sizeof(float), hipMemcpyHostToDevice, s));
+  if (mlp) {
+    HIP_TRY(hipMemcpyAsync(dmlp, params->mlp_params, mlp_count * sizeof(float), hipMemcpyHostToDevice, s));
     p.mlp_params = dmlp;
   }
-  if (params->row_threshold) {
-    HT(hipMalloc(&dth, n * sizeof(float)));
-    HT(hipMemcpyAsync(dth, params->row_threshold, n * sizeof(float), hipMemcpyHostToDevice, s));
+  if (dth) {
+    HIP_TRY(hipMemcpyAsync(dth, params->row_threshold, n * sizeof(float), hipMemcpyHostToDevice, s));
     p.row_threshold = dth;
   }
-  HT(hipMemcpyAsync(dx, x, n * d * sizeof(float), hipMemcpyHostToDevice, s));
-  rc = knn_device_locked(g_dev, dx, n, d, p, didx, dcos, dnorms, dw, stats, s, brute);
+  HIP_TRY(hipMemcpyAsync(dx, x, n * d * sizeof(float), hipMemcpyHostToDevice, s));
+  rc = knn_device_locked(D, dx, n, d, p, didx, dcos, dnorms, dw, stats, s, brute);
   if (rc) {
     (void)hipStreamSynchronize(s);
-    cleanup();
     return rc;
   }
-  HT(hipMemcpyAsync(idx, didx, n * k * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HT(hipMemcpyAsync(cos, dcos, n * k * sizeof(double), hipMemcpyDeviceToHost, s));
-  if (norms) HT(hipMemcpyAsync(norms, dnorms, n * sizeof(double), hipMemcpyDeviceToHost, s));
-  if (weights) HT(hipMemcpyAsync(weights, dw, n * k * sizeof(double), hipMemcpyDeviceToHost, s));
-  HT(hipStreamSynchronize(s));
-#undef HT
-  cleanup();
+  HIP_TRY(hipMemcpyAsync(idx, didx, n * k * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(cos, dcos, n * k * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (norms) HIP_TRY(hipMemcpyAsync(norms, dnorms, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (weights) HIP_TRY(hipMemcpyAsync(weights, dw, n * k * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
   return 0;
 }
 
@@ -1885,10 +2138,13 @@ int costal_knn_batched_device(const costal_sample* samples, uint32_t nsamples, c
                               costal_knn_stats* stats, void* stream) {
   if (!params || (!samples && nsamples)) return fail(COSTAL_E_INVALID, "null pointer argument");
   if (nsamples == 0) return 0;
-  std::lock_guard<std::mutex> lk(g_dev.mu);
-  int rc = device_init(g_dev);
+  Device* Dp = call_device();
+  if (!Dp) return COSTAL_E_DEVICE;
+  Device& D = *Dp;
+  std::lock_guard<std::mutex> lk(D.mu);
+  int rc = device_init(D);
   if (rc) return rc;
-  return knn_batched_locked(g_dev, samples, nsamples, *params, stats, pick_stream(g_dev, stream));
+  return knn_batched_locked(D, samples, nsamples, *params, stats, pick_stream(D, stream));
 }
 
 int costal_knn_bruteforce(const float* x, uint64_t n, uint64_t d, uint64_t k, uint32_t select,
@@ -1905,14 +2161,22 @@ int costal_reweight_device(uint64_t n, uint64_t k, const uint32_t* idx, const do
                            const double* norms, uint32_t weight, double* out, void* stream) {
   if (weight != COSTAL_WEIGHT_TANIMOTO && weight != COSTAL_WEIGHT_COSINE)
     return fail(COSTAL_E_INVALID, "weight %u is not computed on the device", weight);
-  std::lock_guard<std::mutex> lk(g_dev.mu);
-  int rc = device_init(g_dev);
+  Device* Dp = call_device();
+  if (!Dp) return COSTAL_E_DEVICE;
+  Device& D = *Dp;
+  std::lock_guard<std::mutex> lk(D.mu);
+  int rc = device_init(D);
   if (rc) return rc;
+  return reweight_locked(D, n, k, idx, cos, norms, weight, out, pick_stream(D, stream));
+}
+
+// weight is Tanimoto or cosine; D.mu held, device initialised
+static int reweight_locked(Device& D, uint64_t n, uint64_t k, const uint32_t* idx, const double* cos,
+                           const double* norms, uint32_t weight, double* out, hipStream_t s) {
   if (n * k == 0) return 0;
-  hipStream_t s = pick_stream(g_dev, stream);
-  rc = ws_reserve(g_dev, 256);
+  int rc = ws_reserve(D, 256);
   if (rc) return rc;
-  uint32_t* bad = (uint32_t*)g_dev.ws;
+  uint32_t* bad = (uint32_t*)D.ws;
   uint32_t init = NONE, got = NONE;
   HIP_TRY(hipMemcpyAsync(bad, &init, sizeof init, hipMemcpyHostToDevice, s));
   const uint64_t total = n * k;
@@ -1953,34 +2217,35 @@ int costal_reweight(uint64_t n, uint64_t k, const uint32_t* idx, const double* c
   if (weight != COSTAL_WEIGHT_TANIMOTO && weight != COSTAL_WEIGHT_COSINE)
     return fail(COSTAL_E_INVALID, "unknown edge weight %u", weight);
   if (n * k == 0) return 0;
-  int rc;
+  Device* Dp = call_device();
+  if (!Dp) return COSTAL_E_DEVICE;
+  Device& D = *Dp;
+  std::lock_guard<std::mutex> lk(D.mu);
+  int rc = device_init(D);
+  if (rc) return rc;
+  hipStream_t s = D.stream;
   {
-    std::lock_guard<std::mutex> lk(g_dev.mu);
-    rc = device_init(g_dev);
+    Carve c(nullptr);
+    c.take<uint32_t>(n * k);
+    c.take<double>(n * k);
+    c.take<double>(n);
+    c.take<double>(n * k);
+    rc = io_reserve(D, c.off + 256);
     if (rc) return rc;
   }
-  uint32_t* didx = nullptr;
-  double *dcos = nullptr, *dn = nullptr, *dout = nullptr;
-  hipStream_t s = g_dev.stream;
-  rc = 0;
-  if (hipMalloc(&didx, n * k * 4) != hipSuccess || hipMalloc(&dcos, n * k * 8) != hipSuccess ||
-      hipMalloc(&dn, n * 8) != hipSuccess || hipMalloc(&dout, n * k * 8) != hipSuccess) {
-    rc = fail(COSTAL_E_DEVICE, "hipMalloc failed in costal_reweight");
-  } else if (hipMemcpyAsync(didx, idx, n * k * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
-             hipMemcpyAsync(dcos, cos, n * k * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
-             hipMemcpyAsync(dn, norms, n * 8, hipMemcpyHostToDevice, s) != hipSuccess) {
-    rc = fail(COSTAL_E_DEVICE, "copy to device failed in costal_reweight");
-  } else {
-    rc = costal_reweight_device(n, k, didx, dcos, dn, weight, dout, s);
-    if (!rc && (hipMemcpyAsync(out, dout, n * k * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
-                hipStreamSynchronize(s) != hipSuccess))
-      rc = fail(COSTAL_E_DEVICE, "copy to host failed in costal_reweight");
-  }
-  (void)hipFree(didx);
-  (void)hipFree(dcos);
-  (void)hipFree(dn);
-  (void)hipFree(dout);
-  return rc;
+  Carve c(D.io);
+  uint32_t* didx = c.take<uint32_t>(n * k);
+  double* dcos = c.take<double>(n * k);
+  double* dn = c.take<double>(n);
+  double* dout = c.take<double>(n * k);
+  HIP_TRY(hipMemcpyAsync(didx, idx, n * k * 4, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dcos, cos, n * k * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dn, norms, n * 8, hipMemcpyHostToDevice, s));
+  rc = reweight_locked(D, n, k, didx, dcos, dn, weight, dout, s);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(out, dout, n * k * 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return 0;
 }
 
 int costal_fuzzy_union(uint64_t n, uint64_t k, const uint32_t* idx, const double* w, uint32_t* rows,

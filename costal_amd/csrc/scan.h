@@ -37,7 +37,7 @@ typedef void (*CoarseFn)(const uint8_t*, const float*, uint32_t, uint32_t, int, 
                          float*, unsigned long long*);
 typedef void (*CoarseBatchFn)(const BatchSample*, const uint32_t*, uint32_t, uint32_t*, const uint8_t*,
                               const float*, uint32_t, float, uint32_t*, float*, uint32_t*, float*,
-                              unsigned long long*);
+                              unsigned long long*, const unsigned long long*);
 
 // One instantiation unit per (precision form, list capacity): the kernel for J 16-byte row chunks
 // per lane half, or null when the form has none for that J.
@@ -246,7 +246,10 @@ __device__ __forceinline__ void scan_block(
   // Empty list entries hold the lane's admission threshold (-inf without one; +inf for padding
   // queries, which so never take the insert path): the list minimum ls[0] is then always the
   // lane's threshold, and real entries (score above it) are the ones with an index.
-  const float t0 = qvalid ? (tinit ? tinit[qg] : -INFINITY) : INFINITY;
+  float t0 = qvalid ? (tinit ? tinit[qg] : -INFINITY) : INFINITY;
+  // a NaN threshold (caller data or a model with NaN parameters) admits everything: the test reads
+  // the bits, since this code is built without NaN semantics
+  if ((__float_as_uint(t0) & 0x7fffffffu) > 0x7f800000u) t0 = -INFINITY;
   float ls[KPR];
   uint32_t li[KPR];
 #pragma unroll
@@ -495,10 +498,13 @@ __global__ __launch_bounds__(64 * NW) void coarse_batch_kernel(
     const BatchSample* __restrict__ samples, const uint32_t* __restrict__ items, uint32_t nitems,
     uint32_t* __restrict__ queue, const uint8_t* __restrict__ xsq, const float* __restrict__ xs32, uint32_t dp32,
     float eps_pre, uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc, uint32_t* __restrict__ cand_cnt,
-    float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
+    float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts,
+    const unsigned long long* __restrict__ bad) {
   __shared__ __attribute__((aligned(16))) uint8_t tile0[TileShape<J>::TILEB];
   __shared__ __attribute__((aligned(16))) uint8_t tile1[TileShape<J>::TILEB];
   __shared__ uint32_t s_item;
+  // a sample with a NaN or inf row fails the whole call: no score is compared
+  if (*bad != ~0ull) return;
   for (;;) {
     if (tid_x() == 0) s_item = atomicAdd(queue, 1u);
     __syncthreads();

@@ -106,6 +106,15 @@ typedef struct costal_knn_stats {
   double coarse_ms;           /* fused MFMA tile + scan kernel (hipEvent) */
   double rescore_ms;          /* f64 rescoring + certificate kernel (hipEvent) */
   double fallback_ms;         /* full-row kernel (hipEvent) */
+  /* Filled on COSTAL_E_NON_FINITE too: the row of KnnError::NonFinite{row} (knn.rs:102-127), the
+   * first (lowest) row with a NaN or inf, and for a batch its sample; UINT64_MAX otherwise. */
+  uint64_t nonfinite_row;
+  uint64_t nonfinite_sample;
+  /* Measured error of the candidate scores: the largest |list score - f64 cosine| over every
+   * candidate admitted to a list in this call (first pass and rescans), next to the bound the
+   * certificate used for them (knn.rs:164-188).  0 when no candidate list was built (row scan). */
+  double coarse_err_max;
+  double coarse_eps;
 } costal_knn_stats;
 
 /* Exact kNN by params->select from host memory.  norms may be NULL; weights (Tanimoto, n*k) may be
@@ -113,6 +122,13 @@ typedef struct costal_knn_stats {
 int costal_knn_exact(const float* x, uint64_t n, uint64_t d, const costal_knn_params* params,
                      uint32_t* idx, double* cos, double* norms, double* weights,
                      costal_knn_stats* stats);
+
+/* The tile plan costal_knn_exact would use for this shape, without running it: fills path,
+ * row_block, col_block, candidates, col_splits, blocks_total and coarse_eps of stats (the device's
+ * CU count when a device is present, else 256; the first pass may still take the balanced launch,
+ * which reports 2 splits).  Replaces block_sizes (knn.rs:551-569) as the place a caller reads the
+ * blocking from; needs no GPU. */
+int costal_knn_plan(uint64_t n, uint64_t d, const costal_knn_params* params, costal_knn_stats* stats);
 
 /* The same from device memory; every pointer is a device pointer. */
 int costal_knn_exact_device(const float* x, uint64_t n, uint64_t d,
@@ -180,6 +196,14 @@ int costal_mlp_grad_device(const float* x, const float* target, uint64_t n, uint
 int costal_mlp_adam_device(uint64_t count, float* params, const float* grad, float* m, float* v,
                            float lr, float beta1, float beta2, float eps, uint64_t step, void* stream);
 const char* costal_mlp_last_error(void);
+
+/* Self-check of the matrix core (no counterpart in the reference): eight MFMA results into out
+ * (host memory, 8 floats) that show whether FP32 accumulation rounds to nearest and whether FP16
+ * subnormal inputs are kept, the two properties the split-FP16 candidate bound relies on.
+ * Expected on round-to-nearest hardware that keeps subnormals:
+ *   out[0] = 1 + 2^-23, out[1] = 1 + 2^-23 (sum rounded once) or 1 (rounded per addition),
+ *   out[2] = 1 - 2^-24, out[3] = out[4] = out[7] = 2^-24, out[5] = 1 + 2^-23, out[6] = 1 - 2^-24. */
+int costal_mfma_probe(float* out);
 
 /* Thread-local message of the last failing call on this thread. */
 const char* costal_last_error(void);

@@ -187,3 +187,88 @@ def test_batch_plan_drops_splits_when_samples_fill_the_gpu(tmp_path):
     assert (prec, margins, splits) == ("fp16x3", [7], [4])
     _, _, splits = tn.plan_batch([(300, 32), (200000, 40), (100000, 48)], 15)  # 2,348 query blocks
     assert splits == [1, 1, 1]
+
+
+def test_ctypes_mirrors_match_the_header_layout(tmp_path):
+    """The ctypes structures of the Python mirror have the C structures' sizes and field offsets
+    (a C program built against include/costal_hip.h prints them)."""
+    import ctypes as C
+    import subprocess
+
+    structs = {"costal_knn_params": _lib.KnnParams, "costal_knn_stats": _lib.KnnStats,
+               "costal_sample": _lib.Sample}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "costal_hip.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(line.split()[:2]): int(line.split()[2]) for line in out if line}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def _plan(n, d, k, **kw):
+    import ctypes as C
+
+    p = _lib.KnnParams()
+    p.k = k
+    p.margin = kw.get("margin", -1)
+    p.precision = _lib.PRECISION[kw.get("precision", "fp16x3")]
+    p.select = _lib.SELECT[kw.get("select", "cosine")]
+    p.col_splits = kw.get("col_splits", 0)
+    st = _lib.KnnStats()
+    _lib.check(_lib.load().costal_knn_plan(n, d, C.byref(p), C.byref(st)))
+    return st
+
+
+@pytest.mark.parametrize("k,splits", [(10, 128), (10, 512), (30, 512), (64, 100)])
+def test_plan_caps_caller_splits_at_the_rescoring_capacity(k, splits):
+    """A caller's col_splits is speed only: the plan never hands the rescoring more than 2048 list
+    entries per row (2 lists per split, K' entries each), whatever was asked for."""
+    st = _plan(200_000, 15, k, col_splits=splits)
+    assert st.path == 0
+    assert 1 <= st.col_splits <= splits
+    assert st.col_splits * 2 * st.candidates <= 2048
+
+
+def test_plan_paths_and_bounds():
+    st = _plan(396_460, 15, 10)
+    assert (st.path, st.candidates, st.row_block) == (0, 10, 128)
+    eps16x3 = st.coarse_eps
+    st32 = _plan(396_460, 15, 10, precision="fp32")
+    assert st32.coarse_eps == (15 + 4) * 2.0 ** -23  # the reference's bound, knn.rs:185-187
+    assert 0 < eps16x3 < 2 * st32.coarse_eps
+    assert _plan(3605, 2000, 10, precision="fp32").path == 2  # wide rows
+    assert _plan(5000, 15, 10, select="tanimoto").path == 1  # exact row scan
+
+
+def test_plan_rejects_what_the_reference_rejects():
+    with pytest.raises(ValueError, match=re.escape("invalid k=0: need 1 <= k <= n-1 (n=50)")):
+        _plan(50, 4, 0)
+
+
+def test_diagnose_merges_blocks_exactly():
+    """The block-merged column moments equal the direct ones: a matrix centred as a whole but not
+    within any one block is still reported centred, and the fractions count every entry."""
+    n, d = 200_000, 12  # about 2.3 blocks of rows
+    rng = np.random.default_rng(21)
+    x = rng.normal(0, 1, size=(n, d)).astype(np.float32)
+    x[: n // 2] += 3.0
+    x[n // 2:] -= 3.0
+    x -= x.astype(np.float64).mean(axis=0).astype(np.float32)
+    norms = np.linalg.norm(x.astype(np.float64), axis=1)
+    got = _api._diagnose(x, norms)
+    assert got["columns_centered"]
+    assert got["negative_fraction"] == np.count_nonzero(x < 0) / x.size
+    assert got["nonzero_fraction"] == np.count_nonzero(x) / x.size
+    assert np.isclose(got["row_norm_cv"], norms.std() / norms.mean(), rtol=1e-12)
+    y = x + np.float32(0.2)  # |mean| = 0.2 > 0.05 std (std ~ 3.2)
+    assert not _api._diagnose(y, np.linalg.norm(y.astype(np.float64), axis=1))["columns_centered"]

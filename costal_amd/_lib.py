@@ -40,6 +40,7 @@ EXPORTS = (
     "costal_knn_batched_device",
     "costal_knn_plan",
     "costal_mfma_probe",
+    "costal_knn_exact_device_noalloc",
 )
 
 
@@ -136,6 +137,8 @@ def load():
     L.costal_knn_exact_device.argtypes = [P, U64, U64, C.POINTER(KnnParams), P, P, P, P,
                                           C.POINTER(KnnStats), P]
     L.costal_knn_bruteforce.argtypes = [P, U64, U64, U64, U32, P, P, P]
+    L.costal_knn_exact_device_noalloc.argtypes = [P, U64, U64, C.POINTER(KnnParams), P, P, P, P,
+                                                  C.POINTER(KnnStats), P]
     L.costal_mfma_probe.argtypes = [P]
     L.costal_knn_plan.argtypes = [U64, U64, C.POINTER(KnnParams), C.POINTER(KnnStats)]
     L.costal_knn_batched_device.argtypes = [C.POINTER(Sample), U32, C.POINTER(KnnParams), C.POINTER(KnnStats), P]

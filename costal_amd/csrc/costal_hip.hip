@@ -50,6 +50,12 @@
 
 #define COSTAL_VERSION "costal-mi355x 0.2.0 (gfx950)"
 
+// The workspace of costal_knn_exact_device_noalloc: working copies, candidate lists, counters and
+// thresholds of one call, COSTAL_SCRATCH_BYTES per device, held by every GPU the library runs on
+// until the process ends.  Host code never uses this name as an address: device_init looks the
+// device's copy up with hipGetSymbolAddress and keeps that pointer per device.
+static __device__ __align__(256) uint8_t scratch[COSTAL_SCRATCH_BYTES];
+
 
 namespace {
 
@@ -1074,6 +1080,10 @@ struct Device {
   // device copies of the host entry points' arguments (grow-only, kept between calls)
   void* io = nullptr;
   size_t io_cap = 0;
+  // this device's copy of the static scratch (hipGetSymbolAddress at set-up), or null with the
+  // lookup's error
+  void* scratch = nullptr;
+  hipError_t scratch_err = hipSuccess;
   // call start/end and per-kernel brackets: prep, coarse, rescore, fallback
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evk[8] = {};
@@ -1112,6 +1122,12 @@ int device_init(Device& D) {
   HIP_TRY(hipEventCreate(&D.ev0));
   HIP_TRY(hipEventCreate(&D.ev1));
   for (auto& e : D.evk) HIP_TRY(hipEventCreate(&e));
+  // the device's own address of the static scratch (the device is current here)
+  D.scratch_err = hipGetSymbolAddress(&D.scratch, HIP_SYMBOL(scratch));
+  if (D.scratch_err != hipSuccess) {
+    D.scratch = nullptr;
+    (void)hipGetLastError();
+  }
   D.init = true;
   return 0;
 }
@@ -1437,7 +1453,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   if (fixed) {
     if (bytes > fixed->cap)
       return fail(COSTAL_E_UNSUPPORTED,
-                  "n=%u, d=%u, k=%u needs %zu bytes of device workspace; the fixed workspace holds %zu",
+                  "n=%u, d=%u, k=%u needs %zu bytes of device scratch; the library's static scratch holds %zu",
                   n, d, k, bytes, fixed->cap);
   } else if ((rc = ws_reserve(D, bytes))) {
     return rc;
@@ -2066,6 +2082,25 @@ int costal_mfma_probe(float* out) {
   HIP_TRY(hipMemcpyAsync(out, dout, 8 * sizeof(float), hipMemcpyDeviceToHost, D.stream));
   HIP_TRY(hipStreamSynchronize(D.stream));
   return 0;
+}
+
+int costal_knn_exact_device_noalloc(const float* x, uint64_t n, uint64_t d, const costal_knn_params* params,
+                                    uint32_t* idx, double* cos, double* norms, double* weights,
+                                    costal_knn_stats* stats, void* stream) {
+  if (!params || !x || !idx || !cos) return fail(COSTAL_E_INVALID, "null pointer argument");
+  int rc = check_args(n, d, params->k);
+  if (rc) return rc;
+  Device* Dp = call_device();
+  if (!Dp) return COSTAL_E_DEVICE;
+  Device& D = *Dp;
+  std::lock_guard<std::mutex> lk(D.mu);
+  rc = device_init(D);
+  if (rc) return rc;
+  if (!D.scratch)
+    return fail(COSTAL_E_DEVICE, "the static scratch of device %d is not addressable: %s", D.id,
+                hipGetErrorString(D.scratch_err));
+  const FixedWs ws{D.scratch, (size_t)COSTAL_SCRATCH_BYTES};
+  return knn_device_locked(D, x, n, d, *params, idx, cos, norms, weights, stats, pick_stream(D, stream), false, &ws);
 }
 
 static int knn_host(const float* x, uint64_t n, uint64_t d, const costal_knn_params* params, uint32_t* idx,

@@ -135,6 +135,22 @@ int costal_knn_exact_device(const float* x, uint64_t n, uint64_t d,
                             const costal_knn_params* params, uint32_t* idx, double* cos,
                             double* norms, double* weights, costal_knn_stats* stats, void* stream);
 
+/* ---- the same without allocating -------------------------------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).  Arguments, results and
+ * errors are those of costal_knn_exact_device, but every intermediate buffer (working copies,
+ * candidate lists, counters, thresholds) lives in a static device array of COSTAL_SCRATCH_BYTES
+ * per GPU, held from the library's set-up until the process ends; nothing is allocated or freed.
+ * A call whose buffers do not fit, even with one candidate range per row, fails with
+ * COSTAL_E_UNSUPPORTED and a message giving the bytes it needs.  Calls on one GPU share that
+ * GPU's scratch: the library serializes them and each returns only when its work is complete, so
+ * two calls never use it at once (two copies of the library in one process would race). */
+#define COSTAL_SCRATCH_BYTES (64u << 20)
+int costal_knn_exact_device_noalloc(const float* x, uint64_t n, uint64_t d,
+                                    const costal_knn_params* params, uint32_t* idx, double* cos,
+                                    double* norms, double* weights, costal_knn_stats* stats,
+                                    void* stream);
+
 /* ---- batched samples -----------------------------------------------------------------------
  * No single counterpart in the reference (it runs one knn() per sample, knn.rs:360-379): many
  * independent samples of uneven size in one pass, each with exactly the result costal_knn_exact

@@ -236,3 +236,61 @@ def _fuzzy_union(idx, w):
 
 def core_version() -> str:
     return _lib.version()
+
+
+class Csr:
+    """The undirected weighted graph the CPU Leiden consumes, in compressed rows
+    (ike12th/CosTaL crates/costal-core/src/leiden.rs:25-100): ``indptr`` uint64[n+1], ``nbr``
+    uint32 and ``w`` float64 holding both directions of every edge with a positive weight."""
+
+    def __init__(self, indptr, nbr, w):
+        self.indptr, self.nbr, self.w = indptr, nbr, w
+
+    @property
+    def n(self) -> int:
+        return len(self.indptr) - 1
+
+    def n_edges(self) -> int:
+        return len(self.nbr) // 2
+
+    def strengths(self) -> np.ndarray:
+        """k_i = the sum of row i's weights, in row order."""
+        out = np.zeros(self.n)
+        for v in range(self.n):
+            s = 0.0
+            for x in self.w[int(self.indptr[v]):int(self.indptr[v + 1])]:
+                s += float(x)
+            out[v] = s
+        return out
+
+    @classmethod
+    def from_edges(cls, n, rows, cols, w) -> "Csr":
+        """Csr::from_edges (leiden.rs:52-100): each undirected edge once (such as _fuzzy_union's
+        output), rows listing neighbors in edge order, exact-zero weights dropped; ValueError with
+        the reference's message for mismatched lengths, out-of-range nodes, self-loops and negative
+        or non-finite weights."""
+        rows = np.asarray(rows)
+        cols = np.asarray(cols)
+        w = np.asarray(w, dtype=np.float64)
+        if not (len(rows) == len(cols) == len(w)):
+            raise ValueError(f"edge arrays differ in length: {len(rows)} rows, {len(cols)} cols, {len(w)} weights")
+        n = int(n)
+        m = len(rows)
+        r = np.ascontiguousarray(rows, dtype=np.int64)
+        c = np.ascontiguousarray(cols, dtype=np.int64)
+        # indices that do not fit u32 are out of range whatever n is (n < 2^32 is checked below)
+        big = np.flatnonzero((r < 0) | (c < 0) | (r > 0xFFFFFFFF) | (c > 0xFFFFFFFF))
+        if big.size and n <= 0xFFFFFFFF:
+            e = int(big[0])
+            raise ValueError(f"edge {e} ({int(r[e])}, {int(c[e])}) is out of range (n={n})")
+        r32 = np.ascontiguousarray(r, dtype=np.uint32)
+        c32 = np.ascontiguousarray(c, dtype=np.uint32)
+        w = np.ascontiguousarray(w)
+        indptr = np.empty(n + 1, np.uint64)
+        nbr = np.empty(max(2 * m, 1), np.uint32)
+        wo = np.empty(max(2 * m, 1), np.float64)
+        nnz = C.c_uint64(0)
+        _call(_lib.load().costal_csr_from_edges, n, m, _ptr(r32), _ptr(c32), _ptr(w), _ptr(indptr), _ptr(nbr),
+              _ptr(wo), C.byref(nnz))
+        k = nnz.value
+        return cls(indptr, nbr[:k].copy(), wo[:k].copy())

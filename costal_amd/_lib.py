@@ -39,6 +39,7 @@ EXPORTS = (
     "costal_mlp_last_error",
     "costal_knn_batched_device",
     "costal_knn_plan",
+    "costal_csr_from_edges",
     "costal_mfma_probe",
     "costal_knn_exact_device_noalloc",
 )
@@ -140,6 +141,7 @@ def load():
     L.costal_knn_exact_device_noalloc.argtypes = [P, U64, U64, C.POINTER(KnnParams), P, P, P, P,
                                                   C.POINTER(KnnStats), P]
     L.costal_mfma_probe.argtypes = [P]
+    L.costal_csr_from_edges.argtypes = [U64, U64, P, P, P, P, P, P, C.POINTER(C.c_uint64)]
     L.costal_knn_plan.argtypes = [U64, U64, C.POINTER(KnnParams), C.POINTER(KnnStats)]
     L.costal_knn_batched_device.argtypes = [C.POINTER(Sample), U32, C.POINTER(KnnParams), C.POINTER(KnnStats), P]
     L.costal_reweight.argtypes = [U64, U64, P, P, P, U32, P]

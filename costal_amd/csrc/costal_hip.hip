@@ -1684,6 +1684,18 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
 
 hipStream_t pick_stream(Device& D, void* stream) { return stream ? (hipStream_t)stream : D.stream; }
 
+// the shortest decimal that reads back as x (NaN, inf and -inf spelled as Rust prints them)
+std::string shortest(double x) {
+  if (std::isnan(x)) return "NaN";
+  if (std::isinf(x)) return x > 0 ? "inf" : "-inf";
+  char buf[64];
+  for (int p = 1; p <= 17; ++p) {
+    snprintf(buf, sizeof buf, "%.*g", p, x);
+    if (strtod(buf, nullptr) == x) break;
+  }
+  return buf;
+}
+
 // ------------------------------------------------------------------------------------------------
 // Batched samples (host side)
 
@@ -2280,6 +2292,45 @@ int costal_reweight(uint64_t n, uint64_t k, const uint32_t* idx, const double* c
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(out, dout, n * k * 8, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  return 0;
+}
+
+int costal_csr_from_edges(uint64_t n, uint64_t m, const uint32_t* rows, const uint32_t* cols, const double* w,
+                          uint64_t* indptr, uint32_t* nbr, double* wout, uint64_t* nnz) {
+  // leiden.rs:52-100: validate in edge order (range, self-loop, weight), count the kept entries
+  // of both endpoints, prefix-sum the counts into row starts, then place each kept edge in both
+  // rows in edge order.
+  if (!indptr || !nnz || (m && (!rows || !cols || !w || !nbr || !wout)))
+    return fail(COSTAL_E_INVALID, "null pointer argument");
+  if (n > 0xffffffffull) return fail(COSTAL_E_TOO_MANY_ROWS, "%llu nodes exceed the u32 node index", (unsigned long long)n);
+  std::vector<uint64_t> deg(n + 1, 0);
+  for (uint64_t e = 0; e < m; ++e) {
+    const uint64_t a = rows[e], b = cols[e];
+    if (a >= n || b >= n)
+      return fail(COSTAL_E_INVALID, "edge %llu (%llu, %llu) is out of range (n=%llu)", (unsigned long long)e,
+                  (unsigned long long)a, (unsigned long long)b, (unsigned long long)n);
+    if (a == b) return fail(COSTAL_E_INVALID, "edge %llu is a self-loop on node %llu", (unsigned long long)e,
+                            (unsigned long long)a);
+    const double x = w[e];
+    if (!std::isfinite(x) || x < 0.0)
+      return fail(COSTAL_E_INVALID, "edge %llu has weight %s; weights must be finite and non-negative",
+                  (unsigned long long)e, shortest(x).c_str());
+    if (x != 0.0) {
+      ++deg[a + 1];
+      ++deg[b + 1];
+    }
+  }
+  for (uint64_t v = 1; v <= n; ++v) deg[v] += deg[v - 1];
+  std::copy(deg.begin(), deg.end(), indptr);
+  for (uint64_t e = 0; e < m; ++e) {
+    if (w[e] == 0.0) continue;
+    const uint64_t pa = deg[rows[e]]++, pb = deg[cols[e]]++;
+    nbr[pa] = cols[e];
+    wout[pa] = w[e];
+    nbr[pb] = rows[e];
+    wout[pb] = w[e];
+  }
+  *nnz = indptr[n];
   return 0;
 }
 

@@ -12,6 +12,7 @@
  *                           crates/costal-py/src/lib.rs:194-214     costal._costal._reweight
  *   costal_fuzzy_union      crates/costal-core/src/graph.rs:24-68  graph::fuzzy_union
  *                           crates/costal-py/src/lib.rs:218-239     costal._costal._fuzzy_union
+ *   costal_csr_from_edges   crates/costal-core/src/leiden.rs:52-100 Csr::from_edges
  *
  * Conventions
  *   - Plain pointers and sizes only.  The *_device variants take device pointers and an optional
@@ -189,6 +190,16 @@ int costal_reweight_device(uint64_t n, uint64_t k, const uint32_t* idx, const do
  * w must hold n*k entries; the edge count is written to *m. */
 int costal_fuzzy_union(uint64_t n, uint64_t k, const uint32_t* idx, const double* w,
                        uint32_t* rows, uint32_t* cols, double* wout, uint64_t* m);
+
+/* The compressed rows the CPU Leiden consumes (leiden.rs:52-100, Csr::from_edges) from an edge list
+ * holding each undirected edge once (costal_fuzzy_union's output): both directions of every edge
+ * with a positive weight, rows listing neighbors in edge order (ascending for a sorted
+ * upper-triangular list); exact-zero weights dropped.  indptr holds n+1 entries, nbr and wout up
+ * to 2m; the entry count is written to *nnz.  Edges out of range, self-loops and negative or
+ * non-finite weights fail with the reference's message for the first such edge. */
+int costal_csr_from_edges(uint64_t n, uint64_t m, const uint32_t* rows, const uint32_t* cols,
+                          const double* w, uint64_t* indptr, uint32_t* nbr, double* wout,
+                          uint64_t* nnz);
 
 /* ---- learned admission threshold ----------------------------------------------------------
  * No counterpart in the reference: the per-row threshold model of the fused scan.  A 3-layer MLP

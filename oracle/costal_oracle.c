@@ -16,6 +16,7 @@
  *   oracle_tani_from_cos      crates/costal-core/src/weight.rs:28-37
  *   oracle_reweight           crates/costal-core/src/weight.rs:41-93 (tanimoto, cosine, jaccard)
  *   oracle_fuzzy_union        crates/costal-core/src/graph.rs:24-68
+ *   oracle_csr_from_edges     crates/costal-core/src/leiden.rs:52-100
  *   oracle_heap_keep          crates/costal-core/src/knn.rs:791-875 (BlockHeaps, for its own test)
  *
  * How each is pinned (tests/test_oracle.py):
@@ -561,5 +562,60 @@ int oracle_fuzzy_union(uint64_t n, uint64_t k, const uint32_t* idx, const double
   }
   free(e);
   *m = out;
+  return 0;
+}
+
+/* ------------------------------------------------------------------ CSR of the graph */
+
+/* leiden.rs:52-100 (Csr::from_edges): an undirected edge list (each edge once) to compressed rows
+ * holding both directions; exact-zero weights dropped; rows list neighbors in edge order.  The
+ * first offending edge (in edge order, tested as the reference tests it: range, self-loop, weight)
+ * fails the call.  indptr has n+1 entries; nbr and wout hold up to 2m entries. */
+static void fmt_weight(char* buf, size_t len, double x) {
+  if (x != x) { snprintf(buf, len, "NaN"); return; }
+  if (x == 1.0 / 0.0) { snprintf(buf, len, "inf"); return; }
+  if (x == -1.0 / 0.0) { snprintf(buf, len, "-inf"); return; }
+  for (int p = 1; p <= 17; ++p) {
+    snprintf(buf, len, "%.*g", p, x);
+    if (strtod(buf, NULL) == x) return;
+  }
+}
+
+int oracle_csr_from_edges(uint64_t n, uint64_t m, const uint32_t* rows, const uint32_t* cols, const double* w,
+                          uint64_t* indptr, uint32_t* nbr, double* wout) {
+  if (n > 0xffffffffull) return fail(ORACLE_E_INVALID, "%llu nodes exceed the u32 node index", (unsigned long long)n);
+  for (uint64_t v = 0; v <= n; ++v) indptr[v] = 0;
+  for (uint64_t e = 0; e < m; ++e) {
+    const uint64_t i = rows[e], j = cols[e];
+    const double x = w[e];
+    if (i >= n || j >= n)
+      return fail(ORACLE_E_INVALID, "edge %llu (%llu, %llu) is out of range (n=%llu)", (unsigned long long)e,
+                  (unsigned long long)i, (unsigned long long)j, (unsigned long long)n);
+    if (i == j) return fail(ORACLE_E_INVALID, "edge %llu is a self-loop on node %llu", (unsigned long long)e,
+                            (unsigned long long)i);
+    if (!(x >= 0.0 && x < 1.0 / 0.0)) {
+      char b[64];
+      fmt_weight(b, sizeof b, x);
+      return fail(ORACLE_E_INVALID, "edge %llu has weight %s; weights must be finite and non-negative",
+                  (unsigned long long)e, b);
+    }
+    if (x > 0.0) {
+      ++indptr[i + 1];
+      ++indptr[j + 1];
+    }
+  }
+  for (uint64_t v = 0; v < n; ++v) indptr[v + 1] += indptr[v];
+  uint64_t* fill = (uint64_t*)malloc(sizeof(uint64_t) * (n + 1));
+  if (!fill) return fail(ORACLE_E_OOM, "out of memory");
+  for (uint64_t v = 0; v < n; ++v) fill[v] = indptr[v];
+  for (uint64_t e = 0; e < m; ++e) {
+    if (!(w[e] > 0.0)) continue;
+    const uint64_t i = rows[e], j = cols[e];
+    nbr[fill[i]] = (uint32_t)j;
+    wout[fill[i]++] = w[e];
+    nbr[fill[j]] = (uint32_t)i;
+    wout[fill[j]++] = w[e];
+  }
+  free(fill);
   return 0;
 }

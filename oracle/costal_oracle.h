@@ -40,6 +40,9 @@ int oracle_reweight(uint64_t n, uint64_t k, const uint32_t* idx, const double* c
 int oracle_fuzzy_union(uint64_t n, uint64_t k, const uint32_t* idx, const double* w,
                        uint32_t* rows, uint32_t* cols, double* wout, uint64_t* m);
 
+int oracle_csr_from_edges(uint64_t n, uint64_t m, const uint32_t* rows, const uint32_t* cols,
+                          const double* w, uint64_t* indptr, uint32_t* nbr, double* wout);
+
 /* oracle_gemm.c */
 void oracle_tile_nt(const float* a, uint64_t m, const float* b, uint64_t nb, uint64_t d, float* out);
 void oracle_stable_sort_by_key(uint32_t* perm, uint64_t n, const double* key);

@@ -58,6 +58,7 @@ def lib():
         L.oracle_heap_keep.argtypes = [_P, _P, _U64, _U64, _P]
         L.oracle_reweight.argtypes = [_U64, _U64, _P, _P, _P, C.c_int, _P]
         L.oracle_fuzzy_union.argtypes = [_U64, _U64, _P, _P, _P, _P, _P, _P]
+        L.oracle_csr_from_edges.argtypes = [_U64, _U64, _P, _P, _P, _P, _P, _P]
         _lib = L
     return _lib
 
@@ -159,6 +160,20 @@ def fuzzy_union(idx, w):
                                     C.byref(m)))
     m = m.value
     return rows[:m].astype(np.int32), cols[:m].astype(np.int32), wo[:m].copy()
+
+
+def csr_from_edges(n: int, rows, cols, w):
+    """leiden.rs:52-100: (indptr uint64[n+1], nbr uint32[nnz], w float64[nnz])."""
+    rows = np.ascontiguousarray(rows, dtype=np.uint32)
+    cols = np.ascontiguousarray(cols, dtype=np.uint32)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    m = len(rows)
+    indptr = np.empty(n + 1, np.uint64)
+    nbr = np.empty(max(2 * m, 1), np.uint32)
+    wo = np.empty(max(2 * m, 1), np.float64)
+    _check(lib().oracle_csr_from_edges(n, m, _ptr(rows), _ptr(cols), _ptr(w), _ptr(indptr), _ptr(nbr), _ptr(wo)))
+    nnz = int(indptr[n])
+    return indptr, nbr[:nnz].copy(), wo[:nnz].copy()
 
 
 def dot64(a, b) -> float:

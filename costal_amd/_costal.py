@@ -109,10 +109,10 @@ def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None
     k = int(k)
     if k < 0:
         raise OverflowError("can't convert negative int to unsigned")
-    # margin=None: the library's default list slack (k + max(2, ceil(k/4)) candidates per list)
+    m = k if margin is None else int(margin)
     p = _lib.KnnParams()
     p.k = k
-    p.margin = -1 if margin is None else int(margin)
+    p.margin = m
     p.select = _lib.SELECT[sel]
     p.precision = _lib.PRECISION[prec]
     p.row_block = int(row_block or 0)
@@ -145,7 +145,7 @@ def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None
         "row_block": int(st.row_block),
         "col_block": int(st.col_block),
         "matmul": backend,
-        "margin": int(st.candidates) - k,
+        "margin": m,
         "select": sel,
         "blocks_skipped": int(st.blocks_skipped),
         "blocks_total": int(st.blocks_total),

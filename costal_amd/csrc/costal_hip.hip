@@ -890,7 +890,7 @@ __global__ __launch_bounds__(64) void rescore_batch_kernel(
   const uint32_t sidx = find_sample(row_first, nsamples, g);
   const BatchSample& S = samples[sidx];
   const uint32_t i = g - S.row_base;
-  const uint32_t nlists = S.nsplit;
+  const uint32_t nlists = 2 * S.nsplit;
   const int all_admitted = (S.nsplit == 1 && (uint32_t)S.kp >= S.n - 1) ? 1 : 0;
   rescore_row<CMAX>(s_sc, s_id, S.x, S.d, S.norms, k, S.kp, nlists, i, i, cand_idx + S.entry_base,
                     cand_sc + S.entry_base, cand_cnt + S.slot_base, cand_floor + S.slot_base, all_admitted, eps_list,
@@ -915,7 +915,7 @@ __global__ __launch_bounds__(64) void rescore_batch_pair_kernel(
   const BatchSample& S = samples[valid ? find_sample(row_first, nsamples, g) : 0];
   const uint32_t i = valid ? g - S.row_base : 0;
   const int all_admitted = (S.nsplit == 1 && (uint32_t)S.kp >= S.n - 1) ? 1 : 0;
-  rescore_half(s_sc[half], s_id[half], &s_fill[half], valid, lane, S.x, S.d, S.norms, k, S.kp, S.nsplit, i, i,
+  rescore_half(s_sc[half], s_id[half], &s_fill[half], valid, lane, S.x, S.d, S.norms, k, S.kp, 2 * S.nsplit, i, i,
                cand_idx + S.entry_base, cand_sc + S.entry_base, cand_cnt + S.slot_base, cand_floor + S.slot_base,
                all_admitted, eps_list, eps32, S.idx, S.cos, S.wts, g, fb_rows, fb_lo, fb_count, short_count, err_max);
 }
@@ -1253,45 +1253,33 @@ double fp16x3_eps(uint32_t d) {
   return (a + e) * (2.0 + a + e) + (u16 + e) * (u16 + e) + gam * (1.0 + u16 + e) * (1.0 + u16 + e);
 }
 
-int list_template(uint32_t kp) { return kp <= 16 ? 16 : kp <= 32 ? 32 : 64; }
-
-// Capacity K' of a query's candidate list (one per query and candidate split): k + margin (the
-// reference's heap of K' = k + margin, knn.rs:53), at most n - 1.  A negative margin picks the
-// default, k + max(2, ceil(k/4)): the certificate needs the list floor (its K'-th score) below the
-// k-th score by the error bound, and a few entries of slack leave that gap on nearly every row.
-// For narrow rows (`narrow`: the scan instance that runs three waves per SIMD with lists of 16)
-// the default stays within 16 entries when k + 1 fits, since longer lists cost that occupancy;
-// and for k up to 63 it stays within the fused scan's 64 entries (one entry of slack at k = 63).
-// The capacity only trades inserts against fallbacks; the certificate makes every choice exact.
-uint32_t list_capacity(uint64_t n, uint64_t k, int64_t margin, bool narrow = false) {
-  uint64_t kp;
-  if (margin < 0) {
-    kp = k + std::max<uint64_t>(2, (k + 3) / 4);
-    if (narrow && k + 1 <= 16) kp = std::min<uint64_t>(kp, 16);
-    if (kp > FUSED_KP_MAX) kp = std::max<uint64_t>(FUSED_KP_MAX, k + 1);  // k up to 63 stays fused
-  } else {
-    kp = k + (uint64_t)margin;
-  }
-  return (uint32_t)std::min<uint64_t>(kp, n - 1);
+// Capacity of one candidate list.  A query's candidates are split over two lists (the two lanes of
+// its lane pair, each seeing half of every sub-tile) and over the candidate splits; with K' = k +
+// margin candidates per row (the reference's heap size), each list keeps max(k, ceil(K' / 2)): the
+// two lists of a row together hold at least K', and all of its k nearest fit in either one (the
+// default margin k gives lists of k).  The capacity only trades inserts against fallbacks; the
+// certificate makes every choice exact.
+uint32_t list_capacity(uint64_t n, uint64_t k, uint64_t margin) {
+  return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(k, (k + margin + 1) / 2), n - 1);
 }
+
+int list_template(uint32_t kp) { return kp <= 16 ? 16 : kp <= 32 ? 32 : 64; }
 
 Plan make_plan(const Device& D, uint64_t n, uint64_t d, uint64_t k, const costal_knn_params& p) {
   Plan P{};
+  const uint64_t margin = p.margin < 0 ? k : (uint64_t)p.margin;
+  P.kp = list_capacity(n, k, margin);
   P.prec = (int)p.precision;
   P.dp32 = round_up((uint32_t)std::max<uint64_t>(d, 1), 8);
   P.dp16 = round_up((uint32_t)std::max<uint64_t>(d, 1), 16);
-  {
-    const uint32_t j = P.prec == PREC_F16 ? P.dp16 / 16 : P.prec == PREC_F16X3 ? P.dp16 / 8 : P.dp32 / 8;
-    P.kp = list_capacity(n, k, p.margin, j <= 2);
-  }
   const uint32_t dmax = P.prec == PREC_F16 ? 128 : 64;
   P.fused = p.select == COSTAL_SELECT_COSINE && d <= dmax && P.kp <= FUSED_KP_MAX;
   P.J = P.prec == PREC_F16 ? P.dp16 / 16 : P.prec == PREC_F16X3 ? P.dp16 / 8 : P.dp32 / 8;
   P.KP = list_template(P.kp);
   P.NW = 4;
-  P.QB = 64 * P.NW;
-  P.TC = P.J <= 1 ? 768 : P.J <= 2 ? 384 : P.J <= 3 ? 256 : P.J <= 4 ? 192 : 128;  // TileShape<J>::TC
-  P.n_pad = round_up((uint32_t)n, PAD_ROWS);  // whole tiles of every kind can be read
+  P.QB = 32 * P.NW;
+  P.TC = P.J <= 2 ? 512 : P.J <= 4 ? 256 : 128;  // the kernel's 32 KB tiles
+  P.n_pad = round_up((uint32_t)n, 512);  // whole tiles of every kind can be read
   const uint32_t ntile = (uint32_t)((n + P.TC - 1) / P.TC);
   const uint32_t nblk = (uint32_t)((n + P.QB - 1) / P.QB);
   uint32_t ns = p.col_splits;
@@ -1302,7 +1290,7 @@ Plan make_plan(const Device& D, uint64_t n, uint64_t d, uint64_t k, const costal
   }
   ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, std::max<uint32_t>(1, ntile / 2)));
   // the rescoring gathers 2 * splits * K' list entries per row into at most 2048 LDS slots
-  ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, RESCORE_CMAX / P.kp));
+  ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, RESCORE_CMAX / (2 * P.kp)));
   const uint32_t tiles_per = (ntile + ns - 1) / ns;
   P.cols_per_split = tiles_per * P.TC;
   P.nsplit = (ntile + tiles_per - 1) / tiles_per;
@@ -1390,14 +1378,15 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   if (p.select > COSTAL_SELECT_TANIMOTO) return fail(COSTAL_E_INVALID, "unknown select %u", p.select);
   Plan P = make_plan(D, n, d, k, p);
   if (brute) P.fused = P.wide = false;
-  // Balanced first pass (tuned splits only): one workgroup per resident slot, the slots taking
-  // equal shares of the (query block, tile) units instead of whole blocks, so no slot idles while
-  // the last blocks finish (with 256 queries per block, 396,460 rows are 1,549 blocks: a direct
-  // grid of them on 768 slots would run three rounds for 2.02 rounds of work).  A block is cut in
-  // at most ceil(tiles / share) + 1 pieces, each with its own list per query; a share of at least
-  // a quarter of the block keeps that at five.
-  uint32_t units_per_wg = 0, grid_bal = 0, bal_lists = 0;
-  if (P.fused && p.col_splits == 0) {
+  // Balanced first pass (tuned splits only): with at least one query block per resident workgroup
+  // slot, the slots take equal shares of the (query block, tile) units instead of whole blocks,
+  // so no slot idles while the last blocks finish; a block then has at most two pieces.  A second
+  // piece starts with empty lists (more inserts) and the workgroups no longer stream the same tiles
+  // together, so this pays where the scan itself is long against its inserts: the FP32 form, and
+  // split FP16 with long lists (measured on 86,864 x 39 and 396,460 x 15; FP16 and split FP16 with
+  // k = 10 lists were slower).
+  uint32_t units_per_wg = 0, grid_bal = 0;
+  if (P.fused && p.col_splits == 0 && (P.prec == PREC_F32 || (P.prec == PREC_F16X3 && P.KP >= 32))) {
     CoarseFn f0 = coarse_select(P.prec, P.J, P.KP);
     int per_cu = 0;
     if (f0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(f0),
@@ -1405,13 +1394,11 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
       const uint64_t slots = (uint64_t)per_cu * (uint64_t)std::max(D.cus, 1);
       const uint64_t nblk = ((uint64_t)n + P.QB - 1) / P.QB, nt = ((uint64_t)n + P.TC - 1) / P.TC;
       const uint64_t units = nblk * nt;
-      const uint64_t per = std::max<uint64_t>((units + slots - 1) / slots, (nt + 3) / 4);
-      const uint64_t lists = (nt + per - 1) / per + 1;
-      if (units > slots && per <= 0xffffffffull && lists * P.kp <= RESCORE_CMAX) {
+      const uint64_t per = std::max<uint64_t>((units + slots - 1) / slots, nt);
+      if (nblk >= slots && per <= 0xffffffffull) {
         units_per_wg = (uint32_t)per;
         grid_bal = (uint32_t)((units + per - 1) / per);
-        bal_lists = (uint32_t)lists;
-        P.nsplit = bal_lists;
+        P.nsplit = 2;
       }
     }
     (void)hipGetLastError();
@@ -1422,13 +1409,13 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   // list capacity of the rescan of first-pass failures; the list buffers hold both passes' lists
   // (the rescan keeps at most 2 lists of kp2 per failed row per split, and at least one split fits)
   const uint32_t kp2 = (uint32_t)std::min<uint64_t>(
-      std::max<uint64_t>(P.kp, std::min<uint64_t>(2 * (uint64_t)k, FUSED_KP_MAX)), n - 1);
+      std::max<uint64_t>(P.kp, std::min<uint64_t>((uint64_t)k + (k + 1) / 2, FUSED_KP_MAX)), n - 1);
   size_t nq_slots = 0, nq_entries = 0, score_floats = 0;
   const bool learned = P.fused && p.mlp_params && !p.row_threshold;
   auto workspace_bytes = [&]() -> size_t {
-    // one candidate list per query and split (wide rows: two, one per lane of the query's pair)
-    nq_slots = P.fused ? (size_t)n * P.nsplit : P.wide ? (size_t)n * P.nsplit * 2 : 0;
-    nq_entries = P.fused ? std::max<size_t>(nq_slots * P.kp, (size_t)n * kp2) : nq_slots * P.kp;
+    // two candidate lists per query and split (one per lane of the query's lane pair)
+    nq_slots = (P.fused || P.wide) ? (size_t)n * P.nsplit * 2 : 0;
+    nq_entries = P.fused ? std::max<size_t>(nq_slots * P.kp, (size_t)n * 2 * kp2) : nq_slots * P.kp;
     score_floats = P.wide ? (size_t)P.wide_qb * P.n_pad : 0;
     Carve c(nullptr);
     c.take<float>((size_t)P.n_pad * P.dp32);
@@ -1517,8 +1504,9 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   unsigned long long* err_max = reinterpret_cast<unsigned long long*>(counters + 64);  // ERR_SLOTS
   // f64 rescoring of the candidate lists of the queries `rows` (null: every row) with the
   // certificate; rows that fail it are appended to (fbr, fbl) and counted in *fbc
-  auto rescore = [&](uint32_t kp, uint32_t nlists, const uint32_t* rows, uint32_t nq, int all_admitted,
+  auto rescore = [&](uint32_t kp, uint32_t nsplit, const uint32_t* rows, uint32_t nq, int all_admitted,
                      double eps_list, uint32_t* fbr, double* fbl, uint32_t* fbc, uint32_t* shc) -> int {
+    const uint32_t nlists = 2 * nsplit;
     const uint32_t slots = nlists * kp;
 #define RESCORE(CM)                                                                                    \
   rescore_kernel<CM><<<nq, 64, 0, s>>>(x, d, norms, n, k, (int)kp, nlists, rows, nq, cand_idx, cand_sc,       \
@@ -1560,8 +1548,8 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
                          uint32_t nsplit, uint32_t cols_per, uint32_t units, uint32_t* fbr, double* fbl,
                          uint32_t* fbc, uint32_t* shc) -> int {
       const dim3 grid = units ? dim3(grid_bal, 1) : dim3((nq + P.QB - 1) / P.QB, nsplit);
-      f<<<grid, 64 * P.NW, 0, s>>>(xsq, xs32, P.dp32, n, (int)kp, rows, nq, th, units ? bal_lists : cols_per, units,
-                                   eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor, inserts);
+      f<<<grid, 64 * P.NW, 0, s>>>(xsq, xs32, P.dp32, n, (int)kp, rows, nq, th, cols_per, units, eps_pre,
+                                   cand_idx, cand_sc, cand_cnt, cand_floor, inserts);
       HIP_TRY(hipGetLastError());
       if (!rows) HIP_TRY(hipEventRecord(D.evk[3], s));
       const int all_admitted = (nsplit == 1 && kp >= n - 1 && !th) ? 1 : 0;
@@ -1606,11 +1594,11 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
         uint32_t ns = ((uint32_t)std::max(D.cus, 1) + blocks - 1) / blocks;
         ns = std::min<uint32_t>(ns, 32);
         ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, ntile / 2));
-        ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, RESCORE_CMAX / kpx));
+        ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, 2048 / (2 * kpx)));
         // the lists fit in the list buffers (nq_entries entries, nq_slots lists; one split always
         // fits since nr <= n and kpx <= kp2)
-        ns = std::min<uint64_t>(ns, nq_entries / ((uint64_t)nr * kpx));
-        ns = std::min<uint64_t>(ns, nq_slots / (uint64_t)nr);
+        ns = std::min<uint64_t>(ns, nq_entries / ((uint64_t)nr * 2 * kpx));
+        ns = std::min<uint64_t>(ns, nq_slots / ((uint64_t)nr * 2));
         ns = std::max<uint32_t>(ns, 1);
         const uint32_t tiles_per = (ntile + ns - 1) / ns;
         ns = (ntile + tiles_per - 1) / tiles_per;
@@ -1644,7 +1632,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     }
     HIP_TRY(hipEventRecord(D.evk[3], s));
     const int all_admitted = (P.nsplit == 1 && P.kp >= n - 1) ? 1 : 0;
-    rc = rescore(P.kp, 2 * P.nsplit, nullptr, n, all_admitted, eps32, fb_rows, fb_lo, counters + 1, counters + 2);
+    rc = rescore(P.kp, P.nsplit, nullptr, n, all_admitted, eps32, fb_rows, fb_lo, counters + 1, counters + 2);
     if (rc) return rc;
     HIP_TRY(hipEventRecord(D.evk[4], s));
     rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb_rows, counters + 1,
@@ -1770,8 +1758,8 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
   const int prec = (int)p.precision;
   const uint32_t dp32 = round_up(dmax, 8), dp16 = round_up(dmax, 16);
   const uint32_t J = prec == PREC_F16 ? dp16 / 16 : prec == PREC_F16X3 ? dp16 / 8 : dp32 / 8;
-  const uint32_t TC = J <= 1 ? 768 : J <= 2 ? 384 : J <= 3 ? 256 : J <= 4 ? 192 : 128;  // TileShape<J>::TC
-  const uint32_t QB = 256;  // 64 queries per wave, four waves
+  const uint32_t TC = J <= 2 ? 512 : J <= 4 ? 256 : 128;
+  const uint32_t QB = 128;
   const bool need16 = prec != PREC_F32;
   const int split16 = prec == PREC_F16X3 ? 1 : 0;
   const size_t row16 = (size_t)dp16 * (split16 ? 2 : 1);
@@ -1797,13 +1785,13 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
     for (uint32_t i = 0; i < S; ++i) {
       const costal_sample& a = smp[i];
       const uint32_t n = (uint32_t)a.n;
-      const int64_t mg = a.margin >= 0 ? a.margin : p.margin;
-      const uint32_t kp = list_capacity(n, k, mg, J <= 2);
-      if (kp > FUSED_KP_MAX) return fail(COSTAL_E_UNSUPPORTED, "sample %u: k + margin above %u", i, FUSED_KP_MAX);
+      const int64_t mg = a.margin >= 0 ? a.margin : (p.margin >= 0 ? p.margin : (int64_t)k);
+      const uint32_t kp = list_capacity(n, k, (uint64_t)mg);
+      if (kp > FUSED_KP_MAX) return fail(COSTAL_E_UNSUPPORTED, "sample %u: k + ceil(margin/2) above %u", i, FUSED_KP_MAX);
       const uint32_t ntile = (n + TC - 1) / TC;
       uint32_t ns = a.col_splits ? a.col_splits : (uint32_t)(((uint64_t)n + target_cols - 1) / target_cols);
       ns = std::max<uint32_t>(1, std::min<uint32_t>(ns, ntile));
-      ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, RESCORE_CMAX / kp));
+      ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, 2048 / (2 * kp)));
       const uint32_t tiles_per = (ntile + ns - 1) / ns;
       ns = (ntile + tiles_per - 1) / tiles_per;
       BatchSample& b = bs[i];
@@ -1831,10 +1819,10 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
           items.push_back(sp);
         }
       blocks_total += (uint64_t)nqb * ntile;
-      pad += round_up(n, PAD_ROWS);
+      pad += round_up(n, 512);
       base += n;
-      slots += (uint64_t)n * ns;
-      entries += (uint64_t)n * ns * kp;
+      slots += (uint64_t)n * ns * 2;
+      entries += (uint64_t)n * ns * 2 * kp;
       max_split = std::max(max_split, ns);
       max_kp = std::max(max_kp, kp);
     }
@@ -1910,7 +1898,7 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
     const double eps_list = prec == PREC_F16X3 ? fp16x3_eps(dmax) : eps32;
     const float eps_pre = prec == PREC_F16 ? (float)(fp16_eps(dmax) + eps32 + std::ldexp(1.0, -22)) : 0.0f;
     const uint8_t* xsq = need16 ? (const uint8_t*)xs16 : (const uint8_t*)xs32;
-    const uint32_t max_slots = max_split * max_kp;
+    const uint32_t max_slots = 2 * max_split * max_kp;
     const uint32_t grid_scan = 2u * (uint32_t)std::max(D.cus, 1);
     const uint32_t grid_fb = 4u * (uint32_t)std::max(D.cus, 1);
     // counters: [0..1] first bad (sample, row), [2] rows failing the certificate, [3] of them short,

@@ -78,27 +78,28 @@ __device__ __forceinline__ float dot32(const float* __restrict__ a, const float*
 // ------------------------------------------------------------------------------------------------
 // coarse_kernel
 //
-// Workgroup = NW waves; each wave owns 64 queries in two groups of 32, held in registers for the
-// whole launch.  The workgroup streams its candidate range through two LDS buffers of TC-row
-// tiles (about 24 KB each), copied global -> LDS by the load units directly (global_load_lds,
-// 16 bytes per lane) one tile ahead of the compute, so no registers hold staged data.  For every
-// 32-candidate sub-tile a wave issues the MFMAs D[cand][query] = sum_k cand[k] * query[k] for both
-// query groups: in each 32x32 accumulator, lane l holds query (l % 32) of the group and the 16
-// candidates 8*(i/4) + 4*(l/32) + (i%4), i = 0..15.  Lane l owns query l of the wave (group l / 32,
-// column l % 32) and keeps its one sorted candidate list in registers; the other half of its
-// query's scores sits in lane l ^ 32 (which owns a query of the other group).  The fast path
-// compares the maxima of both accumulators with the thresholds of the queries they belong to
-// (the lane's own, and the partner lane's, exchanged once per tile; a stale partner threshold is
-// only lower, so it never turns a candidate away).  A wave with a hit exchanges the accumulator
-// halves with v_permlane32_swap, so each lane holds all 32 scores of its own query, and inserts.
-// MFMAs run one step ahead of the scan (see scan_block).
+// Workgroup = NW waves; each wave owns 32 queries, held in registers for the whole launch.  The
+// workgroup streams its candidate range through two LDS buffers of TC-row tiles (32 KB each),
+// staged global -> registers -> LDS ahead of the compute.  For every 32-candidate
+// sub-tile a wave issues the MFMAs for D[cand][query] = sum_k cand[k] * query[k]: lane l owns query
+// (l % 32) and the 16 candidates 8*(i/4) + 4*(l/32) + (i%4), i = 0..15 (the 32x32 accumulator
+// layout).  The scan takes the max of a step's sub-tiles (max3 chains) and compares once with the
+// lane's threshold; only waves with a hit take the insert path.  MFMAs run one step ahead of the
+// scan (see scan_block).
+//
+// Candidate lists live in registers: each lane keeps its own sorted list of the best kp (score,
+// candidate) pairs among the candidates it sees (its half of every sub-tile), so the two lanes of a
+// query never wait on each other.  Both lists go to the rescoring, which treats them like two
+// candidate-range splits: a candidate outside both lists scored at most the larger of the two
+// floors.
 //
 // Row layout (global and LDS): a row is 2*J 16-byte chunks; chunk q feeds lane half h = q % 2,
 // slot j = q / 2.  Queries and candidates use the same chunk -> (lane half, slot) map, so each
 // MFMA contracts matching feature indices (any consistent permutation of k leaves the sum a sum of
-// the same products).  In LDS a tile is 1 KB groups, group (sub-tile s, slot j) = s*J + j holding
-// the two chunks 2j, 2j+1 of the sub-tile's 32 rows, lane-contiguous (slot 32*half + row): each
-// ds_read_b128 is conflict free, and each group is one wave-wide global_load_lds (lane = slot).
+// the same products).  In LDS the chunks of one (sub-tile, slot, half) are stored for the 32 lanes
+// contiguously (1 KB per wave-wide load), so each ds_read_b128 is conflict free.
+
+
 __device__ __forceinline__ float max16(const f32x16& a) {
   const float m0 = fmaxf(fmaxf(a[0], a[1]), a[2]);
   const float m1 = fmaxf(fmaxf(a[3], a[4]), a[5]);
@@ -186,37 +187,28 @@ __device__ __forceinline__ f32x16 tile_mfma(const uint4 (&b)[J], const uint4 (&q
 
 template <int J>
 struct TileShape {
-  static constexpr int ROWB = 32 * J;  // bytes per working-copy row
-  // candidates per tile: about 24 KB tiles, so three four-wave workgroups fit a CU's LDS (two
-  // for J >= 7); an even number of 32-candidate steps, at least four (the pipeline reads three
-  // steps ahead)
-  static constexpr int TC = J <= 1 ? 768 : J <= 2 ? 384 : J <= 3 ? 256 : J <= 4 ? 192 : 128;
-  static constexpr int TILEB = TC * ROWB;
-  static constexpr int NSTEP = TC / 32;      // one 32-candidate sub-tile per scan step
-  static constexpr int GROUPS = TILEB / 1024;  // 1 KB LDS groups of a tile
-  static_assert(NSTEP % 2 == 0 && NSTEP >= 4, "pipeline steps per tile");
+  static constexpr int TC = J <= 2 ? 512 : J <= 4 ? 256 : 128;  // candidates per tile: 32 KB tiles
+  static constexpr int TILEB = TC * 32 * J;
+  static constexpr int SUBT = TC / 32;                         // 32-candidate sub-tiles per tile
+  static constexpr int GS = (SUBT >= 4 && J <= 4) ? 2 : 1;     // sub-tiles per scan step
+  static constexpr int NSTEP = SUBT / GS;                      // scan steps per tile
 };
 
-// Rows of zero padding every working copy carries past n: a whole number of tiles of every shape
-// (768, 384, 256, 192, 128 rows) and of the wide-row blocks (256 rows).
-constexpr uint32_t PAD_ROWS = 1536;
-
-
-// One work item of the scan: query block `qblock` (QB = 64 NW queries) against candidates
-// [c_begin, c_end) of one sample; the lists go to split `split` of the queries' nsplit lists.
-// `tiles` is the workgroup's two LDS tile buffers.
+// One work item of the scan: query block `qblock` (QB queries) against candidates [c_begin, c_end)
+// of one sample, whose lists go to split `split` of the queries' nsplit list pairs.  tile0/tile1
+// are the workgroup's two LDS tile buffers.
 //
-// Software pipeline over scan steps (one 32-candidate sub-tile each, NSTEP per tile): step p first
-// reads the candidate chunks of step p+2 from LDS into the chunk registers the MFMAs of step p-1
-// have consumed, then issues the MFMAs of step p+1 and scans the accumulators of step p, the MFMAs
-// interleaved with the scan's compares so that the wave never waits on the matrix core to issue
-// them; the LDS reads have a whole step to complete.  Accumulators and chunk registers are double
-// buffered by the parity of p (NSTEP is even).  At step NSTEP-2 of tile t every wave waits for its
-// share of tile t+1's copy, the workgroup meets, and the copy of tile t+2 into the buffer tile t
-// leaves is issued: it has a whole tile of steps to arrive.
+// Software pipeline over scan steps (GS sub-tiles each, NSTEP per tile, G counting steps over the
+// whole candidate range): step G issues the MFMAs of step G+1, reads the candidate chunks of step
+// G+3 from LDS, and then scans the accumulators of step G, so the matrix core works on the next
+// step while the wave compares and inserts, and the LDS reads complete behind two steps of work.
+// Accumulators and chunk registers are double buffered by the parity of G (NSTEP is even, so the
+// parity is the step's parity within its tile).  The next tile is stored to the other LDS buffer,
+// and the workgroup meets at the barrier, at step NSTEP-3 of each tile, before the first read of
+// that tile.
 template <int PREC, int J, int KPR, int NW>
 __device__ __forceinline__ void scan_block(
-    uint8_t* __restrict__ tiles,       // 2 * TILEB bytes of LDS
+    uint8_t* __restrict__ tile0, uint8_t* __restrict__ tile1,
     const uint8_t* __restrict__ xsq,   // n_pad rows of 32*J bytes (the working copy of this form)
     const float* __restrict__ xs32,    // f32 working copy (dp32 per row), for the FP16 re-score
     uint32_t dp32, uint32_t n, int kp,
@@ -227,38 +219,36 @@ __device__ __forceinline__ void scan_block(
     float eps_pre,                     // FP16 form: bound on |FP16 score - f32 score|
     uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc, uint32_t* __restrict__ cand_cnt,
     float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
-  using TS = TileShape<J>;
-  constexpr int QB = 64 * NW;
-  constexpr int TC = TS::TC, ROWB = TS::ROWB, NSTEP = TS::NSTEP, TILEB = TS::TILEB;
-  constexpr int RD = 2;  // chunks of step p + RD are read at step p
-  static_assert(TS::GROUPS % NW == 0, "every wave copies the same number of groups");
-  static_assert(RD == 2, "the tile change is the step before the last");
-  typedef __attribute__((address_space(3))) void lds_t;
-  typedef __attribute__((address_space(1))) const void glob_t;
+  constexpr int QB = 32 * NW;
+  constexpr int TC = TileShape<J>::TC;
+  constexpr int ROWB = 32 * J;
+  constexpr int NT = 64 * NW;
+  constexpr int CHUNKS = TC * 2 * J;        // 16-byte chunks per tile
+  constexpr int CPT = (CHUNKS + NT - 1) / NT;
+  constexpr int GS = TileShape<J>::GS;
+  constexpr int NSTEP = TileShape<J>::NSTEP;
+  static_assert(NSTEP >= 2 && NSTEP % 2 == 0, "pipeline steps per tile");
+  static_assert(GS * NSTEP * 32 == TC, "tile shape");
 
   const int tid = tid_x();
-  const int l = tid & 63, h = l >> 5, l32 = l & 31;
-  const uint32_t w = __builtin_amdgcn_readfirstlane((uint32_t)tid >> 6);
-  const uint32_t qbase = qblock * QB + w * 64;
-  auto row_of = [&](uint32_t q) -> uint32_t { return q < nq ? (rowlist ? rowlist[q] : q) : 0u; };
-  // the two query groups' rows, as B operands: lane l gives column l % 32 of each group
-  const uint32_t g0 = row_of(qbase + l32), g1 = row_of(qbase + 32 + l32);
-  uint4 q0[J], q1[J];
-#pragma unroll
-  for (int j = 0; j < J; ++j) {
-    q0[j] = *reinterpret_cast<const uint4*>(xsq + (size_t)g0 * ROWB + (2 * j + h) * 16);
-    q1[j] = *reinterpret_cast<const uint4*>(xsq + (size_t)g1 * ROWB + (2 * j + h) * 16);
-  }
-  // the lane's own query: number l of the wave
-  const uint32_t qq = qbase + l;
+  const int w = tid >> 6, l = tid & 63, h = l >> 5, l32 = l & 31;
+  const uint32_t qq = qblock * QB + w * 32 + l32;
   const bool qvalid = qq < nq;
-  const uint32_t qg = h ? g1 : g0;
+  const uint32_t qg = qvalid ? (rowlist ? rowlist[qq] : qq) : 0;
+
+  // the wave's queries, in registers for the whole launch
+  uint4 qreg[J];
+  {
+    const uint8_t* qrow = xsq + (size_t)qg * ROWB;
+#pragma unroll
+    for (int j = 0; j < J; ++j) qreg[j] = *reinterpret_cast<const uint4*>(qrow + (2 * j + h) * 16);
+  }
   // Empty list entries hold the lane's admission threshold (-inf without one; +inf for padding
-  // queries, which so never insert): the list minimum ls[0] is then always the lane's threshold,
-  // and real entries (score above it) are the ones with an index.  A NaN threshold (caller data
-  // or a model with NaN parameters) admits everything: the test reads the bits, since this code is
-  // built without NaN semantics.
+  // queries, which so never take the insert path): the list minimum ls[0] is then always the
+  // lane's threshold, and real entries (score above it) are the ones with an index.
   float t0 = qvalid ? (tinit ? tinit[qg] : -INFINITY) : INFINITY;
+  // a NaN threshold (caller data or a model with NaN parameters) admits everything: the test reads
+  // the bits, since this code is built without NaN semantics
   if ((__float_as_uint(t0) & 0x7fffffffu) > 0x7f800000u) t0 = -INFINITY;
   float ls[KPR];
   uint32_t li[KPR];
@@ -268,151 +258,155 @@ __device__ __forceinline__ void scan_block(
     li[e] = NONE;
   }
   float th = ls[0];
-  float thp = partner32(th);  // the partner lane's threshold, refreshed once per tile
-  // fast-path thresholds of the two accumulators (group 0, group 1), minus the FP16 pre-filter
-  // bound in that form
-  const float pre = PREC == PREC_F16 ? eps_pre : 0.0f;
-  float fA = (h ? thp : th) - pre, fB = (h ? th : thp) - pre;
+  float thp = -INFINITY;  // the partner list's threshold, refreshed once per tile
   uint32_t my_inserts = 0;
-  const float* xq32 = xs32 + (size_t)qg * dp32;
 
   const int ntile = c_begin < c_end ? (int)((c_end - c_begin + TC - 1) / TC) : 0;
-  // tile T of the range into LDS buffer b: wave w copies groups w, w + NW, ...; group g = s*J + j
-  // is the chunks 2j, 2j+1 of rows 32s..32s+31, lane l taking chunk 2j + l/32 of row 32s + l%32
-  // (the wave-uniform part of each address first, the lane's 32-bit offset last, so a load can
-  // take a scalar base and one vector offset)
-  const uint32_t lane_off = (uint32_t)l32 * ROWB + 16 * h;
-  auto copy_tile = [&](int T, uint8_t* dst) __attribute__((always_inline)) {
-    const uint8_t* src = xsq + (size_t)(c_begin + (uint32_t)T * TC) * ROWB;
-#pragma unroll
-    for (int c = 0; c < TS::GROUPS / NW; ++c) {
-      const uint32_t g = w + NW * c;
-      const uint32_t s = g / J, j = g % J;
-      const uint8_t* base = src + (size_t)(s * 32 * ROWB + 32 * j);
-      uint32_t lo = lane_off;
-      __asm__ volatile("" : "+v"(lo));  // recomputed per copy, not a hoisted 64-bit address
-      __builtin_amdgcn_global_load_lds((glob_t*)(base + lo), (lds_t*)(dst + g * 1024), 16, 0, 0);
-    }
-  };
-  // this wave's copies complete, every wave's LDS reads complete, then the workgroup meets
-  auto tile_sync = [&]() __attribute__((always_inline)) {
-    __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) lgkmcnt(0)
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  };
 
-  f32x16 acc[2][2];  // [step parity][query group]
-  uint4 bop[2][J];   // [step parity][slot]: the lane's candidate chunks of one sub-tile
-  auto read_b = [&](int slot, const uint8_t* buf, int s) __attribute__((always_inline)) {
-    const uint8_t* p = buf + s * J * 1024 + l * 16;
-#pragma unroll
-    for (int j = 0; j < J; ++j) bop[slot][j] = *reinterpret_cast<const uint4*>(p + j * 1024);
-  };
-  auto mfma_step = [&](int slot) __attribute__((always_inline)) {
-    acc[slot][0] = tile_mfma<PREC, J>(bop[slot], q0);
-    acc[slot][1] = tile_mfma<PREC, J>(bop[slot], q1);
-  };
-  // MFMAs per step: both query groups' tiles
-  constexpr int NMFMA = 2 * (PREC == PREC_F32 ? 4 * J : PREC == PREC_F16 ? J : 3 * (J / 2));
-  // one candidate of the lane's own query: admitted when it beats the threshold
-  auto offer = [&](float v, uint32_t cand) __attribute__((always_inline)) {
-    if (v > th - pre && cand < c_end && cand != qg) {
-      if constexpr (PREC == PREC_F16) {
-        v = dot32(xq32, xs32 + (size_t)cand * dp32, dp32);
-        if (!(v > th)) return;
-      }
-      list_push<KPR>(ls, li, v, cand);
-      th = ls[0];
-      ++my_inserts;
-    }
-  };
-  // scan the accumulators of one step whose candidates start at cbase.  Fast path: the maxima of
-  // both accumulators against the thresholds of the queries they belong to.  Slow path: the
-  // quads (elements 4q..4q+3: candidates 8q + 4h + 0..3) where some lane has a hit (a
-  // wave-uniform mask), and in each such quad every element exchanged with the partner lane, so
-  // the lane offers both halves of its own query's scores (candidates 8q + e and 8q + e + 4).
-  auto fast_test = [&](const f32x16& a, const f32x16& b) __attribute__((always_inline)) -> bool {
-    const float ma = max16(a), mb = max16(b);
-    // two lane masks or-ed in scalar registers (no short-circuit branch between the maxima)
-    return (__builtin_amdgcn_ballot_w64(ma > fA) | __builtin_amdgcn_ballot_w64(mb > fB)) != 0;
-  };
-  auto slow_path = [&](f32x16& a, f32x16& b, uint32_t cbase) __attribute__((always_inline)) {
-    uint32_t qmask = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const float qa = fmaxf(fmaxf(fmaxf(a[4 * q], a[4 * q + 1]), a[4 * q + 2]), a[4 * q + 3]);
-      const float qb = fmaxf(fmaxf(fmaxf(b[4 * q], b[4 * q + 1]), b[4 * q + 2]), b[4 * q + 3]);
-      qmask |= (__builtin_amdgcn_ballot_w64(qa > fA) | __builtin_amdgcn_ballot_w64(qb > fB)) != 0 ? 1u << q : 0u;
-    }
-    while (qmask) {
-      const int q = __builtin_ctz(qmask);
-      qmask &= qmask - 1;
-#pragma unroll 1
-      for (int e = 0; e < 4; ++e) {
-        const int i = 4 * q + e;
-        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a[i]), __float_as_uint(b[i]), false, false);
-        const uint32_t c = cbase + 8 * q + e;
-        offer(__uint_as_float(r[0]), c);
-        offer(__uint_as_float(r[1]), c + 4);
-      }
-    }
-    fA = (h ? thp : th) - pre;
-    fB = (h ? th : thp) - pre;
-  };
-  // step p of tile t (parity S = p & 1): the tile change at step NSTEP-RD, then the chunks of step
-  // p+RD, the MFMAs of step p+1 and the scan of step p.  The step opens with a wait for every LDS
-  // read in flight, which are the chunks its MFMAs use (read a step earlier): the copies into LDS
-  // leave the compiler no in-order count of LDS reads, so without this wait it would wait at the
-  // first MFMA for the reads this step has just issued as well.
+  // Tiles go global -> registers -> LDS: the registers are loaded with tile t+2 while tile t is
+  // scanned (right after the workgroup has stored tile t+1 and met) and stored during tile t+1, so
+  // a load has a whole tile of steps to arrive.  One register set, and a loop body of one tile
+  // whose only exit is at its end, swapping the two LDS buffers: the compiler's wait counts then
+  // stay exact at the loop head (a second staging set, or an exit from the middle of a body
+  // unrolled by two, made it wait there for loads still in flight).
+  static_assert(CPT <= 8, "tile staging uses at most eight 16-byte registers per thread");
+  uint4 ra0 = {}, ra1 = {}, ra2 = {}, ra3 = {}, ra4 = {}, ra5 = {}, ra6 = {}, ra7 = {};
+  // Thread e fills LDS slot e (consecutive lanes, consecutive 16-byte slots: conflict free); slot
+  // e = ((sub*J + j)*2 + half)*32 + lane32 holds chunk 2j + half of tile row sub*32 + lane32.
+#define LOAD_ONE(R, C)                                                                            \
+  if constexpr (C < CPT) {                                                                        \
+    const int e_ = tid + C * NT;                                                                  \
+    if (CHUNKS % NT == 0 || e_ < CHUNKS) {                                                        \
+      const int lr_ = e_ & 31, hh_ = (e_ >> 5) & 1, j_ = (e_ >> 6) % J, sub_ = (e_ >> 6) / J;     \
+      const uint32_t off_ = (uint32_t)((sub_ * 32 + lr_) * ROWB + (2 * j_ + hh_) * 16);            \
+      R##C = *reinterpret_cast<const uint4*>(src_ + off_);                                       \
+    }                                                                                             \
+  }
+#define LOAD_REGS(R, T)                                                                           \
+  {                                                                                               \
+    const uint8_t* src_ = xsq + (size_t)(c_begin + (uint32_t)(T) * TC) * ROWB;                     \
+    LOAD_ONE(R, 0) LOAD_ONE(R, 1) LOAD_ONE(R, 2) LOAD_ONE(R, 3)                                   \
+    LOAD_ONE(R, 4) LOAD_ONE(R, 5) LOAD_ONE(R, 6) LOAD_ONE(R, 7)                                   \
+  }
+#define STORE_ONE(R, C, BUF)                                                                      \
+  if constexpr (C < CPT) {                                                                        \
+    const int e_ = tid + C * NT;                                                                  \
+    if (CHUNKS % NT == 0 || e_ < CHUNKS) *reinterpret_cast<uint4*>(&BUF[e_ * 16]) = R##C;         \
+  }
+#define STORE_REGS(R, BUF) \
+  { STORE_ONE(R, 0, BUF) STORE_ONE(R, 1, BUF) STORE_ONE(R, 2, BUF) STORE_ONE(R, 3, BUF)         \
+    STORE_ONE(R, 4, BUF) STORE_ONE(R, 5, BUF) STORE_ONE(R, 6, BUF) STORE_ONE(R, 7, BUF) }
+  // LDS writes of this wave done, then everyone meets; no wait on the global loads in flight
+#define TILE_BARRIER()                                                                            \
+  {                                                                                               \
+    __builtin_amdgcn_s_waitcnt(0xc07f); /* lgkmcnt(0) */                                          \
+    __builtin_amdgcn_s_barrier();                                                                 \
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");                                        \
+  }
+  // the lane's chunks of scan step S of the tile in BUF, into chunk buffer BS
+#define READ_B(BS, BUF, S)                                                                        \
+  _Pragma("unroll") for (int u_ = 0; u_ < GS; ++u_)                                               \
+    _Pragma("unroll") for (int j_ = 0; j_ < J; ++j_)                                              \
+      bop[BS][u_][j_] = *reinterpret_cast<const uint4*>(                                          \
+          (BUF) + (((((S) * GS + u_) * J + j_) * 2 + h) * 32 + l32) * 16);
+#define MFMA_STEP(SLOT) \
+  _Pragma("unroll") for (int u_ = 0; u_ < GS; ++u_) acc[SLOT][u_] = tile_mfma<PREC, J>(bop[SLOT][u_], qreg);
+
+  const float* xq32 = xs32 + (size_t)qg * dp32;
+  f32x16 acc[2][GS];
+  uint4 bop[2][GS][J];
+  constexpr int RD = 3;  // chunks of step G+RD are read at step G
+  static_assert(NSTEP >= RD, "steps per tile");
   int t = 0;
-  auto step = [&](int p, int S, bool change) __attribute__((always_inline)) {
-    if (p == 0) {
-      thp = partner32(th);
-      fA = (h ? thp : th) - pre;
-      fB = (h ? th : thp) - pre;
-    }
-    const int cur = t & 1;
-    if (change) {
-      tile_sync();
-      if (t + 2 < ntile) copy_tile(t + 2, tiles + cur * TILEB);
-    } else {
-      __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
-    }
-    // one scheduling region: the chunks of step p+RD into bop[S] (read by the MFMAs of the step
-    // before); the MFMAs of step p+1 (chunks in bop[1-S]); and the fast-path compares of step p,
-    // the MFMAs spread over the compares
-    const int src = p + RD < NSTEP ? cur : 1 - cur;
-    const int sstep = p + RD < NSTEP ? p + RD : p + RD - NSTEP;
-    read_b(S, tiles + src * TILEB, sstep);
-    mfma_step(1 - S);
-    const bool hit = fast_test(acc[S][0], acc[S][1]);
-    __builtin_amdgcn_sched_group_barrier(0x100, J, 0);  // the LDS reads first
+  // compare the accumulators of step `p` of tile t with the lane's threshold; rare path: each lane
+  // walks its own hits, sub-tile by sub-tile
+  auto scan_step = [&](const f32x16 (&a)[GS], int p) __attribute__((always_inline)) {
+    float mu[GS];
 #pragma unroll
-    for (int m = 0; m < NMFMA; ++m) {
-      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
-      __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);  // then three VALU
+    for (int u = 0; u < GS; ++u) mu[u] = max16(a[u]);
+    float m = mu[0];
+#pragma unroll
+    for (int u = 1; u < GS; ++u) m = fmaxf(m, mu[u]);
+    float the = (PREC == PREC_F16) ? fmaxf(th, thp) - eps_pre : fmaxf(th, thp);
+    if (__builtin_amdgcn_ballot_w64(m > the) != 0) {
+#pragma unroll
+      for (int u = 0; u < GS; ++u) {
+        if (mu[u] > the) {
+          // bit i = accumulator i above the threshold (built from the top, no wide constants)
+          uint32_t mask = 0;
+#pragma unroll
+          for (int i = 15; i >= 0; --i) mask = (mask << 1) | (a[u][i] > the ? 1u : 0u);
+          // the lane's offset is added here, not folded into per-sub-tile bases hoisted out of the
+          // tile loop (each would hold a register for the whole launch)
+          uint32_t lane_off = 4 * h;
+          __asm__ volatile("" : "+v"(lane_off));
+          const uint32_t cbase = c_begin + (uint32_t)t * TC + (uint32_t)(p * GS + u) * 32 + lane_off;
+          while (mask) {
+            const int i = __builtin_ctz(mask);
+            mask &= mask - 1;
+            float v = pick16(a[u], i);
+            const uint32_t cand = cbase + 8 * (i >> 2) + (i & 3);
+            if (v > the && cand < c_end && cand != qg) {
+              if constexpr (PREC == PREC_F16) {
+                v = dot32(xq32, xs32 + (size_t)cand * dp32, dp32);
+                if (!(v > th)) continue;
+              }
+              list_push<KPR>(ls, li, v, cand);
+              th = ls[0];
+              the = (PREC == PREC_F16) ? fmaxf(th, thp) - eps_pre : fmaxf(th, thp);
+              ++my_inserts;
+            }
+          }
+        }
+      }
     }
-    if (hit) slow_path(acc[S][0], acc[S][1], c_begin + (uint32_t)t * TC + (uint32_t)p * 32);
   };
+  // all NSTEP steps of tile t, held in CUR; the next tile goes from the staging registers to OTH
+#define RUN_TILE(CUR, OTH)                                                                        \
+  _Pragma("unroll") for (int p = 0; p < NSTEP; ++p) {                                             \
+    MFMA_STEP((p + 1) & 1)                                                                        \
+    if (p == 0) thp = partner32(th);                                                              \
+    if (p == NSTEP - RD) {                                                                        \
+      STORE_REGS(ra, OTH);                                                                        \
+      TILE_BARRIER();                                                                             \
+      LOAD_REGS(ra, t + 2 < ntile ? t + 2 : ntile - 1);                                           \
+    }                                                                                             \
+    if (p + RD < NSTEP) {                                                                         \
+      READ_B((p + RD) & 1, CUR, p + RD)                                                           \
+    } else {                                                                                      \
+      READ_B((p + RD) & 1, OTH, p + RD - NSTEP)                                                   \
+    }                                                                                             \
+    scan_step(acc[p & 1], p);                                                                     \
+  }
 
   if (ntile > 0) {
-    copy_tile(0, tiles);
-    tile_sync();
-    if (ntile > 1) copy_tile(1, tiles + TILEB);
-    read_b(0, tiles, 0);
-    read_b(1, tiles, 1);
-    mfma_step(0);
-    // the tile change peeled from the step loop: its copies stay out of the other steps' paths
+    uint8_t* cur = tile0;
+    uint8_t* oth = tile1;
+    LOAD_REGS(ra, 0);
+    STORE_REGS(ra, cur);
+    TILE_BARRIER();
+    LOAD_REGS(ra, ntile > 1 ? 1 : 0);
+    READ_B(0, cur, 0)
+    READ_B(1, cur, 1)
+    MFMA_STEP(0)
+    READ_B(0, cur, 2)
     for (;;) {
-      for (int p = 0; p < NSTEP - RD; p += 2) {
-        step(p, 0, false);
-        step(p + 1, 1, false);
-      }
-      step(NSTEP - 2, 0, true);
-      step(NSTEP - 1, 1, false);
+      // tile t in cur, ra = tile t+1 in flight
+      RUN_TILE(cur, oth)
       if (++t >= ntile) break;
+      uint8_t* const tmp = cur;
+      cur = oth;
+      oth = tmp;
     }
   }
+#undef RUN_TILE
+#undef MFMA_STEP
+#undef READ_B
+#undef TILE_BARRIER
+#undef LOAD_REGS
+#undef STORE_REGS
+#undef LOAD_ONE
+#undef STORE_ONE
 
   if (inserts) {
     // one atomic per wave
@@ -421,8 +415,9 @@ __device__ __forceinline__ void scan_block(
     for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
     if (l == 0) atomicAdd(inserts, (unsigned long long)tot);
   }
+  const float thp_end = partner32(th);
   if (qvalid) {
-    const size_t slot = (size_t)qq * nsplit + split;
+    const size_t slot = ((size_t)qq * nsplit + split) * 2 + h;
     const size_t base = slot * (size_t)kp;
     uint32_t c = 0;
 #pragma unroll
@@ -434,9 +429,7 @@ __device__ __forceinline__ void scan_block(
       }
     }
     cand_cnt[slot] = c;
-    // everything the list turned away scored at most this (the fast path compared the other
-    // half of the query's scores with an older, lower copy of it)
-    cand_floor[slot] = th;
+    cand_floor[slot] = fmaxf(th, thp_end);  // everything this list turned away scored at most this
   }
 }
 
@@ -444,28 +437,19 @@ __device__ __forceinline__ void scan_block(
 //   units_per_wg == 0: grid (query blocks, splits); workgroup (x, y) scans query block x against
 //     columns [y * cols_per_split, (y+1) * cols_per_split).
 //   units_per_wg > 0: one workgroup per resident slot, each taking the same number of (query block,
-//     tile) units in query-block-major order, so every slot ends together.  A query block is cut
-//     into the pieces that fall to different workgroups; piece p of a block (p = workgroup -
-//     the first workgroup touching the block) fills list p of its queries, and the workgroup with
-//     the block's last piece marks the lists after it empty.  cols_per_split then gives the lists
-//     per query (the most pieces a block can have: ceil(tiles / units_per_wg) + 1).
-// Waves per SIMD the kernel is built for: three where the registers allow it (narrow rows, short
-// lists), two for the rest, and no bound for the longest lists on wide rows (which would spill
-// to scratch under one; they serve rescans of a few rows).
-template <int J, int KPR>
-constexpr int scan_waves() {
-  return (J <= 2 && KPR <= 16) ? 3 : (KPR >= 64 || (J >= 7 && KPR >= 32)) ? 1 : 2;
-}
-
+//     tile) units in query-block-major order, so every slot ends together.  A query block whose
+//     units straddle two workgroups is scanned in two pieces: the one from tile 0 fills list pair 0,
+//     the other list pair 1 (nsplit = 2); a block scanned whole marks pair 1 empty.
 template <int PREC, int J, int KPR, int NW>
-__global__ __launch_bounds__(64 * NW, (scan_waves<J, KPR>())) void coarse_kernel(
+__global__ __launch_bounds__(64 * NW) void coarse_kernel(
     const uint8_t* __restrict__ xsq, const float* __restrict__ xs32, uint32_t dp32, uint32_t n, int kp,
     const uint32_t* __restrict__ rowlist, uint32_t nq, const float* __restrict__ tinit, uint32_t cols_per_split,
     uint32_t units_per_wg, float eps_pre, uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc,
     uint32_t* __restrict__ cand_cnt, float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
-  __shared__ __attribute__((aligned(1024))) uint8_t tiles[2 * TileShape<J>::TILEB];
+  __shared__ __attribute__((aligned(16))) uint8_t tile0[TileShape<J>::TILEB];
+  __shared__ __attribute__((aligned(16))) uint8_t tile1[TileShape<J>::TILEB];
   constexpr uint32_t TC = TileShape<J>::TC;
-  constexpr uint32_t QB = 64 * NW;
+  constexpr uint32_t QB = 32 * NW;
   const uint32_t ntile = (n + TC - 1) / TC;
   const uint64_t units = units_per_wg ? (uint64_t)((nq + QB - 1) / QB) * ntile : 1;
   uint64_t u = units_per_wg ? (uint64_t)bid_x() * units_per_wg : 0;
@@ -479,12 +463,12 @@ __global__ __launch_bounds__(64 * NW, (scan_waves<J, KPR>())) void coarse_kernel
       qb = (uint32_t)(u / ntile);
       const uint32_t tb = (uint32_t)(u % ntile);
       const uint32_t te = (uint32_t)(tb + (u_end - u) < ntile ? tb + (u_end - u) : ntile);
-      split = bid_x() - (uint32_t)((uint64_t)qb * ntile / units_per_wg);
-      nspl = cols_per_split;
+      split = tb == 0 ? 0 : 1;
+      nspl = 2;
       c0 = tb * TC;
       c1 = te * TC < n ? te * TC : n;
       len = te - tb;
-      whole = te == ntile;  // the block's last piece
+      whole = tb == 0 && te == ntile;
     } else {
       qb = bid_x();
       split = bid_y();
@@ -494,15 +478,15 @@ __global__ __launch_bounds__(64 * NW, (scan_waves<J, KPR>())) void coarse_kernel
       len = 1;
     }
     if (!first) __syncthreads();  // every wave is done with the previous piece's tiles
-    scan_block<PREC, J, KPR, NW>(tiles, xsq, xs32, dp32, n, kp, rowlist, nq, qb, tinit, split, nspl, c0, c1,
+    scan_block<PREC, J, KPR, NW>(tile0, tile1, xsq, xs32, dp32, n, kp, rowlist, nq, qb, tinit, split, nspl, c0, c1,
                                  eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor, inserts);
     if (whole) {
-      const uint32_t qq = qb * QB + tid_x();
-      if (qq < nq)
-        for (uint32_t s = split + 1; s < nspl; ++s) {
-          cand_cnt[(size_t)qq * nspl + s] = 0;
-          cand_floor[(size_t)qq * nspl + s] = -INFINITY;
-        }
+      const uint32_t qq = qb * QB + (tid_x() >> 6) * 32 + (tid_x() & 31);
+      if (qq < nq) {
+        const size_t slot = ((size_t)qq * 2 + 1) * 2 + ((tid_x() >> 5) & 1);
+        cand_cnt[slot] = 0;
+        cand_floor[slot] = -INFINITY;
+      }
     }
     u += len;
   }
@@ -510,13 +494,14 @@ __global__ __launch_bounds__(64 * NW, (scan_waves<J, KPR>())) void coarse_kernel
 
 // items[3e..3e+2] = (sample, query block, split), in decreasing cost order
 template <int PREC, int J, int KPR, int NW>
-__global__ __launch_bounds__(64 * NW, (scan_waves<J, KPR>())) void coarse_batch_kernel(
+__global__ __launch_bounds__(64 * NW) void coarse_batch_kernel(
     const BatchSample* __restrict__ samples, const uint32_t* __restrict__ items, uint32_t nitems,
     uint32_t* __restrict__ queue, const uint8_t* __restrict__ xsq, const float* __restrict__ xs32, uint32_t dp32,
     float eps_pre, uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc, uint32_t* __restrict__ cand_cnt,
     float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts,
     const unsigned long long* __restrict__ bad) {
-  __shared__ __attribute__((aligned(1024))) uint8_t tiles[2 * TileShape<J>::TILEB];
+  __shared__ __attribute__((aligned(16))) uint8_t tile0[TileShape<J>::TILEB];
+  __shared__ __attribute__((aligned(16))) uint8_t tile1[TileShape<J>::TILEB];
   __shared__ uint32_t s_item;
   // a sample with a NaN or inf row fails the whole call: no score is compared
   if (*bad != ~0ull) return;
@@ -531,9 +516,10 @@ __global__ __launch_bounds__(64 * NW, (scan_waves<J, KPR>())) void coarse_batch_
     const size_t rowb = (size_t)S.row_pad;
     const uint32_t c0 = sp * S.cols_per_split;
     const uint32_t c1 = c0 + S.cols_per_split < S.n ? c0 + S.cols_per_split : S.n;
-    scan_block<PREC, J, KPR, NW>(tiles, xsq + rowb * 32 * J, xs32 + rowb * dp32, dp32, S.n, S.kp, nullptr, S.n,
-                                 qb, nullptr, sp, S.nsplit, c0, c1, eps_pre, cand_idx + S.entry_base,
-                                 cand_sc + S.entry_base, cand_cnt + S.slot_base, cand_floor + S.slot_base, inserts);
+    scan_block<PREC, J, KPR, NW>(tile0, tile1, xsq + rowb * 32 * J, xs32 + rowb * dp32, dp32, S.n, S.kp, nullptr,
+                                 S.n, qb, nullptr, sp, S.nsplit, c0, c1, eps_pre,
+                                 cand_idx + S.entry_base, cand_sc + S.entry_base, cand_cnt + S.slot_base,
+                                 cand_floor + S.slot_base, inserts);
     __syncthreads();
   }
 }

@@ -39,8 +39,7 @@ def _synthetic(n: int, d: int, seed: int = 0) -> np.ndarray:
 
 
 def candidates(k: int):
-    """The grid: precision x list slack (margin: the list holds k + margin candidates) x splits."""
-    margins = [-1] + sorted({2, -(-k // 4), -(-k // 2)})  # -1: the library's default slack
+    margins = sorted({max(0, k // 2), k, min(2 * k, 64 - k)})
     margins = [m for m in margins if k + m <= 64]
     for prec in ("fp16x3", "fp32"):
         for m in margins:
@@ -67,12 +66,12 @@ def time_config(x_dev, n: int, d: int, k: int, cfg: dict, outs, reps: int = 3) -
 
 def choose(results: list, tie: float = 0.03) -> dict:
     """The configuration to keep.  Those within ``tie`` (timing noise) of the fastest are ties:
-    take the default list slack (margin -1), then the shortest lists, then split FP16, then tuned
-    splits, then fewer splits, so the choice does not flip with the noise of one timing run."""
+    take the shortest lists (the fewest inserts, which is what a batch of many samples pays for),
+    then split FP16, then tuned splits, then fewer splits, so the choice does not flip with the
+    noise of one timing run."""
     fastest = min(r["ms"] for r in results)
     near = [r for r in results if r["ms"] <= (1.0 + tie) * fastest]
-    return min(near, key=lambda r: (r["margin"] != -1, r["margin"], r["precision"] != "fp16x3", r["col_splits"] != 0,
-                                    r["col_splits"]))
+    return min(near, key=lambda r: (r["margin"], r["precision"] != "fp16x3", r["col_splits"] != 0, r["col_splits"]))
 
 
 def tune_shape(n: int, d: int, k: int, *, reps: int = 3, seed: int = 0) -> dict:

@@ -174,6 +174,7 @@ def time_form(run, form: str, steps: int, warmup: int):
     ms = (t1 - t0) * 1000.0 / steps
     kern = {
         "prep": float(np.mean([s.prep_ms for s in stats])),
+        "order": float(np.mean([s.order_ms for s in stats])),
         "coarse": float(np.mean([s.coarse_ms for s in stats])),
         "rescore": float(np.mean([s.rescore_ms for s in stats])),
         "fallback": float(np.mean([s.fallback_ms for s in stats])),
@@ -280,7 +281,8 @@ def main():
                          "short_rows": int(st.short_rows),
                          "coarse_err_max": float(st.coarse_err_max), "coarse_eps": float(st.coarse_eps),
                          "col_splits": int(st.col_splits), "candidates": int(st.candidates),
-                         "inserts": int(st.inserts), "rowscan_rows": int(st.rowscan_rows), "path": int(st.path)}
+                         "inserts": int(st.inserts), "rowscan_rows": int(st.rowscan_rows), "path": int(st.path),
+                         "tile_pairs": int(st.blocks_total), "tile_pairs_skipped": int(st.blocks_skipped)}
         outs[form] = run.outputs()
     head = forms[-1]
     ref = outs[forms[0]]

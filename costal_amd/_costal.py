@@ -17,8 +17,10 @@ The ``matmul`` argument names the coarse-tile backend. On this build it is an MF
 f32) or ``"mfma-f16x3"`` (FP16 hi/lo split tiles, three FP16 MFMAs, an FP32-class error bound; the
 default); ``"gemm"`` is accepted as the reference's portable name and means ``"mfma-f32"``;
 ``"accelerate"`` is unavailable here, as on the reference's Linux builds. The output never depends
-on the backend. ``sort_by_norm``, ``prune`` and ``n_threads`` are accepted for compatibility and
-cannot change the output (cosine selection never prunes; the GPU path has no host threads).
+on the backend. ``sort_by_norm`` and ``n_threads`` are accepted for compatibility and cannot
+change the output (the GPU path has its own working order and no host threads); ``prune`` switches
+the fused scan's exact tile pruning (on by default, as the reference's knob), which cannot change
+the output either.
 """
 
 from __future__ import annotations
@@ -90,7 +92,8 @@ def _call(fn, *args, stats=None):
 
 def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None, matmul=None,
                sort_by_norm=True, prune=True, n_threads=None, row_threshold=None,
-               weights=False, col_splits=None, threshold_model=None, threshold_bf16=True):
+               weights=False, col_splits=None, threshold_model=None, threshold_bf16=True,
+               prune_clusters=None):
     """Exact kNN: ``((idx int32[n,k], cos f64[n,k], norms f64[n]), stats)``.
 
     Extensions beyond the reference signature (keyword-only, default off):
@@ -99,6 +102,9 @@ def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None
     ``col_splits`` -- number of candidate-range splits (speed only; 0/None = tuned).
     ``threshold_model`` -- a :class:`costal_amd.learned.ThresholdModel`; its thresholds are computed
     on the GPU inside the call (BF16 unless ``threshold_bf16=False``); the result stays exact.
+    ``prune_clusters`` -- k-means centers of the locality order of the pruned scan (0/None = about
+    n/3072).  ``prune`` keeps the reference's meaning of "skip what provably cannot qualify": here
+    candidate tiles whose cosine bound is below every list's threshold (speed only).
     """
     x = _matrix_f32(x)
     n, d = x.shape
@@ -118,6 +124,8 @@ def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None
     p.row_block = int(row_block or 0)
     p.col_block = int(col_block or 0)
     p.col_splits = int(col_splits or 0)
+    p.no_prune = 0 if prune else 1
+    p.prune_clusters = int(prune_clusters or 0)
     th = None
     if row_threshold is not None:
         th = np.ascontiguousarray(row_threshold, dtype=np.float32)
@@ -160,7 +168,7 @@ def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None
         "device_ms": float(st.device_ms),
         "coarse_err_max": float(st.coarse_err_max),
         "coarse_eps": float(st.coarse_eps),
-        "kernel_ms": {"prep": st.prep_ms, "coarse": st.coarse_ms, "rescore": st.rescore_ms,
+        "kernel_ms": {"prep": st.prep_ms, "order": st.order_ms, "coarse": st.coarse_ms, "rescore": st.rescore_ms,
                       "fallback": st.fallback_ms},
     }
     if w is not None:

@@ -54,11 +54,13 @@ class KnnParams(C.Structure):
         ("row_block", C.c_uint64),
         ("col_block", C.c_uint64),
         ("col_splits", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("no_prune", C.c_uint32),
         ("row_threshold", C.c_void_p),
         ("mlp_params", C.c_void_p),
         ("mlp_offset", C.c_float),
         ("mlp_bf16", C.c_uint32),
+        ("prune_clusters", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -84,6 +86,7 @@ class KnnStats(C.Structure):
         ("nonfinite_sample", C.c_uint64),
         ("coarse_err_max", C.c_double),
         ("coarse_eps", C.c_double),
+        ("order_ms", C.c_double),
     ]
 
 

@@ -32,6 +32,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "order.h"
 #include "scan.h"
 
 #include <stdarg.h>
@@ -1211,6 +1212,8 @@ struct Plan {
   uint32_t QB, TC;  // queries per workgroup, candidates per tile
   uint32_t nsplit, cols_per_split;
   uint32_t dp32, dp16, n_pad;
+  bool prune;       // first pass in the locality order with tile pruning (order.h)
+  costal::OrderShape order;
 };
 
 using costal::CoarseFn;
@@ -1295,6 +1298,16 @@ Plan make_plan(const Device& D, uint64_t n, uint64_t d, uint64_t k, const costal
   P.cols_per_split = tiles_per * P.TC;
   P.nsplit = (ntile + tiles_per - 1) / tiles_per;
   if (P.nsplit < 1) P.nsplit = 1;
+  // Pruned first pass: the splits take turns over each block's tiles in bound order (at most one
+  // split per candidate tile of PRUNE_TILE rows).
+  P.prune = false;
+  if (P.fused && !p.no_prune) {
+    P.order = costal::order_shape((uint32_t)n, P.dp32, P.QB, p.prune_clusters);
+    if (costal::order_supported(P.order)) {
+      P.prune = true;
+      P.nsplit = std::min<uint32_t>(P.nsplit, P.order.npt);
+    }
+  }
   P.wide = !P.fused && p.select == COSTAL_SELECT_COSINE && P.kp <= FUSED_KP_MAX;
   if (P.wide) {
     // rows of a multiple of 32 features; score blocks of up to 512 MB; the scan's candidate
@@ -1386,7 +1399,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   // split FP16 with long lists (measured on 86,864 x 39 and 396,460 x 15; FP16 and split FP16 with
   // k = 10 lists were slower).
   uint32_t units_per_wg = 0, grid_bal = 0;
-  if (P.fused && p.col_splits == 0 && (P.prec == PREC_F32 || (P.prec == PREC_F16X3 && P.KP >= 32))) {
+  if (P.fused && !P.prune && p.col_splits == 0 && (P.prec == PREC_F32 || (P.prec == PREC_F16X3 && P.KP >= 32))) {
     CoarseFn f0 = coarse_select(P.prec, P.J, P.KP);
     int per_cu = 0;
     if (f0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(f0),
@@ -1420,6 +1433,10 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     Carve c(nullptr);
     c.take<float>((size_t)P.n_pad * P.dp32);
     if (need16) c.take<_Float16>((size_t)P.n_pad * row16);
+    if (P.prune) {
+      if (!need16 || P.prec == PREC_F16) c.take<float>((size_t)P.n_pad * P.dp32);
+      c.take<uint8_t>(costal::order_bytes(P.order));
+    }
     c.take<double>(n);
     c.take<uint32_t>(nq_entries);
     c.take<float>(nq_entries);
@@ -1436,6 +1453,10 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     return c.off + 256;
   };
   size_t bytes = workspace_bytes();
+  if (fixed && bytes > fixed->cap && P.prune) {
+    P.prune = false;  // the bound matrix does not fit: scan every tile
+    bytes = workspace_bytes();
+  }
   if (fixed && bytes > fixed->cap && (P.fused || P.wide)) {
     costal_knn_params p1 = p;
     p1.col_splits = 1;
@@ -1461,6 +1482,13 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   Carve c(fixed ? fixed->base : D.ws);
   float* xs32 = c.take<float>((size_t)P.n_pad * P.dp32);
   _Float16* xs16 = need16 ? c.take<_Float16>((size_t)P.n_pad * row16) : nullptr;
+  // pruned pass: xs16 holds the working order; the f32 copy in working order when the scan reads it
+  float* xs32w = nullptr;
+  costal::OrderBufs ob{};
+  if (P.prune) {
+    if (!need16 || P.prec == PREC_F16) xs32w = c.take<float>((size_t)P.n_pad * P.dp32);
+    ob = costal::order_carve(c.take<uint8_t>(costal::order_bytes(P.order)), P.order);
+  }
   double* norms_ws = c.take<double>(n);
   uint32_t* cand_idx = c.take<uint32_t>(nq_entries);
   float* cand_sc = c.take<float>(nq_entries);
@@ -1488,7 +1516,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
                                                                            counters);
   else
     prep_kernel<<<(P.n_pad + 255) / 256, 256, 0, s>>>(x, n, d, P.n_pad, P.dp32, P.dp16, split16, norms, xs32,
-                                                      xs16, counters);
+                                                      P.prune ? nullptr : xs16, counters);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(D.evk[1], s));
   uint32_t bad = NONE;
@@ -1497,6 +1525,14 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   if (bad != NONE) {
     if (st) st->nonfinite_row = bad;
     return fail(COSTAL_E_NON_FINITE, "row %u contains NaN or inf", bad);
+  }
+  if (P.prune) {
+    // the locality order, the working copies in it, and every block's tiles in bound order
+    HIP_TRY(hipEventRecord(D.evk[6], s));
+    HIP_TRY(costal::order_rows(s, xs32, P.order, ob));
+    HIP_TRY(costal::order_permute(s, xs32, P.order, ob, P.n_pad, P.dp16, split16, xs32w, xs16));
+    HIP_TRY(costal::order_bounds(s, xs32, P.order, ob));
+    HIP_TRY(hipEventRecord(D.evk[7], s));
   }
 
   const double eps32 = fp32_eps(d);
@@ -1530,9 +1566,22 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     CoarseFn fn = coarse_select(P.prec, P.J, P.KP);
     if (!fn) return fail(COSTAL_E_UNSUPPORTED, "no coarse kernel for d=%u", d);
     const float eps_pre = P.prec == PREC_F16 ? (float)(fp16_eps(d) + eps32 + std::ldexp(1.0, -22)) : 0.0f;
-    const uint8_t* xsq = need16 ? (const uint8_t*)xs16 : (const uint8_t*)xs32;
+    const uint8_t* xsq = need16 ? (const uint8_t*)xs16 : (const uint8_t*)(xs32w ? xs32w : xs32);
+    const float* xs32q = xs32w ? xs32w : xs32;  // the f32 rows the FP16 form re-scores, in scan order
     const double eps_list = P.prec == PREC_F16X3 ? fp16x3_eps(d) : eps32;
     const float* row_th = p.row_threshold;
+    unsigned long long* scanned = reinterpret_cast<unsigned long long*>(counters + 10);
+    // the first pass prunes; the rescans of failed rows read the same working copies, unpruned
+    PruneArgs pr_first{}, pr_rescan{};
+    if (P.prune) {
+      pr_rescan.perm = ob.perm;
+      pr_rescan.iperm = ob.iperm;
+      pr_first = pr_rescan;
+      pr_first.seq = ob.seq;
+      pr_first.sb = ob.sb;
+      pr_first.npt = P.order.npt;
+      pr_first.scanned = scanned;
+    }
     if (learned) {
       // learned thresholds, lowered by twice the list error bound so that a row whose k-th
       // similarity the model bounds correctly also clears the certificate
@@ -1548,8 +1597,8 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
                          uint32_t nsplit, uint32_t cols_per, uint32_t units, uint32_t* fbr, double* fbl,
                          uint32_t* fbc, uint32_t* shc) -> int {
       const dim3 grid = units ? dim3(grid_bal, 1) : dim3((nq + P.QB - 1) / P.QB, nsplit);
-      f<<<grid, 64 * P.NW, 0, s>>>(xsq, xs32, P.dp32, n, (int)kp, rows, nq, th, cols_per, units, eps_pre,
-                                   cand_idx, cand_sc, cand_cnt, cand_floor, inserts);
+      f<<<grid, 64 * P.NW, 0, s>>>(xsq, xs32q, P.dp32, n, (int)kp, rows, nq, th, cols_per, units, eps_pre,
+                                   cand_idx, cand_sc, cand_cnt, cand_floor, inserts, rows ? pr_rescan : pr_first);
       HIP_TRY(hipGetLastError());
       if (!rows) HIP_TRY(hipEventRecord(D.evk[3], s));
       const int all_admitted = (nsplit == 1 && kp >= n - 1 && !th) ? 1 : 0;
@@ -1660,8 +1709,15 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     st->short_rows = cnt[2];
     st->row_block = (P.fused || P.wide) ? P.QB : 1;
     st->col_block = (P.fused || P.wide) ? P.TC : 256;
-    st->blocks_skipped = 0;
-    st->blocks_total = (P.fused || P.wide) ? (uint64_t)((n + P.QB - 1) / P.QB) * ((n + P.TC - 1) / P.TC) : n;
+    if (P.prune) {
+      const uint64_t total = (uint64_t)P.order.nblk * P.order.npt;
+      const uint64_t done = (uint64_t)cnt[10] | ((uint64_t)cnt[11] << 32);
+      st->blocks_total = total;
+      st->blocks_skipped = total > done ? total - done : 0;
+    } else {
+      st->blocks_skipped = 0;
+      st->blocks_total = (P.fused || P.wide) ? (uint64_t)((n + P.QB - 1) / P.QB) * ((n + P.TC - 1) / P.TC) : n;
+    }
     st->candidates = P.kp;
     st->col_splits = (P.fused || P.wide) ? P.nsplit : 1;
     st->path = P.fused ? 0 : P.wide ? 2 : 1;
@@ -1676,6 +1732,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     st->coarse_ms = span(2, 3);
     st->rescore_ms = span(3, 4);
     st->fallback_ms = span(4, 5);
+    st->order_ms = P.prune ? span(6, 7) : 0.0;
     st->coarse_err_max = slots_max(cnt + 64);
     st->coarse_eps = P.fused ? (P.prec == PREC_F16X3 ? fp16x3_eps(d) : fp32_eps(d)) : P.wide ? fp32_eps(d) : 0.0;
   }
@@ -2057,7 +2114,8 @@ int costal_knn_plan(uint64_t n, uint64_t d, const costal_knn_params* params, cos
   stats->col_block = (P.fused || P.wide) ? P.TC : 256;
   stats->candidates = P.kp;
   stats->col_splits = (P.fused || P.wide) ? P.nsplit : 1;
-  stats->blocks_total = (P.fused || P.wide) ? (uint64_t)((n + P.QB - 1) / P.QB) * ((n + P.TC - 1) / P.TC) : n;
+  stats->blocks_total = P.prune ? (uint64_t)P.order.nblk * P.order.npt
+                       : (P.fused || P.wide) ? (uint64_t)((n + P.QB - 1) / P.QB) * ((n + P.TC - 1) / P.TC) : n;
   stats->coarse_eps = P.fused ? (P.prec == PREC_F16X3 ? fp16x3_eps((uint32_t)d) : fp32_eps((uint32_t)d))
                               : P.wide ? fp32_eps((uint32_t)d) : 0.0;
   return 0;

@@ -1,0 +1,75 @@
+// order.h -- locality order of the rows and the similarity bounds that let the fused scan skip
+// candidate tiles (exact pruning for cosine selection).
+//
+// The reference scans every column block for cosine selection and prunes only for Tanimoto, by
+// norm windows over a norm-sorted working order (crates/costal-core/src/knn.rs:413-436, 603-644,
+// 663-678).  Here the working order groups similar rows instead: spherical k-means on a sample of
+// the normalized rows, every row assigned to its nearest center, rows stably sorted by center.
+// Every group of consecutive working rows (a query block, a candidate tile) gets a cone: the
+// normalized mean of its rows and the largest angle from it to any of them.  For a query block Q
+// and a candidate tile T with cone angles aQ, aT and center angle th, no pair (q, c) has a cosine
+// above cos(max(0, th - aQ - aT)) (spherical triangle inequality).  The scan visits each block's
+// tiles in decreasing bound order and stops at the first bound below every query's admission
+// threshold: the candidates left out score below every list floor, so the certificate of the
+// rescoring (knn.rs:704-714) holds unchanged and the result is the brute force bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace costal {
+
+// Rows of a candidate tile of the bound matrix (a whole number of the scan's LDS tiles).
+constexpr uint32_t PRUNE_TILE = 512;
+// Largest tile count per query block the bound sort handles (bitonic sort in LDS).
+constexpr uint32_t PRUNE_MAX_TILES = 4096;
+// Bytes of LDS the k-means kernels hold per center feature (f32 center + i64 sum).
+constexpr uint32_t KM_LDS_BYTES = 12;
+constexpr uint32_t KM_LDS_MAX = 60 * 1024;
+
+struct OrderBufs {
+  uint32_t* perm;    // n: working row -> original row
+  uint32_t* iperm;   // n: original row -> working row
+  float* centers;    // C x dp
+  long long* sums;   // C x dp, fixed-point center sums (zero between calls)
+  uint32_t* counts;  // C
+  uint32_t* assign;  // n
+  uint32_t* rank;    // n
+  uint32_t* hist;    // (C + 1) x row blocks of 256, then their exclusive prefix sums
+  float* qcen;       // query blocks x dp
+  double* qang;      // query blocks
+  float* tcen;       // candidate tiles x dp
+  double* tang;      // candidate tiles
+  uint32_t* seq;     // query blocks x tiles: tile ids in decreasing bound order
+  float* sb;         // query blocks x tiles: the bounds in that order
+};
+
+struct OrderShape {
+  uint32_t n, dp;      // rows, f32 working-copy width
+  uint32_t C;          // k-means centers
+  uint32_t sample;     // rows of the k-means sample (spread evenly over all rows)
+  uint32_t iters;      // k-means iterations on the sample
+  uint32_t qb;         // rows of a query block
+  uint32_t nblk, npt;  // query blocks, candidate tiles of PRUNE_TILE rows
+};
+
+// The default shape for n rows of dp features with query blocks of qb rows.
+OrderShape order_shape(uint32_t n, uint32_t dp, uint32_t qb, uint32_t clusters);
+// Whether the bound matrix fits the sort (and the scan has something to gain).
+bool order_supported(const OrderShape& o);
+// Device bytes of the buffers (each carved on 256-byte boundaries).
+size_t order_bytes(const OrderShape& o);
+// Carve the buffers out of `base` (order_bytes(o) bytes).
+OrderBufs order_carve(void* base, const OrderShape& o);
+
+// Working order of the rows of xs32 (normalized f32 rows, dp per row, original order): perm/iperm.
+hipError_t order_rows(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b);
+// Working copies in the working order, rows [n, n_pad) zero: xs32w (f32, dp32) and/or xs16w
+// (FP16, dp16 per row, or hi then lo rows of dp16 when split16).  Either may be null.
+hipError_t order_permute(hipStream_t s, const float* xs32, const OrderShape& o, const OrderBufs& b, uint32_t n_pad,
+                         uint32_t dp16, int split16, float* xs32w, _Float16* xs16w);
+// Cones of the query blocks and candidate tiles, the bound matrix and its per-block order.
+hipError_t order_bounds(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b);
+
+}  // namespace costal

@@ -1,0 +1,503 @@
+// order.hip -- locality order of the rows, cones of row groups and the per-block bound order that
+// the pruned fused scan follows (see order.h for the argument).  Every kernel here is deterministic:
+// the k-means sums are fixed-point integers (order-independent), ties go to the lowest index, and
+// the sort is stable, so a call's working order depends on its input only.
+#include "order.h"
+
+#include <math.h>
+
+#include <algorithm>
+#include <type_traits>
+
+#include "device_ids.h"
+
+namespace costal {
+namespace {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+constexpr double KM_SCALE = 1099511627776.0;  // 2^40: fixed-point scale of the center sums
+constexpr uint32_t NO_CENTER = 0xffffffffu;
+// Angle added to every cone: covers the distance between a normalized f32 row and the exact unit
+// vector of the original row (at most about 2^-24 rad) with a wide margin.
+constexpr double CONE_MARGIN = 1e-6;
+// Added to every bound: the f64 evaluation error of the bound and of the rescored cosine.
+constexpr double BOUND_MARGIN = 1e-6;
+
+// One row of dp floats into DPM registers (zero beyond dp).
+template <int DPM>
+__device__ __forceinline__ void load_row(const float* __restrict__ row, uint32_t dp, float (&v)[DPM]) {
+#pragma unroll
+  for (int f = 0; f < DPM; f += 4) {
+    if ((uint32_t)f < dp) {
+      const f32x4 q = *reinterpret_cast<const f32x4*>(row + f);
+      v[f] = q[0], v[f + 1] = q[1], v[f + 2] = q[2], v[f + 3] = q[3];
+    } else {
+      v[f] = v[f + 1] = v[f + 2] = v[f + 3] = 0.0f;
+    }
+  }
+}
+
+// The center (LDS, stride DPM) of largest inner product with v; ties to the lowest index; NO_CENTER
+// for a zero row.
+template <int DPM>
+__device__ __forceinline__ uint32_t nearest(const float (&v)[DPM], const float* __restrict__ s_cen, uint32_t C) {
+  float nz = 0.0f;
+#pragma unroll
+  for (int f = 0; f < DPM; ++f) nz = fmaxf(nz, fabsf(v[f]));
+  if (nz == 0.0f) return NO_CENTER;
+  float best = -INFINITY;
+  uint32_t bc = 0;
+  for (uint32_t c = 0; c < C; ++c) {
+    const float* cen = s_cen + c * DPM;
+    float s = 0.0f;
+#pragma unroll
+    for (int f = 0; f < DPM; ++f) s = __builtin_fmaf(v[f], cen[f], s);
+    if (s > best) {
+      best = s;
+      bc = c;
+    }
+  }
+  return bc;
+}
+
+// centers[c] = sample row floor(c * sample / C)
+__global__ void km_init_kernel(const float* __restrict__ xs32, uint32_t dp, uint32_t C, uint32_t sample,
+                               uint32_t stride, float* __restrict__ centers) {
+  const uint32_t c = bid_x(), f = tid_x();
+  if (f >= dp) return;
+  const uint64_t r = (uint64_t)c * sample / C * stride;
+  centers[(size_t)c * dp + f] = xs32[r * dp + f];
+}
+
+// One k-means assignment pass over the sample: every sample row adds itself (fixed point) to the
+// sum of its nearest center.
+template <int DPM>
+__global__ __launch_bounds__(256) void km_step_kernel(const float* __restrict__ xs32, uint32_t dp, uint32_t C,
+                                                      uint32_t sample, uint32_t stride,
+                                                      const float* __restrict__ centers, long long* __restrict__ sums,
+                                                      uint32_t* __restrict__ counts) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t km_lds[];
+  long long* s_sum = reinterpret_cast<long long*>(km_lds);
+  float* s_cen = reinterpret_cast<float*>(s_sum + (size_t)C * DPM);
+  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_cen + (size_t)C * DPM);
+  for (uint32_t e = tid_x(); e < C * DPM; e += 256) {
+    const uint32_t c = e / DPM, f = e % DPM;
+    s_sum[e] = 0;
+    s_cen[e] = f < dp ? centers[(size_t)c * dp + f] : 0.0f;
+  }
+  for (uint32_t c = tid_x(); c < C; c += 256) s_cnt[c] = 0;
+  __syncthreads();
+  const uint32_t r = bid_x() * 256 + tid_x();
+  if (r < sample) {
+    float v[DPM];
+    load_row<DPM>(xs32 + (size_t)r * stride * dp, dp, v);
+    const uint32_t c = nearest<DPM>(v, s_cen, C);
+    if (c != NO_CENTER) {
+#pragma unroll
+      for (int f = 0; f < DPM; ++f)
+        if (v[f] != 0.0f)
+          atomicAdd(reinterpret_cast<unsigned long long*>(&s_sum[c * DPM + f]),
+                    (unsigned long long)(long long)rint((double)v[f] * KM_SCALE));
+      atomicAdd(&s_cnt[c], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t e = tid_x(); e < C * DPM; e += 256) {
+    const uint32_t c = e / DPM, f = e % DPM;
+    if (f < dp && s_sum[e] != 0)
+      atomicAdd(reinterpret_cast<unsigned long long*>(&sums[(size_t)c * dp + f]), (unsigned long long)s_sum[e]);
+  }
+  for (uint32_t c = tid_x(); c < C; c += 256)
+    if (s_cnt[c]) atomicAdd(&counts[c], s_cnt[c]);
+}
+
+__global__ void km_clear_kernel(long long* __restrict__ sums, uint32_t* __restrict__ counts, uint32_t C, uint32_t dp) {
+  const uint32_t e = bid_x() * bdim_x() + tid_x();
+  if (e < C * dp) sums[e] = 0;
+  if (e < C) counts[e] = 0;
+}
+
+// New centers: the normalized means of their rows (a center with no rows keeps its place); the
+// sums and counts are cleared for the next pass.
+__global__ void km_update_kernel(uint32_t dp, uint32_t C, long long* __restrict__ sums, uint32_t* __restrict__ counts,
+                                 float* __restrict__ centers) {
+  const uint32_t c = bid_x() * bdim_x() + tid_x();
+  if (c >= C) return;
+  long long* s = sums + (size_t)c * dp;
+  if (counts[c] > 0) {
+    double nn = 0.0;
+    for (uint32_t f = 0; f < dp; ++f) nn += (double)s[f] * (double)s[f];
+    if (nn > 0.0) {
+      const double inv = 1.0 / sqrt(nn);
+      for (uint32_t f = 0; f < dp; ++f) centers[(size_t)c * dp + f] = (float)((double)s[f] * inv);
+    }
+  }
+  for (uint32_t f = 0; f < dp; ++f) s[f] = 0;
+  counts[c] = 0;
+}
+
+// Every row's center (zero rows: center C, sorted last), its rank among the rows of its 256-row
+// block with the same center, and the per-block center histogram, stored center-major
+// (hist[c * nb + block]) so that one exclusive scan gives every (center, block) its first slot.
+template <int DPM>
+__global__ __launch_bounds__(256) void km_assign_kernel(const float* __restrict__ xs32, uint32_t dp, uint32_t n,
+                                                        uint32_t C, const float* __restrict__ centers,
+                                                        uint32_t* __restrict__ assign, uint32_t* __restrict__ rank,
+                                                        uint32_t* __restrict__ hist) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t km_lds[];
+  float* s_cen = reinterpret_cast<float*>(km_lds);
+  uint32_t* s_h = reinterpret_cast<uint32_t*>(s_cen + (size_t)C * DPM);  // C + 1
+  __shared__ uint32_t s_a[256];
+  for (uint32_t e = tid_x(); e < C * DPM; e += 256) {
+    const uint32_t c = e / DPM, f = e % DPM;
+    s_cen[e] = f < dp ? centers[(size_t)c * dp + f] : 0.0f;
+  }
+  for (uint32_t c = tid_x(); c <= C; c += 256) s_h[c] = 0;
+  __syncthreads();
+  const uint32_t i = bid_x() * 256 + tid_x();
+  uint32_t a = NO_CENTER;
+  if (i < n) {
+    float v[DPM];
+    load_row<DPM>(xs32 + (size_t)i * dp, dp, v);
+    a = nearest<DPM>(v, s_cen, C);
+    if (a == NO_CENTER) a = C;
+    atomicAdd(&s_h[a], 1u);
+  }
+  s_a[tid_x()] = a;
+  __syncthreads();
+  if (i < n) {
+    uint32_t r = 0;
+    for (uint32_t t = 0; t < tid_x(); ++t) r += s_a[t] == a ? 1u : 0u;
+    assign[i] = a;
+    rank[i] = r;
+  }
+  const uint32_t nb = gdim_x();
+  for (uint32_t c = tid_x(); c <= C; c += 256) hist[(size_t)c * nb + bid_x()] = s_h[c];
+}
+
+// In-place exclusive prefix sum of m counts (one workgroup).
+__global__ __launch_bounds__(1024) void exclusive_scan_kernel(uint32_t* __restrict__ v, uint32_t m) {
+  __shared__ uint32_t s[1024];
+  const uint32_t t = tid_x();
+  const uint32_t per = (m + 1023) / 1024;
+  const uint32_t lo = std::min(m, t * per), hi = std::min(m, lo + per);
+  uint32_t sum = 0;
+  for (uint32_t e = lo; e < hi; ++e) sum += v[e];
+  s[t] = sum;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024; off <<= 1) {
+    const uint32_t add = t >= off ? s[t - off] : 0u;
+    __syncthreads();
+    s[t] += add;
+    __syncthreads();
+  }
+  uint32_t run = s[t] - sum;
+  for (uint32_t e = lo; e < hi; ++e) {
+    const uint32_t x = v[e];
+    v[e] = run;
+    run += x;
+  }
+}
+
+__global__ void km_scatter_kernel(const uint32_t* __restrict__ assign, const uint32_t* __restrict__ rank,
+                                  const uint32_t* __restrict__ first, uint32_t n, uint32_t nb,
+                                  uint32_t* __restrict__ perm, uint32_t* __restrict__ iperm) {
+  const uint32_t i = bid_x() * bdim_x() + tid_x();
+  if (i >= n) return;
+  const uint32_t pos = first[(size_t)assign[i] * nb + i / 256] + rank[i];
+  perm[pos] = i;
+  iperm[i] = pos;
+}
+
+// Working copies in working order (hi = fp16(v), lo = fp16(v - hi), as the original-order prep).
+__global__ void permute_kernel(const float* __restrict__ xs32, const uint32_t* __restrict__ perm, uint32_t n,
+                               uint32_t n_pad, uint32_t dp, uint32_t dp16, int split16, float* __restrict__ xs32w,
+                               _Float16* __restrict__ xs16w) {
+  const uint32_t w = bid_x() * bdim_x() + tid_x();
+  if (w >= n_pad) return;
+  const float* src = w < n ? xs32 + (size_t)perm[w] * dp : nullptr;
+  if (xs32w)
+    for (uint32_t t = 0; t < dp; t += 4)
+      *reinterpret_cast<f32x4*>(xs32w + (size_t)w * dp + t) = src ? *reinterpret_cast<const f32x4*>(src + t) : f32x4{};
+  if (xs16w) {
+    const uint32_t s16 = split16 ? 2 * dp16 : dp16;
+    for (uint32_t t = 0; t < dp16; t += 8) {
+      f16x8 hi, lo;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const float v = (src && t + c < dp) ? src[t + c] : 0.0f;
+        hi[c] = (_Float16)v;
+        lo[c] = (_Float16)(v - (float)hi[c]);
+      }
+      *reinterpret_cast<f16x8*>(xs16w + (size_t)w * s16 + t) = hi;
+      if (split16) *reinterpret_cast<f16x8*>(xs16w + (size_t)w * s16 + dp16 + t) = lo;
+    }
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+  // fixed butterfly: the same order on every run
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  return x;
+}
+
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = fmin(x, __shfl_xor(x, off));
+  return x;
+}
+
+// Cone of working rows [g*G, min(n, g*G + G)) (one wave): center = the normalized f64 sum of the
+// rows, rounded to f32; angle = the largest angle between that f32 center and any row, in f64 with
+// both vectors' own norms, plus CONE_MARGIN; pi when the sum is zero (the cone covers everything).
+template <int DPM>
+__global__ __launch_bounds__(64) void cone_kernel(const float* __restrict__ xs32, uint32_t dp,
+                                                  const uint32_t* __restrict__ perm, uint32_t n, uint32_t G,
+                                                  float* __restrict__ cen, double* __restrict__ ang) {
+  __shared__ float s_c[DPM];
+  const uint32_t g = bid_x(), l = tid_x();
+  const uint32_t w0 = g * G, w1 = std::min(n, w0 + G);
+  double acc[DPM];
+#pragma unroll
+  for (int f = 0; f < DPM; ++f) acc[f] = 0.0;
+  for (uint32_t w = w0 + l; w < w1; w += 64) {
+    float v[DPM];
+    load_row<DPM>(xs32 + (size_t)perm[w] * dp, dp, v);
+#pragma unroll
+    for (int f = 0; f < DPM; ++f) acc[f] += (double)v[f];
+  }
+  double nn = 0.0;
+#pragma unroll
+  for (int f = 0; f < DPM; ++f) {
+    acc[f] = wave_sum(acc[f]);
+    nn += acc[f] * acc[f];
+  }
+  const double nrm = sqrt(nn);
+  if (!(nrm > 0.0)) {
+    if (l < dp) cen[(size_t)g * dp + l] = 0.0f;
+    if (l == 0) ang[g] = M_PI;
+    return;
+  }
+  if (l < DPM) {
+    double mine = 0.0;
+#pragma unroll
+    for (int f = 0; f < DPM; ++f) mine = (uint32_t)f == l ? acc[f] : mine;
+    const float c = (float)(mine / nrm);
+    s_c[l] = c;
+    if (l < dp) cen[(size_t)g * dp + l] = c;
+  }
+  __syncthreads();
+  double cn = 0.0;
+#pragma unroll
+  for (int f = 0; f < DPM; ++f) cn += (double)s_c[f] * (double)s_c[f];
+  cn = sqrt(cn);
+  double m = 2.0;
+  for (uint32_t w = w0 + l; w < w1; w += 64) {
+    float v[DPM];
+    load_row<DPM>(xs32 + (size_t)perm[w] * dp, dp, v);
+    double dot = 0.0, vn = 0.0;
+#pragma unroll
+    for (int f = 0; f < DPM; ++f) {
+      dot += (double)s_c[f] * (double)v[f];
+      vn += (double)v[f] * (double)v[f];
+    }
+    // a zero row has cosine 0 with everything: it widens the cone to at least pi/2
+    const double c = vn > 0.0 ? dot / (cn * sqrt(vn)) : 0.0;
+    m = fmin(m, c);
+  }
+  m = wave_min(m);
+  if (l == 0) ang[g] = acos(fmax(-1.0, fmin(1.0, m))) + CONE_MARGIN;
+}
+
+// Bound of every candidate tile for query block bid_x(), sorted in decreasing order (ties: lower
+// tile first): seq[b*npt + r] = tile of rank r, sb[b*npt + r] = its bound, an f32 rounded up.
+__global__ __launch_bounds__(256) void bound_sort_kernel(const float* __restrict__ qcen, const double* __restrict__ qang,
+                                                         const float* __restrict__ tcen, const double* __restrict__ tang,
+                                                         uint32_t dp, uint32_t npt, uint32_t p2, uint32_t* __restrict__ seq,
+                                                         float* __restrict__ sb) {
+  __shared__ float s_key[PRUNE_MAX_TILES];
+  __shared__ uint32_t s_id[PRUNE_MAX_TILES];
+  __shared__ double s_q[64];
+  const uint32_t b = bid_x(), t0 = tid_x();
+  if (t0 < 64) s_q[t0] = t0 < dp ? (double)qcen[(size_t)b * dp + t0] : 0.0;
+  __syncthreads();
+  double qn = 0.0;
+  for (uint32_t f = 0; f < dp; ++f) qn += s_q[f] * s_q[f];
+  qn = sqrt(qn);
+  const double aq = qang[b];
+  for (uint32_t t = t0; t < p2; t += 256) {
+    float key = -INFINITY;
+    if (t < npt) {
+      const float* tc = tcen + (size_t)t * dp;
+      double dot = 0.0, tn = 0.0;
+      for (uint32_t f = 0; f < dp; ++f) {
+        const double c = (double)tc[f];
+        dot += s_q[f] * c;
+        tn += c * c;
+      }
+      const double a = aq + tang[t];
+      double bound = 1.0;
+      if (a < M_PI && qn > 0.0 && tn > 0.0) {
+        const double th = acos(fmax(-1.0, fmin(1.0, dot / (qn * sqrt(tn)))));
+        if (th > a) bound = cos(th - a);
+      }
+      bound += BOUND_MARGIN;
+      key = (float)bound;
+      if ((double)key < bound) key = nextafterf(key, INFINITY);
+    }
+    s_key[t] = key;
+    s_id[t] = t;
+  }
+  __syncthreads();
+  // bitonic sort: larger key first, then lower id
+  for (uint32_t k = 2; k <= p2; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = t0; i < p2; i += 256) {
+        const uint32_t o = i ^ j;
+        if (o > i) {
+          const float ka = s_key[i], kb = s_key[o];
+          const uint32_t ia = s_id[i], ib = s_id[o];
+          const bool a_first = ka > kb || (ka == kb && ia < ib);
+          const bool desc = (i & k) == 0;
+          if (desc != a_first) {
+            s_key[i] = kb, s_key[o] = ka;
+            s_id[i] = ib, s_id[o] = ia;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t t = t0; t < npt; t += 256) {
+    seq[(size_t)b * npt + t] = s_id[t];
+    sb[(size_t)b * npt + t] = s_key[t];
+  }
+}
+
+template <class F>
+hipError_t by_dpm(uint32_t dp, F f) {
+  if (dp <= 16) return f(std::integral_constant<int, 16>{});
+  if (dp <= 32) return f(std::integral_constant<int, 32>{});
+  if (dp <= 48) return f(std::integral_constant<int, 48>{});
+  if (dp <= 64) return f(std::integral_constant<int, 64>{});
+  return hipErrorInvalidValue;
+}
+
+inline uint32_t dpm_of(uint32_t dp) { return dp <= 16 ? 16 : dp <= 32 ? 32 : dp <= 48 ? 48 : 64; }
+
+}  // namespace
+
+OrderShape order_shape(uint32_t n, uint32_t dp, uint32_t qb, uint32_t clusters) {
+  OrderShape o{};
+  o.n = n;
+  o.dp = dp;
+  uint32_t C = clusters;
+  if (!C) {
+    // about 3,000 rows per center, a power of two in [16, 256]
+    C = 16;
+    while (C < 256 && (uint64_t)C * 2 * 3072 <= n) C *= 2;
+  }
+  const uint32_t cmax = KM_LDS_MAX / (KM_LDS_BYTES * dpm_of(dp)) - 1;
+  o.C = std::max<uint32_t>(1, std::min<uint32_t>({C, cmax, std::max<uint32_t>(1, n / 4)}));
+  o.sample = std::min<uint32_t>(n, 32768);
+  o.iters = 5;
+  o.qb = qb;
+  o.nblk = (n + qb - 1) / qb;
+  o.npt = (n + PRUNE_TILE - 1) / PRUNE_TILE;
+  return o;
+}
+
+bool order_supported(const OrderShape& o) { return o.dp <= 64 && o.npt >= 8 && o.npt <= PRUNE_MAX_TILES; }
+
+namespace {
+struct Carver {
+  uint8_t* base;
+  size_t off = 0;
+  template <class T>
+  T* take(size_t count) {
+    off = (off + 255) & ~(size_t)255;
+    T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+    off += count * sizeof(T);
+    return p;
+  }
+};
+
+OrderBufs carve(Carver& c, const OrderShape& o) {
+  const size_t nb = (o.n + 255) / 256;
+  OrderBufs b{};
+  b.perm = c.take<uint32_t>(o.n);
+  b.iperm = c.take<uint32_t>(o.n);
+  b.centers = c.take<float>((size_t)o.C * o.dp);
+  b.sums = c.take<long long>((size_t)o.C * o.dp);
+  b.counts = c.take<uint32_t>(o.C);
+  b.assign = c.take<uint32_t>(o.n);
+  b.rank = c.take<uint32_t>(o.n);
+  b.hist = c.take<uint32_t>((size_t)(o.C + 1) * nb);
+  b.qcen = c.take<float>((size_t)o.nblk * o.dp);
+  b.qang = c.take<double>(o.nblk);
+  b.tcen = c.take<float>((size_t)o.npt * o.dp);
+  b.tang = c.take<double>(o.npt);
+  b.seq = c.take<uint32_t>((size_t)o.nblk * o.npt);
+  b.sb = c.take<float>((size_t)o.nblk * o.npt);
+  return b;
+}
+}  // namespace
+
+size_t order_bytes(const OrderShape& o) {
+  Carver c{nullptr};
+  carve(c, o);
+  return c.off + 256;
+}
+
+OrderBufs order_carve(void* base, const OrderShape& o) {
+  Carver c{reinterpret_cast<uint8_t*>(base)};
+  return carve(c, o);
+}
+
+hipError_t order_rows(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b) {
+  const uint32_t stride = std::max<uint32_t>(1, o.n / o.sample);
+  const uint32_t nb = (o.n + 255) / 256;
+  return by_dpm(o.dp, [&](auto dpm) -> hipError_t {
+    constexpr int DPM = decltype(dpm)::value;
+    km_init_kernel<<<o.C, 64, 0, s>>>(xs32, o.dp, o.C, o.sample, stride, b.centers);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // the sums start at zero (a kernel, not a memset: callers may capture the stream); the update
+    // kernel clears them after every pass
+    km_clear_kernel<<<(o.C * o.dp + 255) / 256, 256, 0, s>>>(b.sums, b.counts, o.C, o.dp);
+    const size_t lds_step = (size_t)o.C * DPM * 12 + o.C * 4;
+    for (uint32_t it = 0; it < o.iters; ++it) {
+      km_step_kernel<DPM><<<(o.sample + 255) / 256, 256, lds_step, s>>>(xs32, o.dp, o.C, o.sample, stride, b.centers,
+                                                                          b.sums, b.counts);
+      km_update_kernel<<<(o.C + 63) / 64, 64, 0, s>>>(o.dp, o.C, b.sums, b.counts, b.centers);
+    }
+    const size_t lds_assign = (size_t)o.C * DPM * 4 + (o.C + 1) * 4;
+    km_assign_kernel<DPM><<<nb, 256, lds_assign, s>>>(xs32, o.dp, o.n, o.C, b.centers, b.assign, b.rank, b.hist);
+    exclusive_scan_kernel<<<1, 1024, 0, s>>>(b.hist, (o.C + 1) * nb);
+    km_scatter_kernel<<<nb, 256, 0, s>>>(b.assign, b.rank, b.hist, o.n, nb, b.perm, b.iperm);
+    return hipGetLastError();
+  });
+}
+
+hipError_t order_permute(hipStream_t s, const float* xs32, const OrderShape& o, const OrderBufs& b, uint32_t n_pad,
+                         uint32_t dp16, int split16, float* xs32w, _Float16* xs16w) {
+  permute_kernel<<<(n_pad + 255) / 256, 256, 0, s>>>(xs32, b.perm, o.n, n_pad, o.dp, dp16, split16, xs32w, xs16w);
+  return hipGetLastError();
+}
+
+hipError_t order_bounds(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b) {
+  return by_dpm(o.dp, [&](auto dpm) -> hipError_t {
+    constexpr int DPM = decltype(dpm)::value;
+    cone_kernel<DPM><<<o.nblk, 64, 0, s>>>(xs32, o.dp, b.perm, o.n, o.qb, b.qcen, b.qang);
+    cone_kernel<DPM><<<o.npt, 64, 0, s>>>(xs32, o.dp, b.perm, o.n, PRUNE_TILE, b.tcen, b.tang);
+    uint32_t p2 = 1;
+    while (p2 < o.npt) p2 <<= 1;
+    bound_sort_kernel<<<o.nblk, 256, 0, s>>>(b.qcen, b.qang, b.tcen, b.tang, o.dp, o.npt, p2, b.seq, b.sb);
+    return hipGetLastError();
+  });
+}
+
+}  // namespace costal

@@ -10,6 +10,7 @@
 
 #include "../../include/costal_hip.h"
 #include "device_ids.h"
+#include "order.h"
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -32,9 +33,20 @@ struct BatchSample {
   double* wts;
 };
 
+// The pruned scan's inputs (order.h): the working order of the rows and, per query block, the
+// candidate tiles in decreasing bound order.  All null: candidates in row order, no pruning.
+struct PruneArgs {
+  const uint32_t* perm;          // working row -> original row (null: the identity)
+  const uint32_t* iperm;         // original row -> working row (for a row list)
+  const uint32_t* seq;           // query block b: tiles seq[b*npt ..] in decreasing bound order
+  const float* sb;               // their bounds (upper bounds of every query-candidate cosine)
+  uint32_t npt;                  // candidate tiles (PRUNE_TILE rows each)
+  unsigned long long* scanned;   // candidate tiles scanned (summed over blocks and splits)
+};
+
 typedef void (*CoarseFn)(const uint8_t*, const float*, uint32_t, uint32_t, int, const uint32_t*,
                          uint32_t, const float*, uint32_t, uint32_t, float, uint32_t*, float*, uint32_t*,
-                         float*, unsigned long long*);
+                         float*, unsigned long long*, PruneArgs);
 typedef void (*CoarseBatchFn)(const BatchSample*, const uint32_t*, uint32_t, uint32_t*, const uint8_t*,
                               const float*, uint32_t, float, uint32_t*, float*, uint32_t*, float*,
                               unsigned long long*, const unsigned long long*);
@@ -54,6 +66,7 @@ COSTAL_SCAN_DECL(2, 16) COSTAL_SCAN_DECL(2, 32) COSTAL_SCAN_DECL(2, 64)
 namespace {
 
 using costal::BatchSample;
+using costal::PruneArgs;
 
 constexpr int PREC_F32 = COSTAL_PREC_FP32;
 constexpr int PREC_F16 = COSTAL_PREC_FP16;
@@ -218,7 +231,8 @@ __device__ __forceinline__ void scan_block(
     uint32_t c_begin, uint32_t c_end,  // candidate range; c_begin a multiple of the tile, c_end <= n
     float eps_pre,                     // FP16 form: bound on |FP16 score - f32 score|
     uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc, uint32_t* __restrict__ cand_cnt,
-    float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
+    float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts,
+    const PruneArgs& pr, float* __restrict__ s_thr) {
   constexpr int QB = 32 * NW;
   constexpr int TC = TileShape<J>::TC;
   constexpr int ROWB = 32 * J;
@@ -234,7 +248,18 @@ __device__ __forceinline__ void scan_block(
   const int w = tid >> 6, l = tid & 63, h = l >> 5, l32 = l & 31;
   const uint32_t qq = qblock * QB + w * 32 + l32;
   const bool qvalid = qq < nq;
-  const uint32_t qg = qvalid ? (rowlist ? rowlist[qq] : qq) : 0;
+  // the query's original row (thresholds, output slot) and its row in the working copy (query
+  // chunks, self exclusion: candidates are working rows)
+  uint32_t qo = 0, qg = 0;
+  if (qvalid) {
+    if (rowlist) {
+      qo = rowlist[qq];
+      qg = pr.iperm ? pr.iperm[qo] : qo;
+    } else {
+      qg = qq;
+      qo = pr.perm ? pr.perm[qq] : qq;
+    }
+  }
 
   // the wave's queries, in registers for the whole launch
   uint4 qreg[J];
@@ -246,7 +271,7 @@ __device__ __forceinline__ void scan_block(
   // Empty list entries hold the lane's admission threshold (-inf without one; +inf for padding
   // queries, which so never take the insert path): the list minimum ls[0] is then always the
   // lane's threshold, and real entries (score above it) are the ones with an index.
-  float t0 = qvalid ? (tinit ? tinit[qg] : -INFINITY) : INFINITY;
+  float t0 = qvalid ? (tinit ? tinit[qo] : -INFINITY) : INFINITY;
   // a NaN threshold (caller data or a model with NaN parameters) admits everything: the test reads
   // the bits, since this code is built without NaN semantics
   if ((__float_as_uint(t0) & 0x7fffffffu) > 0x7f800000u) t0 = -INFINITY;
@@ -261,7 +286,33 @@ __device__ __forceinline__ void scan_block(
   float thp = -INFINITY;  // the partner list's threshold, refreshed once per tile
   uint32_t my_inserts = 0;
 
-  const int ntile = c_begin < c_end ? (int)((c_end - c_begin + TC - 1) / TC) : 0;
+  // Candidate tiles: the range [c_begin, c_end) in row order, or (pruned) the split's share of the
+  // query block's tiles in decreasing bound order (positions split, split + nsplit, ...), each
+  // R LDS tiles long, stopping at the first whose bound is below every list's threshold.
+  constexpr int R = (int)(costal::PRUNE_TILE / TC);
+  static_assert(R * TC == (int)costal::PRUNE_TILE, "LDS tiles per candidate tile");
+  const uint32_t* seq_b = pr.seq ? pr.seq + (size_t)qblock * pr.npt : nullptr;
+  const float* sb_b = pr.seq ? pr.sb + (size_t)qblock * pr.npt : nullptr;
+  // In the working order (rows grouped by similarity) a split of a row list takes every nsplit-th
+  // LDS tile instead of a contiguous range, so that a row's near neighbors, which sit close
+  // together, spread over all of its lists.
+  const bool interleave = !seq_b && pr.perm && nsplit > 1;
+  int ntile;
+  if (seq_b) {
+    c_end = n;
+    ntile = pr.npt > split ? (int)((pr.npt - split + nsplit - 1) / nsplit) * R : 0;
+  } else if (interleave) {
+    c_end = n;
+    const uint32_t all = (n + TC - 1) / TC;
+    ntile = all > split ? (int)((all - split + nsplit - 1) / nsplit) : 0;
+  } else {
+    ntile = c_begin < c_end ? (int)((c_end - c_begin + TC - 1) / TC) : 0;
+  }
+  auto tile_base = [&](int T) -> uint32_t {
+    if (seq_b) return seq_b[split + (uint32_t)(T / R) * nsplit] * costal::PRUNE_TILE + (uint32_t)(T % R) * TC;
+    if (interleave) return (split + (uint32_t)T * nsplit) * TC;
+    return c_begin + (uint32_t)T * TC;
+  };
 
   // Tiles go global -> registers -> LDS: the registers are loaded with tile t+2 while tile t is
   // scanned (right after the workgroup has stored tile t+1 and met) and stored during tile t+1, so
@@ -282,9 +333,9 @@ __device__ __forceinline__ void scan_block(
       R##C = *reinterpret_cast<const uint4*>(src_ + off_);                                       \
     }                                                                                             \
   }
-#define LOAD_REGS(R, T)                                                                           \
+#define LOAD_REGS(R, BASE)                                                                        \
   {                                                                                               \
-    const uint8_t* src_ = xsq + (size_t)(c_begin + (uint32_t)(T) * TC) * ROWB;                     \
+    const uint8_t* src_ = xsq + (size_t)(BASE) * ROWB;                                             \
     LOAD_ONE(R, 0) LOAD_ONE(R, 1) LOAD_ONE(R, 2) LOAD_ONE(R, 3)                                   \
     LOAD_ONE(R, 4) LOAD_ONE(R, 5) LOAD_ONE(R, 6) LOAD_ONE(R, 7)                                   \
   }
@@ -318,6 +369,27 @@ __device__ __forceinline__ void scan_block(
   constexpr int RD = 3;  // chunks of step G+RD are read at step G
   static_assert(NSTEP >= RD, "steps per tile");
   int t = 0;
+  // first candidate of tile t (in cur), of tile t+1 (in the staging registers) and of tile t+2
+  uint32_t b_cur = 0, b_nx = 0, b_nn = 0;
+  // Pruning, once per tile: every wave posts the lowest admission threshold of its lists (the
+  // list minimum or the partner's, whichever is higher: what the list turns away) before the
+  // barrier; after it, the workgroup's lowest threshold decides whether tile t+2 is scanned.
+  // Thresholds only rise and the bounds fall along the order, so the first tile below it ends the
+  // scan.  The slots alternate by tile, so a wave never overwrites a value another wave is still
+  // to read.
+  auto wave_threshold = [&]() __attribute__((always_inline)) {
+    float m = fmaxf(th, thp);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off));
+    if (l == 0) s_thr[(t & 1) * NW + w] = m;
+  };
+  auto prune_next = [&]() __attribute__((always_inline)) {
+    float tb = s_thr[(t & 1) * NW];
+#pragma unroll
+    for (int v = 1; v < NW; ++v) tb = fminf(tb, s_thr[(t & 1) * NW + v]);
+    const int T = t + 2;
+    if (T < ntile && T % R == 0 && sb_b[split + (uint32_t)(T / R) * nsplit] < tb) ntile = T;
+  };
   // compare the accumulators of step `p` of tile t with the lane's threshold; rare path: each lane
   // walks its own hits, sub-tile by sub-tile
   auto scan_step = [&](const f32x16 (&a)[GS], int p) __attribute__((always_inline)) {
@@ -340,7 +412,7 @@ __device__ __forceinline__ void scan_block(
           // tile loop (each would hold a register for the whole launch)
           uint32_t lane_off = 4 * h;
           __asm__ volatile("" : "+v"(lane_off));
-          const uint32_t cbase = c_begin + (uint32_t)t * TC + (uint32_t)(p * GS + u) * 32 + lane_off;
+          const uint32_t cbase = b_cur + (uint32_t)(p * GS + u) * 32 + lane_off;
           while (mask) {
             const int i = __builtin_ctz(mask);
             mask &= mask - 1;
@@ -368,8 +440,11 @@ __device__ __forceinline__ void scan_block(
     if (p == 0) thp = partner32(th);                                                              \
     if (p == NSTEP - RD) {                                                                        \
       STORE_REGS(ra, OTH);                                                                        \
+      if (seq_b) wave_threshold();                                                                \
       TILE_BARRIER();                                                                             \
-      LOAD_REGS(ra, t + 2 < ntile ? t + 2 : ntile - 1);                                           \
+      if (seq_b) prune_next();                                                                    \
+      b_nn = tile_base(t + 2 < ntile ? t + 2 : ntile - 1);                                        \
+      LOAD_REGS(ra, b_nn);                                                                        \
     }                                                                                             \
     if (p + RD < NSTEP) {                                                                         \
       READ_B((p + RD) & 1, CUR, p + RD)                                                           \
@@ -382,10 +457,12 @@ __device__ __forceinline__ void scan_block(
   if (ntile > 0) {
     uint8_t* cur = tile0;
     uint8_t* oth = tile1;
-    LOAD_REGS(ra, 0);
+    b_cur = tile_base(0);
+    LOAD_REGS(ra, b_cur);
     STORE_REGS(ra, cur);
     TILE_BARRIER();
-    LOAD_REGS(ra, ntile > 1 ? 1 : 0);
+    b_nx = tile_base(ntile > 1 ? 1 : 0);
+    LOAD_REGS(ra, b_nx);
     READ_B(0, cur, 0)
     READ_B(1, cur, 1)
     MFMA_STEP(0)
@@ -397,8 +474,11 @@ __device__ __forceinline__ void scan_block(
       uint8_t* const tmp = cur;
       cur = oth;
       oth = tmp;
+      b_cur = b_nx;
+      b_nx = b_nn;
     }
   }
+  if (seq_b && pr.scanned && tid == 0) atomicAdd(pr.scanned, (unsigned long long)((ntile + R - 1) / R));
 #undef RUN_TILE
 #undef MFMA_STEP
 #undef READ_B
@@ -417,13 +497,14 @@ __device__ __forceinline__ void scan_block(
   }
   const float thp_end = partner32(th);
   if (qvalid) {
-    const size_t slot = ((size_t)qq * nsplit + split) * 2 + h;
+    // lists by original row (a row list: by position in it), entries as original rows
+    const size_t slot = ((size_t)(rowlist ? qq : qo) * nsplit + split) * 2 + h;
     const size_t base = slot * (size_t)kp;
     uint32_t c = 0;
 #pragma unroll
     for (int e = 0; e < KPR; ++e) {
       if (e < kp && li[e] != NONE) {
-        cand_idx[base + c] = li[e];
+        cand_idx[base + c] = pr.perm ? pr.perm[li[e]] : li[e];
         cand_sc[base + c] = ls[e];
         ++c;
       }
@@ -445,9 +526,11 @@ __global__ __launch_bounds__(64 * NW) void coarse_kernel(
     const uint8_t* __restrict__ xsq, const float* __restrict__ xs32, uint32_t dp32, uint32_t n, int kp,
     const uint32_t* __restrict__ rowlist, uint32_t nq, const float* __restrict__ tinit, uint32_t cols_per_split,
     uint32_t units_per_wg, float eps_pre, uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc,
-    uint32_t* __restrict__ cand_cnt, float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts) {
+    uint32_t* __restrict__ cand_cnt, float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts,
+    PruneArgs pr) {
   __shared__ __attribute__((aligned(16))) uint8_t tile0[TileShape<J>::TILEB];
   __shared__ __attribute__((aligned(16))) uint8_t tile1[TileShape<J>::TILEB];
+  __shared__ float s_thr[2 * NW];
   constexpr uint32_t TC = TileShape<J>::TC;
   constexpr uint32_t QB = 32 * NW;
   const uint32_t ntile = (n + TC - 1) / TC;
@@ -479,7 +562,7 @@ __global__ __launch_bounds__(64 * NW) void coarse_kernel(
     }
     if (!first) __syncthreads();  // every wave is done with the previous piece's tiles
     scan_block<PREC, J, KPR, NW>(tile0, tile1, xsq, xs32, dp32, n, kp, rowlist, nq, qb, tinit, split, nspl, c0, c1,
-                                 eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor, inserts);
+                                 eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor, inserts, pr, s_thr);
     if (whole) {
       const uint32_t qq = qb * QB + (tid_x() >> 6) * 32 + (tid_x() & 31);
       if (qq < nq) {
@@ -502,6 +585,7 @@ __global__ __launch_bounds__(64 * NW) void coarse_batch_kernel(
     const unsigned long long* __restrict__ bad) {
   __shared__ __attribute__((aligned(16))) uint8_t tile0[TileShape<J>::TILEB];
   __shared__ __attribute__((aligned(16))) uint8_t tile1[TileShape<J>::TILEB];
+  __shared__ float s_thr[2 * NW];
   __shared__ uint32_t s_item;
   // a sample with a NaN or inf row fails the whole call: no score is compared
   if (*bad != ~0ull) return;
@@ -519,7 +603,7 @@ __global__ __launch_bounds__(64 * NW) void coarse_batch_kernel(
     scan_block<PREC, J, KPR, NW>(tile0, tile1, xsq + rowb * 32 * J, xs32 + rowb * dp32, dp32, S.n, S.kp, nullptr,
                                  S.n, qb, nullptr, sp, S.nsplit, c0, c1, eps_pre,
                                  cand_idx + S.entry_base, cand_sc + S.entry_base, cand_cnt + S.slot_base,
-                                 cand_floor + S.slot_base, inserts);
+                                 cand_floor + S.slot_base, inserts, PruneArgs{}, s_thr);
     __syncthreads();
   }
 }

@@ -72,7 +72,12 @@ typedef struct costal_knn_params {
   uint64_t row_block;    /* query rows per workgroup hint; 0 = tuned (speed only) */
   uint64_t col_block;    /* candidate columns per LDS tile hint; 0 = tuned (speed only) */
   uint32_t col_splits;   /* split the candidate range in this many parts; 0 = tuned (speed only) */
-  uint32_t reserved;
+  /* KnnParams::prune, inverted so that zero is the default (on): 0 = the fused scan works on the
+   * rows in a locality order (spherical k-means on a sample, rows sorted by nearest center) and
+   * skips every candidate tile whose cosine bound (from the cones of the query block and the tile)
+   * is below the admission threshold of every list of the block; nonzero = every candidate tile
+   * is scanned in row order.  Speed only: the result is the same bit for bit. */
+  uint32_t no_prune;
   /* Optional per-row admission thresholds (f32, n entries; host memory for costal_knn_exact,
    * device memory for costal_knn_exact_device): candidates whose coarse similarity does not exceed
    * the row's threshold are never inserted.  Rows left with fewer than k admitted candidates, or
@@ -86,6 +91,8 @@ typedef struct costal_knn_params {
   const float* mlp_params;
   float mlp_offset;
   uint32_t mlp_bf16;     /* 1 = BF16 inference (products of bf16-rounded values, f32 sums) */
+  uint32_t prune_clusters; /* k-means centers of the locality order; 0 = about n/3072 in [16, 256] */
+  uint32_t reserved;
 } costal_knn_params;
 
 /* KnnStats (knn.rs:344-357) plus GPU-side counters */
@@ -93,8 +100,9 @@ typedef struct costal_knn_stats {
   uint64_t fallback_rows;     /* rows whose first scan failed the certificate and were recomputed */
   uint64_t row_block;         /* query rows per workgroup used */
   uint64_t col_block;         /* candidate columns per LDS tile used */
-  uint64_t blocks_skipped;    /* always 0 for cosine selection */
-  uint64_t blocks_total;      /* row blocks x column tiles */
+  uint64_t blocks_skipped;    /* (query block, candidate tile) pairs the pruned scan never read */
+  uint64_t blocks_total;      /* query blocks x candidate tiles (x splits of a block's tiles: each
+                                 pair is counted once) */
   uint64_t candidates;        /* K' = min(k + margin, n - 1) */
   uint64_t col_splits;        /* candidate-range splits used (2 for the balanced launch: a row's
                                  range in at most two pieces) */
@@ -116,6 +124,7 @@ typedef struct costal_knn_stats {
    * certificate used for them (knn.rs:164-188).  0 when no candidate list was built (row scan). */
   double coarse_err_max;
   double coarse_eps;
+  double order_ms;            /* locality order, working copies and tile bounds of the pruned scan */
 } costal_knn_stats;
 
 /* Exact kNN by params->select from host memory.  norms may be NULL; weights (Tanimoto, n*k) may be

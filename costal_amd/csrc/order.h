@@ -20,10 +20,13 @@
 
 namespace costal {
 
-// Rows of a candidate tile of the bound matrix (a whole number of the scan's LDS tiles).
-constexpr uint32_t PRUNE_TILE = 512;
+// Rows of a candidate tile of the bound matrix (a whole number of the scan's LDS tiles): 256, or
+// 512 where 256-row tiles would be more than the bound sort holds.
+constexpr uint32_t PRUNE_TILE = 256;
 // Largest tile count per query block the bound sort handles (bitonic sort in LDS).
 constexpr uint32_t PRUNE_MAX_TILES = 4096;
+// Rows of a rescan (rows that failed the first certificate) that get their own cones and bounds.
+constexpr uint32_t RESCAN_PRUNE_ROWS = 8192;
 // Bytes of LDS the k-means kernels hold per center feature (f32 center + i64 sum).
 constexpr uint32_t KM_LDS_BYTES = 12;
 constexpr uint32_t KM_LDS_MAX = 60 * 1024;
@@ -36,13 +39,19 @@ struct OrderBufs {
   uint32_t* counts;  // C
   uint32_t* assign;  // n
   uint32_t* rank;    // n
-  uint32_t* hist;    // (C + 1) x row blocks of 256, then their exclusive prefix sums
+  uint32_t* hist;    // (C + 1) x row blocks of 256, then the first working row of each
+  uint32_t* totals;  // C + 1: rows per center, then the first working row of each
   float* qcen;       // query blocks x dp
-  double* qang;      // query blocks
+  double* qang;      // query blocks x 3: cone angle, its cosine, its sine
   float* tcen;       // candidate tiles x dp
-  double* tang;      // candidate tiles
+  double* tang;      // candidate tiles x 3
   uint32_t* seq;     // query blocks x tiles: tile ids in decreasing bound order
   float* sb;         // query blocks x tiles: the bounds in that order
+  // the same for the blocks of a rescan's row list (RESCAN_PRUNE_ROWS / qb blocks at most)
+  float* rcen;
+  double* rang;
+  uint32_t* rseq;
+  float* rsb;
 };
 
 struct OrderShape {
@@ -51,7 +60,8 @@ struct OrderShape {
   uint32_t sample;     // rows of the k-means sample (spread evenly over all rows)
   uint32_t iters;      // k-means iterations on the sample
   uint32_t qb;         // rows of a query block
-  uint32_t nblk, npt;  // query blocks, candidate tiles of PRUNE_TILE rows
+  uint32_t pt;         // rows of a candidate tile (PRUNE_TILE or twice that)
+  uint32_t nblk, npt;  // query blocks, candidate tiles
 };
 
 // The default shape for n rows of dp features with query blocks of qb rows.
@@ -71,5 +81,10 @@ hipError_t order_permute(hipStream_t s, const float* xs32, const OrderShape& o, 
                          uint32_t dp16, int split16, float* xs32w, _Float16* xs16w);
 // Cones of the query blocks and candidate tiles, the bound matrix and its per-block order.
 hipError_t order_bounds(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b);
+// A rescan's row list (original rows, nr <= RESCAN_PRUNE_ROWS): sorted in place into working
+// order (so that its blocks of qb rows are groups of similar rows), their cones, and every block's
+// tiles in bound order in rseq / rsb.  Needs order_bounds first (the tile cones).
+hipError_t order_rescan(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b, uint32_t* rows,
+                        uint32_t nr);
 
 }  // namespace costal

@@ -113,10 +113,12 @@ __global__ __launch_bounds__(256) void km_step_kernel(const float* __restrict__ 
     if (s_cnt[c]) atomicAdd(&counts[c], s_cnt[c]);
 }
 
-__global__ void km_clear_kernel(long long* __restrict__ sums, uint32_t* __restrict__ counts, uint32_t C, uint32_t dp) {
+__global__ void km_clear_kernel(long long* __restrict__ sums, uint32_t* __restrict__ counts,
+                                uint32_t* __restrict__ totals, uint32_t C, uint32_t dp) {
   const uint32_t e = bid_x() * bdim_x() + tid_x();
   if (e < C * dp) sums[e] = 0;
   if (e < C) counts[e] = 0;
+  if (e <= C) totals[e] = 0;
 }
 
 // New centers: the normalized means of their rows (a center with no rows keeps its place); the
@@ -139,13 +141,13 @@ __global__ void km_update_kernel(uint32_t dp, uint32_t C, long long* __restrict_
 }
 
 // Every row's center (zero rows: center C, sorted last), its rank among the rows of its 256-row
-// block with the same center, and the per-block center histogram, stored center-major
-// (hist[c * nb + block]) so that one exclusive scan gives every (center, block) its first slot.
+// block with the same center, the per-block center histogram, stored center-major
+// (hist[c * nb + block]), and the rows per center (totals, integer atomics).
 template <int DPM>
 __global__ __launch_bounds__(256) void km_assign_kernel(const float* __restrict__ xs32, uint32_t dp, uint32_t n,
                                                         uint32_t C, const float* __restrict__ centers,
                                                         uint32_t* __restrict__ assign, uint32_t* __restrict__ rank,
-                                                        uint32_t* __restrict__ hist) {
+                                                        uint32_t* __restrict__ hist, uint32_t* __restrict__ totals) {
   extern __shared__ __attribute__((aligned(16))) uint8_t km_lds[];
   float* s_cen = reinterpret_cast<float*>(km_lds);
   uint32_t* s_h = reinterpret_cast<uint32_t*>(s_cen + (size_t)C * DPM);  // C + 1
@@ -174,29 +176,46 @@ __global__ __launch_bounds__(256) void km_assign_kernel(const float* __restrict_
     rank[i] = r;
   }
   const uint32_t nb = gdim_x();
-  for (uint32_t c = tid_x(); c <= C; c += 256) hist[(size_t)c * nb + bid_x()] = s_h[c];
+  for (uint32_t c = tid_x(); c <= C; c += 256) {
+    hist[(size_t)c * nb + bid_x()] = s_h[c];
+    if (s_h[c]) atomicAdd(&totals[c], s_h[c]);
+  }
 }
 
-// In-place exclusive prefix sum of m counts (one workgroup).
-__global__ __launch_bounds__(1024) void exclusive_scan_kernel(uint32_t* __restrict__ v, uint32_t m) {
-  __shared__ uint32_t s[1024];
+// First working row of every center: exclusive prefix sum of the C + 1 totals, in place.
+__global__ void center_start_kernel(uint32_t* __restrict__ totals, uint32_t m) {
+  if (tid_x() != 0) return;
+  uint32_t run = 0;
+  for (uint32_t c = 0; c < m; ++c) {
+    const uint32_t v = totals[c];
+    totals[c] = run;
+    run += v;
+  }
+}
+
+// First working row of every (center, block): the center's first row plus the exclusive prefix
+// sum of its per-block counts (one workgroup per center), in place.
+__global__ __launch_bounds__(256) void block_offsets_kernel(uint32_t* __restrict__ hist, uint32_t nb,
+                                                            const uint32_t* __restrict__ start) {
+  __shared__ uint32_t s[256];
+  uint32_t* h = hist + (size_t)bid_x() * nb;
   const uint32_t t = tid_x();
-  const uint32_t per = (m + 1023) / 1024;
-  const uint32_t lo = std::min(m, t * per), hi = std::min(m, lo + per);
+  const uint32_t per = (nb + 255) / 256;
+  const uint32_t lo = std::min(nb, t * per), hi = std::min(nb, lo + per);
   uint32_t sum = 0;
-  for (uint32_t e = lo; e < hi; ++e) sum += v[e];
+  for (uint32_t e = lo; e < hi; ++e) sum += h[e];
   s[t] = sum;
   __syncthreads();
-  for (uint32_t off = 1; off < 1024; off <<= 1) {
+  for (uint32_t off = 1; off < 256; off <<= 1) {
     const uint32_t add = t >= off ? s[t - off] : 0u;
     __syncthreads();
     s[t] += add;
     __syncthreads();
   }
-  uint32_t run = s[t] - sum;
+  uint32_t run = start[bid_x()] + s[t] - sum;
   for (uint32_t e = lo; e < hi; ++e) {
-    const uint32_t x = v[e];
-    v[e] = run;
+    const uint32_t x = h[e];
+    h[e] = run;
     run += x;
   }
 }
@@ -250,12 +269,14 @@ __device__ __forceinline__ double wave_min(double x) {
   return x;
 }
 
-// Cone of working rows [g*G, min(n, g*G + G)) (one wave): center = the normalized f64 sum of the
-// rows, rounded to f32; angle = the largest angle between that f32 center and any row, in f64 with
-// both vectors' own norms, plus CONE_MARGIN; pi when the sum is zero (the cone covers everything).
+// Cone of positions [g*G, min(n, g*G + G)) of a row list (rows[w]: the original row at position w,
+// e.g. the working order perm) (one wave): center = the normalized f64 sum of the rows, rounded to
+// f32; angle = the largest angle between that f32 center and any row, in f64 with both vectors'
+// own norms, plus CONE_MARGIN; pi when the sum is zero (the cone covers everything).  ang[3g ..]
+// = (angle, its cosine, its sine).
 template <int DPM>
 __global__ __launch_bounds__(64) void cone_kernel(const float* __restrict__ xs32, uint32_t dp,
-                                                  const uint32_t* __restrict__ perm, uint32_t n, uint32_t G,
+                                                  const uint32_t* __restrict__ rows, uint32_t n, uint32_t G,
                                                   float* __restrict__ cen, double* __restrict__ ang) {
   __shared__ float s_c[DPM];
   const uint32_t g = bid_x(), l = tid_x();
@@ -265,7 +286,7 @@ __global__ __launch_bounds__(64) void cone_kernel(const float* __restrict__ xs32
   for (int f = 0; f < DPM; ++f) acc[f] = 0.0;
   for (uint32_t w = w0 + l; w < w1; w += 64) {
     float v[DPM];
-    load_row<DPM>(xs32 + (size_t)perm[w] * dp, dp, v);
+    load_row<DPM>(xs32 + (size_t)rows[w] * dp, dp, v);
 #pragma unroll
     for (int f = 0; f < DPM; ++f) acc[f] += (double)v[f];
   }
@@ -278,7 +299,7 @@ __global__ __launch_bounds__(64) void cone_kernel(const float* __restrict__ xs32
   const double nrm = sqrt(nn);
   if (!(nrm > 0.0)) {
     if (l < dp) cen[(size_t)g * dp + l] = 0.0f;
-    if (l == 0) ang[g] = M_PI;
+    if (l == 0) ang[3 * g] = M_PI, ang[3 * g + 1] = -1.0, ang[3 * g + 2] = 0.0;
     return;
   }
   if (l < DPM) {
@@ -297,7 +318,7 @@ __global__ __launch_bounds__(64) void cone_kernel(const float* __restrict__ xs32
   double m = 2.0;
   for (uint32_t w = w0 + l; w < w1; w += 64) {
     float v[DPM];
-    load_row<DPM>(xs32 + (size_t)perm[w] * dp, dp, v);
+    load_row<DPM>(xs32 + (size_t)rows[w] * dp, dp, v);
     double dot = 0.0, vn = 0.0;
 #pragma unroll
     for (int f = 0; f < DPM; ++f) {
@@ -309,7 +330,10 @@ __global__ __launch_bounds__(64) void cone_kernel(const float* __restrict__ xs32
     m = fmin(m, c);
   }
   m = wave_min(m);
-  if (l == 0) ang[g] = acos(fmax(-1.0, fmin(1.0, m))) + CONE_MARGIN;
+  if (l == 0) {
+    const double a = acos(fmax(-1.0, fmin(1.0, m))) + CONE_MARGIN;
+    ang[3 * g] = a, ang[3 * g + 1] = cos(a), ang[3 * g + 2] = sin(a);
+  }
 }
 
 // Bound of every candidate tile for query block bid_x(), sorted in decreasing order (ties: lower
@@ -327,7 +351,7 @@ __global__ __launch_bounds__(256) void bound_sort_kernel(const float* __restrict
   double qn = 0.0;
   for (uint32_t f = 0; f < dp; ++f) qn += s_q[f] * s_q[f];
   qn = sqrt(qn);
-  const double aq = qang[b];
+  const double aq = qang[3 * b], cq = qang[3 * b + 1], sq = qang[3 * b + 2];
   for (uint32_t t = t0; t < p2; t += 256) {
     float key = -INFINITY;
     if (t < npt) {
@@ -338,11 +362,14 @@ __global__ __launch_bounds__(256) void bound_sort_kernel(const float* __restrict
         dot += s_q[f] * c;
         tn += c * c;
       }
-      const double a = aq + tang[t];
+      // cos(max(0, th - a)) with a = aq + at, from the cosines and sines of the angles: th > a iff
+      // cos th < cos a (both in [0, pi]); then cos(th - a) = cos th cos a + sin th sin a
+      const double at = tang[3 * t], ct = tang[3 * t + 1], st = tang[3 * t + 2];
       double bound = 1.0;
-      if (a < M_PI && qn > 0.0 && tn > 0.0) {
-        const double th = acos(fmax(-1.0, fmin(1.0, dot / (qn * sqrt(tn)))));
-        if (th > a) bound = cos(th - a);
+      if (aq + at < M_PI && qn > 0.0 && tn > 0.0) {
+        const double ca = cq * ct - sq * st, sa = sq * ct + cq * st;
+        const double cth = fmax(-1.0, fmin(1.0, dot / (qn * sqrt(tn))));
+        if (cth < ca) bound = cth * ca + sqrt(fmax(0.0, 1.0 - cth * cth)) * sa;
       }
       bound += BOUND_MARGIN;
       key = (float)bound;
@@ -377,6 +404,30 @@ __global__ __launch_bounds__(256) void bound_sort_kernel(const float* __restrict
   }
 }
 
+// rows[0..nr) (original rows) into working order: their working positions sorted (one workgroup,
+// bitonic in LDS), mapped back through perm.
+__global__ __launch_bounds__(1024) void sort_rows_kernel(uint32_t* __restrict__ rows, uint32_t nr, uint32_t p2,
+                                                         const uint32_t* __restrict__ perm,
+                                                         const uint32_t* __restrict__ iperm) {
+  __shared__ uint32_t s_k[RESCAN_PRUNE_ROWS];
+  const uint32_t t0 = tid_x();
+  for (uint32_t i = t0; i < p2; i += 1024) s_k[i] = i < nr ? iperm[rows[i]] : 0xffffffffu;
+  __syncthreads();
+  for (uint32_t k = 2; k <= p2; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = t0; i < p2; i += 1024) {
+        const uint32_t o = i ^ j;
+        if (o > i) {
+          const uint32_t a = s_k[i], b = s_k[o];
+          if (((i & k) == 0) == (a > b)) s_k[i] = b, s_k[o] = a;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (uint32_t i = t0; i < nr; i += 1024) rows[i] = perm[s_k[i]];
+}
+
 template <class F>
 hipError_t by_dpm(uint32_t dp, F f) {
   if (dp <= 16) return f(std::integral_constant<int, 16>{});
@@ -406,7 +457,8 @@ OrderShape order_shape(uint32_t n, uint32_t dp, uint32_t qb, uint32_t clusters) 
   o.iters = 5;
   o.qb = qb;
   o.nblk = (n + qb - 1) / qb;
-  o.npt = (n + PRUNE_TILE - 1) / PRUNE_TILE;
+  o.pt = (n + PRUNE_TILE - 1) / PRUNE_TILE <= PRUNE_MAX_TILES ? PRUNE_TILE : 2 * PRUNE_TILE;
+  o.npt = (n + o.pt - 1) / o.pt;
   return o;
 }
 
@@ -436,12 +488,18 @@ OrderBufs carve(Carver& c, const OrderShape& o) {
   b.assign = c.take<uint32_t>(o.n);
   b.rank = c.take<uint32_t>(o.n);
   b.hist = c.take<uint32_t>((size_t)(o.C + 1) * nb);
+  b.totals = c.take<uint32_t>(o.C + 1);
   b.qcen = c.take<float>((size_t)o.nblk * o.dp);
-  b.qang = c.take<double>(o.nblk);
+  b.qang = c.take<double>(3 * (size_t)o.nblk);
   b.tcen = c.take<float>((size_t)o.npt * o.dp);
-  b.tang = c.take<double>(o.npt);
+  b.tang = c.take<double>(3 * (size_t)o.npt);
   b.seq = c.take<uint32_t>((size_t)o.nblk * o.npt);
   b.sb = c.take<float>((size_t)o.nblk * o.npt);
+  const size_t rb = (RESCAN_PRUNE_ROWS + o.qb - 1) / o.qb;
+  b.rcen = c.take<float>(rb * o.dp);
+  b.rang = c.take<double>(3 * rb);
+  b.rseq = c.take<uint32_t>(rb * o.npt);
+  b.rsb = c.take<float>(rb * o.npt);
   return b;
 }
 }  // namespace
@@ -467,7 +525,7 @@ hipError_t order_rows(hipStream_t s, const float* xs32, const OrderShape& o, Ord
     if (e != hipSuccess) return e;
     // the sums start at zero (a kernel, not a memset: callers may capture the stream); the update
     // kernel clears them after every pass
-    km_clear_kernel<<<(o.C * o.dp + 255) / 256, 256, 0, s>>>(b.sums, b.counts, o.C, o.dp);
+    km_clear_kernel<<<(o.C * o.dp + 255) / 256, 256, 0, s>>>(b.sums, b.counts, b.totals, o.C, o.dp);
     const size_t lds_step = (size_t)o.C * DPM * 12 + o.C * 4;
     for (uint32_t it = 0; it < o.iters; ++it) {
       km_step_kernel<DPM><<<(o.sample + 255) / 256, 256, lds_step, s>>>(xs32, o.dp, o.C, o.sample, stride, b.centers,
@@ -475,8 +533,10 @@ hipError_t order_rows(hipStream_t s, const float* xs32, const OrderShape& o, Ord
       km_update_kernel<<<(o.C + 63) / 64, 64, 0, s>>>(o.dp, o.C, b.sums, b.counts, b.centers);
     }
     const size_t lds_assign = (size_t)o.C * DPM * 4 + (o.C + 1) * 4;
-    km_assign_kernel<DPM><<<nb, 256, lds_assign, s>>>(xs32, o.dp, o.n, o.C, b.centers, b.assign, b.rank, b.hist);
-    exclusive_scan_kernel<<<1, 1024, 0, s>>>(b.hist, (o.C + 1) * nb);
+    km_assign_kernel<DPM><<<nb, 256, lds_assign, s>>>(xs32, o.dp, o.n, o.C, b.centers, b.assign, b.rank, b.hist,
+                                                       b.totals);
+    center_start_kernel<<<1, 64, 0, s>>>(b.totals, o.C + 1);
+    block_offsets_kernel<<<o.C + 1, 256, 0, s>>>(b.hist, nb, b.totals);
     km_scatter_kernel<<<nb, 256, 0, s>>>(b.assign, b.rank, b.hist, o.n, nb, b.perm, b.iperm);
     return hipGetLastError();
   });
@@ -492,10 +552,27 @@ hipError_t order_bounds(hipStream_t s, const float* xs32, const OrderShape& o, O
   return by_dpm(o.dp, [&](auto dpm) -> hipError_t {
     constexpr int DPM = decltype(dpm)::value;
     cone_kernel<DPM><<<o.nblk, 64, 0, s>>>(xs32, o.dp, b.perm, o.n, o.qb, b.qcen, b.qang);
-    cone_kernel<DPM><<<o.npt, 64, 0, s>>>(xs32, o.dp, b.perm, o.n, PRUNE_TILE, b.tcen, b.tang);
+    cone_kernel<DPM><<<o.npt, 64, 0, s>>>(xs32, o.dp, b.perm, o.n, o.pt, b.tcen, b.tang);
     uint32_t p2 = 1;
     while (p2 < o.npt) p2 <<= 1;
     bound_sort_kernel<<<o.nblk, 256, 0, s>>>(b.qcen, b.qang, b.tcen, b.tang, o.dp, o.npt, p2, b.seq, b.sb);
+    return hipGetLastError();
+  });
+}
+
+hipError_t order_rescan(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b, uint32_t* rows,
+                        uint32_t nr) {
+  if (nr == 0 || nr > RESCAN_PRUNE_ROWS) return hipErrorInvalidValue;
+  uint32_t p2 = 1;
+  while (p2 < nr) p2 <<= 1;
+  sort_rows_kernel<<<1, 1024, 0, s>>>(rows, nr, p2, b.perm, b.iperm);
+  const uint32_t nblk = (nr + o.qb - 1) / o.qb;
+  return by_dpm(o.dp, [&](auto dpm) -> hipError_t {
+    constexpr int DPM = decltype(dpm)::value;
+    cone_kernel<DPM><<<nblk, 64, 0, s>>>(xs32, o.dp, rows, nr, o.qb, b.rcen, b.rang);
+    uint32_t q2 = 1;
+    while (q2 < o.npt) q2 <<= 1;
+    bound_sort_kernel<<<nblk, 256, 0, s>>>(b.rcen, b.rang, b.tcen, b.tang, o.dp, o.npt, q2, b.rseq, b.rsb);
     return hipGetLastError();
   });
 }

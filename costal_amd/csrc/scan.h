@@ -40,7 +40,8 @@ struct PruneArgs {
   const uint32_t* iperm;         // original row -> working row (for a row list)
   const uint32_t* seq;           // query block b: tiles seq[b*npt ..] in decreasing bound order
   const float* sb;               // their bounds (upper bounds of every query-candidate cosine)
-  uint32_t npt;                  // candidate tiles (PRUNE_TILE rows each)
+  uint32_t npt;                  // candidate tiles
+  uint32_t pt;                   // rows of a candidate tile (a multiple of the LDS tile)
   unsigned long long* scanned;   // candidate tiles scanned (summed over blocks and splits)
 };
 
@@ -241,6 +242,7 @@ __device__ __forceinline__ void scan_block(
   constexpr int CPT = (CHUNKS + NT - 1) / NT;
   constexpr int GS = TileShape<J>::GS;
   constexpr int NSTEP = TileShape<J>::NSTEP;
+  constexpr int SUBT = TileShape<J>::SUBT;
   static_assert(NSTEP >= 2 && NSTEP % 2 == 0, "pipeline steps per tile");
   static_assert(GS * NSTEP * 32 == TC, "tile shape");
 
@@ -286,21 +288,28 @@ __device__ __forceinline__ void scan_block(
   float thp = -INFINITY;  // the partner list's threshold, refreshed once per tile
   uint32_t my_inserts = 0;
 
-  // Candidate tiles: the range [c_begin, c_end) in row order, or (pruned) the split's share of the
-  // query block's tiles in decreasing bound order (positions split, split + nsplit, ...), each
-  // R LDS tiles long, stopping at the first whose bound is below every list's threshold.
-  constexpr int R = (int)(costal::PRUNE_TILE / TC);
-  static_assert(R * TC == (int)costal::PRUNE_TILE, "LDS tiles per candidate tile");
+  // Candidate tiles.  Without pruning: the range [c_begin, c_end) in row order, one LDS tile after
+  // another, or (a row list in the working order) every nsplit-th LDS tile of all rows, so that a
+  // row's near neighbors, which sit close together, spread over all of its lists.  Pruned: the
+  // split's share of the query block's candidate tiles in decreasing bound order (positions
+  // split, split + nsplit, ...); an LDS tile is a part of one candidate tile (R = pt / TC of them
+  // per candidate tile) or two candidate tiles of half its rows (H = 2), and the scan stops before
+  // the first LDS tile whose first candidate tile's bound is below every list's threshold.  Each
+  // LDS tile is two halves of TC/2 rows with first rows (x, y); a second half past the split's
+  // last position numbers its candidates from n, so that none is admitted (and loads the first
+  // half's rows again).
   const uint32_t* seq_b = pr.seq ? pr.seq + (size_t)qblock * pr.npt : nullptr;
   const float* sb_b = pr.seq ? pr.sb + (size_t)qblock * pr.npt : nullptr;
-  // In the working order (rows grouped by similarity) a split of a row list takes every nsplit-th
-  // LDS tile instead of a contiguous range, so that a row's near neighbors, which sit close
-  // together, spread over all of its lists.
   const bool interleave = !seq_b && pr.perm && nsplit > 1;
-  int ntile;
+  int ntile, R = 1;
+  bool two = false;
+  uint32_t npos = 0;
   if (seq_b) {
     c_end = n;
-    ntile = pr.npt > split ? (int)((pr.npt - split + nsplit - 1) / nsplit) * R : 0;
+    npos = pr.npt > split ? (pr.npt - split + nsplit - 1) / nsplit : 0;
+    two = pr.pt < (uint32_t)TC;  // pt = TC / 2
+    R = two ? 1 : (int)(pr.pt / (uint32_t)TC);
+    ntile = two ? (int)((npos + 1) / 2) : (int)npos * R;
   } else if (interleave) {
     c_end = n;
     const uint32_t all = (n + TC - 1) / TC;
@@ -308,10 +317,19 @@ __device__ __forceinline__ void scan_block(
   } else {
     ntile = c_begin < c_end ? (int)((c_end - c_begin + TC - 1) / TC) : 0;
   }
-  auto tile_base = [&](int T) -> uint32_t {
-    if (seq_b) return seq_b[split + (uint32_t)(T / R) * nsplit] * costal::PRUNE_TILE + (uint32_t)(T % R) * TC;
-    if (interleave) return (split + (uint32_t)T * nsplit) * TC;
-    return c_begin + (uint32_t)T * TC;
+  auto tile_base = [&](int T) -> uint2 {
+    uint32_t a;
+    if (seq_b) {
+      if (two) {
+        const uint32_t p0 = 2 * (uint32_t)T;
+        a = seq_b[split + p0 * nsplit] * pr.pt;
+        return make_uint2(a, p0 + 1 < npos ? seq_b[split + (p0 + 1) * nsplit] * pr.pt : n);
+      }
+      a = seq_b[split + (uint32_t)(T / R) * nsplit] * pr.pt + (uint32_t)(T % R) * TC;
+    } else {
+      a = interleave ? (split + (uint32_t)T * nsplit) * TC : c_begin + (uint32_t)T * TC;
+    }
+    return make_uint2(a, a + TC / 2);
   };
 
   // Tiles go global -> registers -> LDS: the registers are loaded with tile t+2 while tile t is
@@ -321,6 +339,11 @@ __device__ __forceinline__ void scan_block(
   // stay exact at the loop head (a second staging set, or an exit from the middle of a body
   // unrolled by two, made it wait there for loads still in flight).
   static_assert(CPT <= 8, "tile staging uses at most eight 16-byte registers per thread");
+  // Tiles of 512 rows may be two candidate tiles of 256: the halves load from their own rows (every
+  // thread's chunk C lies in one half, since NT divides half a tile); smaller tiles are always one
+  // contiguous range (the second half's first row is the first half's plus TC/2).
+  constexpr bool SPLIT_HALVES = TC == 512;
+  static_assert(!SPLIT_HALVES || (CHUNKS / 2) % NT == 0, "halves of a tile in whole chunks per thread");
   uint4 ra0 = {}, ra1 = {}, ra2 = {}, ra3 = {}, ra4 = {}, ra5 = {}, ra6 = {}, ra7 = {};
   // Thread e fills LDS slot e (consecutive lanes, consecutive 16-byte slots: conflict free); slot
   // e = ((sub*J + j)*2 + half)*32 + lane32 holds chunk 2j + half of tile row sub*32 + lane32.
@@ -329,13 +352,16 @@ __device__ __forceinline__ void scan_block(
     const int e_ = tid + C * NT;                                                                  \
     if (CHUNKS % NT == 0 || e_ < CHUNKS) {                                                        \
       const int lr_ = e_ & 31, hh_ = (e_ >> 5) & 1, j_ = (e_ >> 6) % J, sub_ = (e_ >> 6) / J;     \
-      const uint32_t off_ = (uint32_t)((sub_ * 32 + lr_) * ROWB + (2 * j_ + hh_) * 16);            \
-      R##C = *reinterpret_cast<const uint4*>(src_ + off_);                                       \
+      /* the half of the tile: chunk C of every thread lies in one half (SPLIT_HALVES) */          \
+      constexpr int hs_ = SPLIT_HALVES && C * NT >= CHUNKS / 2 ? 1 : 0;                           \
+      const uint32_t off_ = (uint32_t)(((sub_ - hs_ * (SUBT / 2)) * 32 + lr_) * ROWB + (2 * j_ + hh_) * 16); \
+      R##C = *reinterpret_cast<const uint4*>((hs_ ? srcb_ : srca_) + off_);                      \
     }                                                                                             \
   }
-#define LOAD_REGS(R, BASE)                                                                        \
+#define LOAD_REGS(R, HB)                                                                          \
   {                                                                                               \
-    const uint8_t* src_ = xsq + (size_t)(BASE) * ROWB;                                             \
+    const uint8_t* srca_ = xsq + (size_t)(HB).x * ROWB;                                            \
+    const uint8_t* srcb_ = SPLIT_HALVES ? xsq + (size_t)((HB).y < n ? (HB).y : (HB).x) * ROWB : srca_; \
     LOAD_ONE(R, 0) LOAD_ONE(R, 1) LOAD_ONE(R, 2) LOAD_ONE(R, 3)                                   \
     LOAD_ONE(R, 4) LOAD_ONE(R, 5) LOAD_ONE(R, 6) LOAD_ONE(R, 7)                                   \
   }
@@ -369,8 +395,9 @@ __device__ __forceinline__ void scan_block(
   constexpr int RD = 3;  // chunks of step G+RD are read at step G
   static_assert(NSTEP >= RD, "steps per tile");
   int t = 0;
-  // first candidate of tile t (in cur), of tile t+1 (in the staging registers) and of tile t+2
-  uint32_t b_cur = 0, b_nx = 0, b_nn = 0;
+  // first candidates of the halves of tile t (in cur), of tile t+1 (in the staging registers)
+  // and of tile t+2
+  uint2 b_cur = make_uint2(0, 0), b_nx = b_cur, b_nn = b_cur;
   // Pruning, once per tile: every wave posts the lowest admission threshold of its lists (the
   // list minimum or the partner's, whichever is higher: what the list turns away) before the
   // barrier; after it, the workgroup's lowest threshold decides whether tile t+2 is scanned.
@@ -388,7 +415,8 @@ __device__ __forceinline__ void scan_block(
 #pragma unroll
     for (int v = 1; v < NW; ++v) tb = fminf(tb, s_thr[(t & 1) * NW + v]);
     const int T = t + 2;
-    if (T < ntile && T % R == 0 && sb_b[split + (uint32_t)(T / R) * nsplit] < tb) ntile = T;
+    const uint32_t pos = two ? 2 * (uint32_t)T : (uint32_t)(T / R);
+    if (T < ntile && (two || T % R == 0) && sb_b[split + pos * nsplit] < tb) ntile = T;
   };
   // compare the accumulators of step `p` of tile t with the lane's threshold; rare path: each lane
   // walks its own hits, sub-tile by sub-tile
@@ -412,7 +440,9 @@ __device__ __forceinline__ void scan_block(
           // tile loop (each would hold a register for the whole launch)
           uint32_t lane_off = 4 * h;
           __asm__ volatile("" : "+v"(lane_off));
-          const uint32_t cbase = b_cur + (uint32_t)(p * GS + u) * 32 + lane_off;
+          const int sub = p * GS + u;  // the sub-tile, in the first or second half of the tile
+          const uint32_t cbase =
+              (sub < SUBT / 2 ? b_cur.x + (uint32_t)sub * 32 : b_cur.y + (uint32_t)(sub - SUBT / 2) * 32) + lane_off;
           while (mask) {
             const int i = __builtin_ctz(mask);
             mask &= mask - 1;
@@ -478,7 +508,10 @@ __device__ __forceinline__ void scan_block(
       b_nx = b_nn;
     }
   }
-  if (seq_b && pr.scanned && tid == 0) atomicAdd(pr.scanned, (unsigned long long)((ntile + R - 1) / R));
+  if (seq_b && pr.scanned && tid == 0) {
+    const uint32_t done = two ? (2 * (uint32_t)ntile < npos ? 2 * (uint32_t)ntile : npos) : (uint32_t)((ntile + R - 1) / R);
+    atomicAdd(pr.scanned, (unsigned long long)done);
+  }
 #undef RUN_TILE
 #undef MFMA_STEP
 #undef READ_B
@@ -521,8 +554,18 @@ __device__ __forceinline__ void scan_block(
 //     tile) units in query-block-major order, so every slot ends together.  A query block whose
 //     units straddle two workgroups is scanned in two pieces: the one from tile 0 fills list pair 0,
 //     the other list pair 1 (nsplit = 2); a block scanned whole marks pair 1 empty.
+// Waves per SIMD the scan kernel is built for: two where the registers allow it without scratch
+// (the compiler otherwise lets such instances grow past 256 registers by a few, which halves the
+// occupancy: the split-FP16 headline instance measured 12.3 ms unbounded against 6.2 ms bounded);
+// left to the compiler where two would spill: J of 3 or 4, lists of 64, and lists of 32 with J of
+// 2, 7 or 8.
+template <int J, int KPR>
+constexpr int scan_waves() {
+  return (KPR >= 64 || J == 3 || J == 4 || (KPR >= 32 && (J == 2 || J >= 7))) ? 1 : 2;
+}
+
 template <int PREC, int J, int KPR, int NW>
-__global__ __launch_bounds__(64 * NW) void coarse_kernel(
+__global__ __launch_bounds__(64 * NW, (scan_waves<J, KPR>())) void coarse_kernel(
     const uint8_t* __restrict__ xsq, const float* __restrict__ xs32, uint32_t dp32, uint32_t n, int kp,
     const uint32_t* __restrict__ rowlist, uint32_t nq, const float* __restrict__ tinit, uint32_t cols_per_split,
     uint32_t units_per_wg, float eps_pre, uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc,

@@ -336,14 +336,102 @@ __global__ __launch_bounds__(64) void cone_kernel(const float* __restrict__ xs32
   }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Bitonic sort of one workgroup's values into descending order: 1024 threads, value s*1024 + t in
+// thread t's register v[s] (s < E, E a power of two up to EMAX).  Exchange distances of 1024 and
+// more stay inside a thread, those below 64 inside a wave (shuffles), and only the ones in between
+// go through LDS (s_buf, 1024*E values), so a sort of 4096 values meets at 2*15 barriers.
+template <class T>
+__device__ __forceinline__ T shfl_xor_t(T x, int j) {
+  if constexpr (sizeof(T) == 8) {
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    return (T)(((uint64_t)(uint32_t)__shfl_xor((int)hi, j) << 32) | (uint32_t)__shfl_xor((int)lo, j));
+  } else {
+    return (T)__shfl_xor((int)x, j);
+  }
+}
+
+// the pair (a at index i, b at its partner i ^ j) of a pass of stage k: what index i keeps
+template <class T>
+__device__ __forceinline__ T keep(T a, T b, uint32_t i, uint32_t j, uint32_t k) {
+  const bool lower = (i & j) == 0, desc = (i & k) == 0;
+  const T hi = a > b ? a : b, lo = a > b ? b : a;
+  return lower == desc ? hi : lo;
+}
+
+// distance SJ * 1024: the pairs (s, s | SJ) of one thread's registers
+template <int SJ, int S = 0, int EMAX, class T>
+__device__ __forceinline__ void reg_pass(T (&v)[EMAX], int E, uint32_t k, uint32_t t) {
+  if constexpr (S < EMAX) {
+    if constexpr ((S & SJ) == 0 && (S | SJ) < EMAX) {
+      if ((S | SJ) < E) {
+        const uint32_t i = (uint32_t)S * 1024 + t;
+        const T a = v[S], b = v[S | SJ];
+        v[S] = keep(a, b, i, (uint32_t)SJ * 1024, k);
+        v[S | SJ] = keep(b, a, i + (uint32_t)SJ * 1024, (uint32_t)SJ * 1024, k);
+      }
+    }
+    reg_pass<SJ, S + 1>(v, E, k, t);
+  }
+}
+
+template <int EMAX, class T>
+__device__ void block_sort_desc(T (&v)[EMAX], int E, T* s_buf) {
+  const uint32_t t = tid_x();
+  const uint32_t N = 1024u * (uint32_t)E;
+  for (uint32_t k = 2; k <= N; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      if (j >= 1024) {
+        const uint32_t sj = j >> 10;
+        if (sj == 1) reg_pass<1>(v, E, k, t);
+        if constexpr (EMAX > 2) {
+          if (sj == 2) reg_pass<2>(v, E, k, t);
+        }
+        if constexpr (EMAX > 4) {
+          if (sj == 4) reg_pass<4>(v, E, k, t);
+        }
+      } else if (j >= 64) {
+#pragma unroll
+        for (int s = 0; s < EMAX; ++s)
+          if (s < E) s_buf[s * 1024 + t] = v[s];
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < EMAX; ++s)
+          if (s < E) {
+            const uint32_t i = (uint32_t)s * 1024 + t;
+            v[s] = keep(v[s], s_buf[i ^ j], i, j, k);
+          }
+        __syncthreads();
+      } else {
+#pragma unroll
+        for (int s = 0; s < EMAX; ++s)
+          if (s < E) {
+            const uint32_t i = (uint32_t)s * 1024 + t;
+            v[s] = keep(v[s], shfl_xor_t(v[s], (int)j), i, j, k);
+          }
+      }
+    }
+  }
+}
+
+// f32 as an unsigned key of the same order (non-NaN values), and back
+__device__ __forceinline__ uint32_t float_key(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
 // Bound of every candidate tile for query block bid_x(), sorted in decreasing order (ties: lower
 // tile first): seq[b*npt + r] = tile of rank r, sb[b*npt + r] = its bound, an f32 rounded up.
-__global__ __launch_bounds__(256) void bound_sort_kernel(const float* __restrict__ qcen, const double* __restrict__ qang,
-                                                         const float* __restrict__ tcen, const double* __restrict__ tang,
-                                                         uint32_t dp, uint32_t npt, uint32_t p2, uint32_t* __restrict__ seq,
-                                                         float* __restrict__ sb) {
-  __shared__ float s_key[PRUNE_MAX_TILES];
-  __shared__ uint32_t s_id[PRUNE_MAX_TILES];
+// 1024 threads; a value is (bound key << 32) | ~tile, so that one comparison orders both.
+__global__ __launch_bounds__(1024) void bound_sort_kernel(const float* __restrict__ qcen, const double* __restrict__ qang,
+                                                          const float* __restrict__ tcen, const double* __restrict__ tang,
+                                                          uint32_t dp, uint32_t npt, uint32_t p2, uint32_t* __restrict__ seq,
+                                                          float* __restrict__ sb) {
+  constexpr int EMAX = PRUNE_MAX_TILES / 1024;
+  __shared__ uint64_t s_buf[PRUNE_MAX_TILES];
   __shared__ double s_q[64];
   const uint32_t b = bid_x(), t0 = tid_x();
   if (t0 < 64) s_q[t0] = t0 < dp ? (double)qcen[(size_t)b * dp + t0] : 0.0;
@@ -352,9 +440,13 @@ __global__ __launch_bounds__(256) void bound_sort_kernel(const float* __restrict
   for (uint32_t f = 0; f < dp; ++f) qn += s_q[f] * s_q[f];
   qn = sqrt(qn);
   const double aq = qang[3 * b], cq = qang[3 * b + 1], sq = qang[3 * b + 2];
-  for (uint32_t t = t0; t < p2; t += 256) {
-    float key = -INFINITY;
-    if (t < npt) {
+  const int E = (int)(p2 > 1024 ? p2 / 1024 : 1);
+  uint64_t v[EMAX];
+#pragma unroll
+  for (int s = 0; s < EMAX; ++s) {
+    v[s] = 0;  // padding: below every real value, sorted last
+    const uint32_t t = (uint32_t)s * 1024 + t0;
+    if (s < E && t < npt) {
       const float* tc = tcen + (size_t)t * dp;
       double dot = 0.0, tn = 0.0;
       for (uint32_t f = 0; f < dp; ++f) {
@@ -372,60 +464,43 @@ __global__ __launch_bounds__(256) void bound_sort_kernel(const float* __restrict
         if (cth < ca) bound = cth * ca + sqrt(fmax(0.0, 1.0 - cth * cth)) * sa;
       }
       bound += BOUND_MARGIN;
-      key = (float)bound;
+      float key = (float)bound;
       if ((double)key < bound) key = nextafterf(key, INFINITY);
-    }
-    s_key[t] = key;
-    s_id[t] = t;
-  }
-  __syncthreads();
-  // bitonic sort: larger key first, then lower id
-  for (uint32_t k = 2; k <= p2; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = t0; i < p2; i += 256) {
-        const uint32_t o = i ^ j;
-        if (o > i) {
-          const float ka = s_key[i], kb = s_key[o];
-          const uint32_t ia = s_id[i], ib = s_id[o];
-          const bool a_first = ka > kb || (ka == kb && ia < ib);
-          const bool desc = (i & k) == 0;
-          if (desc != a_first) {
-            s_key[i] = kb, s_key[o] = ka;
-            s_id[i] = ib, s_id[o] = ia;
-          }
-        }
-      }
-      __syncthreads();
+      v[s] = ((uint64_t)float_key(key) << 32) | (uint32_t)~t;
     }
   }
-  for (uint32_t t = t0; t < npt; t += 256) {
-    seq[(size_t)b * npt + t] = s_id[t];
-    sb[(size_t)b * npt + t] = s_key[t];
+  block_sort_desc<EMAX>(v, E, s_buf);
+#pragma unroll
+  for (int s = 0; s < EMAX; ++s) {
+    const uint32_t r = (uint32_t)s * 1024 + t0;
+    if (s < E && r < npt) {
+      seq[(size_t)b * npt + r] = ~(uint32_t)v[s];
+      sb[(size_t)b * npt + r] = key_float((uint32_t)(v[s] >> 32));
+    }
   }
 }
 
-// rows[0..nr) (original rows) into working order: their working positions sorted (one workgroup,
-// bitonic in LDS), mapped back through perm.
+// rows[0..nr) (original rows) into working order: their working positions sorted (one workgroup),
+// mapped back through perm.  A value is ~position, so that the descending sort is ascending.
 __global__ __launch_bounds__(1024) void sort_rows_kernel(uint32_t* __restrict__ rows, uint32_t nr, uint32_t p2,
                                                          const uint32_t* __restrict__ perm,
                                                          const uint32_t* __restrict__ iperm) {
-  __shared__ uint32_t s_k[RESCAN_PRUNE_ROWS];
+  constexpr int EMAX = RESCAN_PRUNE_ROWS / 1024;
+  __shared__ uint32_t s_buf[RESCAN_PRUNE_ROWS];
   const uint32_t t0 = tid_x();
-  for (uint32_t i = t0; i < p2; i += 1024) s_k[i] = i < nr ? iperm[rows[i]] : 0xffffffffu;
-  __syncthreads();
-  for (uint32_t k = 2; k <= p2; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = t0; i < p2; i += 1024) {
-        const uint32_t o = i ^ j;
-        if (o > i) {
-          const uint32_t a = s_k[i], b = s_k[o];
-          if (((i & k) == 0) == (a > b)) s_k[i] = b, s_k[o] = a;
-        }
-      }
-      __syncthreads();
-    }
+  const int E = (int)(p2 > 1024 ? p2 / 1024 : 1);
+  uint32_t v[EMAX];
+#pragma unroll
+  for (int s = 0; s < EMAX; ++s) {
+    const uint32_t i = (uint32_t)s * 1024 + t0;
+    v[s] = (s < E && i < nr) ? ~iperm[rows[i]] : 0u;  // padding (0) sorts last
   }
-  for (uint32_t i = t0; i < nr; i += 1024) rows[i] = perm[s_k[i]];
+  block_sort_desc<EMAX>(v, E, s_buf);
+#pragma unroll
+  for (int s = 0; s < EMAX; ++s) {
+    const uint32_t i = (uint32_t)s * 1024 + t0;
+    if (s < E && i < nr) rows[i] = perm[~v[s]];
+  }
 }
 
 template <class F>
@@ -555,7 +630,7 @@ hipError_t order_bounds(hipStream_t s, const float* xs32, const OrderShape& o, O
     cone_kernel<DPM><<<o.npt, 64, 0, s>>>(xs32, o.dp, b.perm, o.n, o.pt, b.tcen, b.tang);
     uint32_t p2 = 1;
     while (p2 < o.npt) p2 <<= 1;
-    bound_sort_kernel<<<o.nblk, 256, 0, s>>>(b.qcen, b.qang, b.tcen, b.tang, o.dp, o.npt, p2, b.seq, b.sb);
+    bound_sort_kernel<<<o.nblk, 1024, 0, s>>>(b.qcen, b.qang, b.tcen, b.tang, o.dp, o.npt, p2, b.seq, b.sb);
     return hipGetLastError();
   });
 }
@@ -572,7 +647,7 @@ hipError_t order_rescan(hipStream_t s, const float* xs32, const OrderShape& o, O
     cone_kernel<DPM><<<nblk, 64, 0, s>>>(xs32, o.dp, rows, nr, o.qb, b.rcen, b.rang);
     uint32_t q2 = 1;
     while (q2 < o.npt) q2 <<= 1;
-    bound_sort_kernel<<<nblk, 256, 0, s>>>(b.rcen, b.rang, b.tcen, b.tang, o.dp, o.npt, q2, b.rseq, b.rsb);
+    bound_sort_kernel<<<nblk, 1024, 0, s>>>(b.rcen, b.rang, b.tcen, b.tang, o.dp, o.npt, q2, b.rseq, b.rsb);
     return hipGetLastError();
   });
 }

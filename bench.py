@@ -71,6 +71,10 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline duration")
     ap.add_argument("--forms", default=None, help="forms to time (headline = last)")
     ap.add_argument("--train-cells", type=int, default=None, help="threshold-model training cells")
+    ap.add_argument("--quantile", type=float, default=0.995,
+                    help="held-out rows whose learned threshold must stay below their k-th similarity")
+    ap.add_argument("--model-cache", default=None,
+                    help="threshold model file: loaded when it exists, else trained and written there")
     return ap.parse_args()
 
 
@@ -128,6 +132,8 @@ class DeviceRun:
         if form.endswith("+learned"):
             p.mlp_params = self.model_params.data_ptr()
             p.mlp_offset = float(self.model.offset)
+            if self.model.offset2 is not None:
+                p.mlp_offset2 = float(self.model.offset2)
             p.mlp_bf16 = 1
         st = self.lib.KnnStats()
         rc = self.L.costal_knn_exact_device(
@@ -267,7 +273,13 @@ def main():
         from costal_amd import learned
 
         t0 = time.perf_counter()
-        model = learned.train(train_data(), k, n_target=n, seed=0)
+        if args.model_cache and os.path.exists(args.model_cache):
+            model = learned.ThresholdModel.load(args.model_cache)
+            model.meta.update({"k_label": None, "holdout_rmse": None, "quantile": None, "loaded_from": args.model_cache})
+        else:
+            model = learned.train(train_data(), k, n_target=n, seed=0, quantile=args.quantile)
+            if args.model_cache:
+                model.save(args.model_cache)
         run.set_model(model)
         model_info = {"train_cells": train_cells, "k_label": model.meta["k_label"],
                       "holdout_rmse": model.meta["holdout_rmse"], "offset": model.offset,
@@ -295,16 +307,28 @@ def main():
             "w_max_rel": float(np.max(np.abs(o[3] - ref[3]) / np.maximum(np.abs(ref[3]), 1e-300))),
         }
 
-    # roofline of the dominant kernel: algorithmic flops 2*N*N*d per launch (all ordered pairs)
+    # Roofline of the dominant kernel.  Algorithmic flops: 2*d per (query, candidate) pair the
+    # kernel scores.  The dense algorithm scores all N*N ordered pairs (2*N*N*d, as the reference
+    # computes them); the pruned scan scores only the (query block, candidate tile) pairs it does
+    # not skip, so its flops are the scanned fraction of that (the dense-equivalent rate is
+    # reported beside it).
     hk = results[head]["kernel_ms"]
     path = results[head]["path"]
-    kern_name = {0: "coarse_kernel", 1: "rowscan_kernel", 2: "batched graph (prep+scan+rescore)"}[path]
+    if batched:
+        kern_name = "batched graph (prep+scan+rescore)"
+    else:
+        kern_name = {0: "coarse_kernel", 1: "rowscan_kernel", 2: "wide_gemm_kernel + wide_scan_kernel"}[path]
     kern_ms = hk["coarse"] if path != 1 else hk["fallback"]
-    achieved = flops / (kern_ms / 1000.0) / 1e12 if kern_ms > 0 else 0.0
+    pairs, skipped = results[head]["tile_pairs"], results[head]["tile_pairs_skipped"]
+    scanned = 1.0 - skipped / pairs if (path == 0 and not batched and pairs > 0) else 1.0
+    exec_flops = flops * scanned
+    achieved = exec_flops / (kern_ms / 1000.0) / 1e12 if kern_ms > 0 else 0.0
     peak = PEAK_TFLOPS[PRECISION_OF[head]]
     roofline = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                 "frac": achieved / peak, "traffic": MEASURED_TRAFFIC.get((args.workload, head)), "kernel": kern_name,
-                "flops_per_launch": flops, "avg_launch_ms": kern_ms}
+                "flops_per_launch": exec_flops, "avg_launch_ms": kern_ms, "scanned_fraction": scanned,
+                "dense_flops_per_launch": flops,
+                "dense_equivalent_tflops": flops / (kern_ms / 1000.0) / 1e12 if kern_ms > 0 else 0.0}
 
     h2h = None
     if not batched:

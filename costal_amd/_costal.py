@@ -139,6 +139,8 @@ def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None
         mp = np.ascontiguousarray(threshold_model.params, dtype=np.float32)
         p.mlp_params = mp.ctypes.data
         p.mlp_offset = float(threshold_model.offset)
+        if getattr(threshold_model, "offset2", None) is not None:
+            p.mlp_offset2 = float(threshold_model.offset2)
         p.mlp_bf16 = int(bool(threshold_bf16))
     idx = np.empty((n, max(k, 0)), np.uint32)
     cos = np.empty((n, max(k, 0)), np.float64)

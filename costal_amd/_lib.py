@@ -60,7 +60,7 @@ class KnnParams(C.Structure):
         ("mlp_offset", C.c_float),
         ("mlp_bf16", C.c_uint32),
         ("prune_clusters", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("mlp_offset2", C.c_float),
     ]
 
 

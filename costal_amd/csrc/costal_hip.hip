@@ -959,6 +959,34 @@ __global__ void rescan_thresholds_kernel(const uint32_t* __restrict__ rows, cons
   th[rows[q]] = t;
 }
 
+// Second-chance thresholds of the rows left short by the learned ones: th = 1 - e - margin with
+// e = exp(u + offset), so e * exp(offset2 - offset) = exp(u + offset2).  Any threshold keeps the
+// result exact (a row short again is recomputed by the row scan); this one only has to be low.
+__global__ void second_chance_kernel(const uint32_t* __restrict__ rows, uint32_t nrows, const float* __restrict__ th,
+                                     float margin, float factor, float* __restrict__ th2) {
+  const uint32_t q = bid_x() * bdim_x() + tid_x();
+  if (q >= nrows) return;
+  const uint32_t r = rows[q];
+  th2[r] = 1.0f - (1.0f - th[r] - margin) * factor - margin;
+}
+
+// The failures of a rescan split by kind: rows with a k-th score (lo finite) to (bounded,
+// bounded_lo), rows with fewer than k candidates to unbounded; counts in *nb, *nu.
+__global__ void split_failures_kernel(const uint32_t* __restrict__ rows, const double* __restrict__ lo, uint32_t nrows,
+                                      uint32_t* __restrict__ bounded, double* __restrict__ bounded_lo,
+                                      uint32_t* __restrict__ nb, uint32_t* __restrict__ unbounded,
+                                      uint32_t* __restrict__ nu) {
+  const uint32_t q = bid_x() * bdim_x() + tid_x();
+  if (q >= nrows) return;
+  if (lo[q] > -INFINITY) {
+    const uint32_t p = atomicAdd(nb, 1u);
+    bounded[p] = rows[q];
+    bounded_lo[p] = lo[q];
+  } else {
+    unbounded[atomicAdd(nu, 1u)] = rows[q];
+  }
+}
+
 // out = the rows with a finite threshold, then the others (one workgroup; the order within each
 // kind is not kept).
 __global__ __launch_bounds__(1024) void partition_rows_kernel(const uint32_t* __restrict__ rows, uint32_t nrows,
@@ -1088,6 +1116,7 @@ struct Device {
   // call start/end and per-kernel brackets: prep, coarse, rescore, fallback
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evk[8] = {};
+  hipEvent_t evcopy = nullptr;  // a device-to-host copy done (the stream may run on behind it)
 };
 
 // One state per device, selected by the device current at the call (a process may drive several
@@ -1122,6 +1151,7 @@ int device_init(Device& D) {
   HIP_TRY(hipStreamCreateWithFlags(&D.stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&D.ev0));
   HIP_TRY(hipEventCreate(&D.ev1));
+  HIP_TRY(hipEventCreateWithFlags(&D.evcopy, hipEventDisableTiming));
   for (auto& e : D.evk) HIP_TRY(hipEventCreate(&e));
   // the device's own address of the static scratch (the device is current here)
   D.scratch_err = hipGetSymbolAddress(&D.scratch, HIP_SYMBOL(scratch));
@@ -1446,6 +1476,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     c.take<double>(n);
     c.take<uint32_t>(n);
     c.take<double>(n);
+    if (P.fused) c.take<uint32_t>(n);
     c.take<uint32_t>(COUNTER_WORDS);
     if (learned) c.take<float>(n);
     if (P.fused) c.take<float>(n);
@@ -1498,9 +1529,11 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   double* fb_lo = c.take<double>(n);
   uint32_t* fb2_rows = c.take<uint32_t>(n);
   double* fb2_lo = c.take<double>(n);
+  uint32_t* fb3_rows = P.fused ? c.take<uint32_t>(n) : nullptr;
   // [0] first bad row, [1] rows failing the first scan, [2] of them with < k admitted, [3] rows
-  // for the full-row kernel after a threshold-free rescan, [4] short rows of that rescan,
-  // [8..9] candidate-list inserts (u64)
+  // failing the rescans, [4] short rows of them, [5] and [6] those with a k-th score and the
+  // short ones (after a second-chance rescan), [7] rows failing the rescan of the short ones,
+  // [12] short rows of that, [8..9] candidate-list inserts (u64), [10..11] tiles scanned (u64)
   uint32_t* counters = c.take<uint32_t>(COUNTER_WORDS);
   float* th_ws = learned ? c.take<float>(n) : nullptr;
   float* th_rescan = P.fused ? c.take<float>(n) : nullptr;
@@ -1521,8 +1554,21 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   HIP_TRY(hipEventRecord(D.evk[1], s));
   uint32_t bad = NONE;
   HIP_TRY(hipMemcpyAsync(&bad, counters, sizeof bad, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(hipEventRecord(D.evcopy, s));
+  // The learned thresholds go behind the copy, so that the device works on them while the host
+  // waits for the non-finite check (the model only reads x; its output is unused if x is bad).
+  const bool learned_early = P.fused && learned;
+  if (learned_early) {
+    // lowered by twice the list error bound, so that a row whose k-th similarity the model bounds
+    // correctly also clears the certificate
+    const double eps_l = P.prec == PREC_F16X3 ? fp16x3_eps(d) : fp32_eps(d);
+    rc = costal_mlp_predict_device(x, n, d, p.mlp_params, p.mlp_bf16, p.mlp_offset, (float)(2.0 * eps_l), 0, th_ws,
+                                   s);
+    if (rc) return fail(rc, "threshold model: %s", costal_mlp_last_error());
+  }
+  HIP_TRY(hipEventSynchronize(D.evcopy));
   if (bad != NONE) {
+    if (learned_early) HIP_TRY(hipStreamSynchronize(s));  // nothing still reads x when we return
     if (st) st->nonfinite_row = bad;
     return fail(COSTAL_E_NON_FINITE, "row %u contains NaN or inf", bad);
   }
@@ -1562,6 +1608,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     HIP_TRY(hipGetLastError());
     return 0;
   };
+  bool third = false;  // rows left short by a second-chance rescan were rescanned once more
   if (P.fused) {
     CoarseFn fn = coarse_select(P.prec, P.J, P.KP);
     if (!fn) return fail(COSTAL_E_UNSUPPORTED, "no coarse kernel for d=%u", d);
@@ -1583,14 +1630,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
       pr_first.pt = P.order.pt;
       pr_first.scanned = scanned;
     }
-    if (learned) {
-      // learned thresholds, lowered by twice the list error bound so that a row whose k-th
-      // similarity the model bounds correctly also clears the certificate
-      rc = costal_mlp_predict_device(x, n, d, p.mlp_params, p.mlp_bf16, p.mlp_offset, (float)(2.0 * eps_list), 0,
-                                     th_ws, s);
-      if (rc) return fail(rc, "threshold model: %s", costal_mlp_last_error());
-      row_th = th_ws;
-    }
+    if (learned) row_th = th_ws;  // predicted behind the non-finite check
     const uint32_t ntile = (n + P.TC - 1) / P.TC;
     // one scan pass over the queries `rows` (null: every row), then rescoring; rows that fail the
     // certificate are appended to (fbr, fbl) and counted in *fbc
@@ -1637,16 +1677,19 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipMemcpyAsync(fb_rows, fb2_rows, (size_t)nfb * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
       }
-      auto rescan = [&](CoarseFn f, uint32_t kpx, uint32_t* rows, uint32_t nr, const float* th) -> int {
+      // max_splits, and list entries per row for the rescoring (2 * splits * kpx at most max_entries)
+      auto rescan = [&](CoarseFn f, uint32_t kpx, uint32_t* rows, uint32_t nr, const float* th, uint32_t* outr,
+                        double* outl, uint32_t* outc, uint32_t* outs, uint32_t max_splits,
+                        uint32_t max_entries) -> int {
         // about one workgroup per CU: the rows are few, so splits are what fills the GPU, but
         // every split fills its lists from scratch, so more than one per CU was slower (three per
         // CU in the locality order: 24 M -> 37 M inserts on 396,460 x 15, 1.3 -> 1.4 ms)
         const bool pruned = P.prune && nr <= costal::RESCAN_PRUNE_ROWS;
         const uint32_t blocks = (nr + P.QB - 1) / P.QB;
         uint32_t ns = ((uint32_t)std::max(D.cus, 1) + blocks - 1) / blocks;
-        ns = std::min<uint32_t>(ns, 32);
+        ns = std::min<uint32_t>(ns, max_splits);
         ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, ntile / 2));
-        ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, 2048 / (2 * kpx)));
+        ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, max_entries / (2 * kpx)));
         // the lists fit in the list buffers (nq_entries entries, nq_slots lists; one split always
         // fits since nr <= n and kpx <= kp2)
         ns = std::min<uint64_t>(ns, nq_entries / ((uint64_t)nr * 2 * kpx));
@@ -1665,14 +1708,56 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
           pr.pt = P.order.pt;
           ns = std::min<uint32_t>(ns, P.order.npt);
         }
-        return scan_pass(f, kpx, rows, nr, th, ns, tiles_per * P.TC, 0, fb2_rows, fb2_lo, counters + 3,
-                         counters + 4, pr);
+        return scan_pass(f, kpx, rows, nr, th, ns, tiles_per * P.TC, 0, outr, outl, outc, outs, pr);
       };
-      if (nwith > 0 && (rc = rescan(fn2, kp2, fb_rows, nwith, th_rescan))) return rc;
-      if (nfb > nwith && (rc = rescan(fn, P.kp, fb_rows + nwith, nfb - nwith, nullptr))) return rc;
+      if (nwith > 0 && (rc = rescan(fn2, kp2, fb_rows, nwith, th_rescan, fb2_rows, fb2_lo, counters + 3, counters + 4,
+                                    32, RESCORE_CMAX)))
+        return rc;
+      if (nfb > nwith) {
+        // short rows: a second, lower learned threshold when the model has one (so that the rescan
+        // prunes at once), else none
+        const float* th_short = nullptr;
+        if (learned && p.mlp_offset2 != 0.0f && p.mlp_offset2 > p.mlp_offset) {
+          const float margin = (float)(2.0 * eps_list);
+          const float factor = (float)std::exp((double)p.mlp_offset2 - (double)p.mlp_offset);
+          second_chance_kernel<<<(nfb - nwith + 255) / 256, 256, 0, s>>>(fb_rows + nwith, nfb - nwith, th_ws, margin,
+                                                                         factor, th_rescan);
+          HIP_TRY(hipGetLastError());
+          th_short = th_rescan;
+        }
+        if ((rc = rescan(fn, P.kp, fb_rows + nwith, nfb - nwith, th_short, fb2_rows, fb2_lo, counters + 3,
+                         counters + 4, 32, RESCORE_CMAX)))
+          return rc;
+        if (th_short) {
+          // Rows the second chance left short are few, and the latency of their scan is what
+          // counts: a threshold-free rescan with up to 96 splits (and at most 1024 list entries per
+          // row, so that the rescoring of the few rows stays quick), not the row scan (which would
+          // stream every candidate of such a row through one workgroup).
+          uint32_t c34[2] = {0, 0};
+          HIP_TRY(hipMemcpyAsync(c34, counters + 3, sizeof c34, hipMemcpyDeviceToHost, s));
+          HIP_TRY(hipStreamSynchronize(s));
+          if (c34[1] > 0) {
+            split_failures_kernel<<<(c34[0] + 255) / 256, 256, 0, s>>>(fb2_rows, fb2_lo, c34[0], fb_rows, fb_lo,
+                                                                      counters + 5, fb3_rows, counters + 6);
+            HIP_TRY(hipGetLastError());
+            if ((rc = rescan(fn, P.kp, fb3_rows, c34[1], nullptr, fb2_rows, fb2_lo, counters + 7, counters + 12,
+                             96, 1024)))
+              return rc;
+            third = true;
+          }
+        }
+      }
     }
-    rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb2_rows, counters + 3,
-                        0, fb2_lo, eps32, idx, cos, wts, grid_fb);
+    if (third) {
+      rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb_rows, counters + 5,
+                          0, fb_lo, eps32, idx, cos, wts, grid_fb);
+      if (!rc)
+        rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb2_rows, counters + 7,
+                            0, fb2_lo, eps32, idx, cos, wts, grid_fb);
+    } else {
+      rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb2_rows, counters + 3,
+                          0, fb2_lo, eps32, idx, cos, wts, grid_fb);
+    }
     if (rc) return rc;
     HIP_TRY(hipEventRecord(D.evk[5], s));
   } else if (P.wide) {
@@ -1736,7 +1821,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     st->col_splits = (P.fused || P.wide) ? P.nsplit : 1;
     st->path = P.fused ? 0 : P.wide ? 2 : 1;
     st->inserts = (uint64_t)cnt[8] | ((uint64_t)cnt[9] << 32);
-    st->rowscan_rows = P.fused ? cnt[3] : P.wide ? cnt[1] : n;
+    st->rowscan_rows = P.fused ? (third ? cnt[5] + cnt[7] : cnt[3]) : P.wide ? cnt[1] : n;
     st->device_ms = ms;
     auto span = [&](int a, int b) {
       float v = 0.0f;

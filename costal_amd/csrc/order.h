@@ -41,14 +41,14 @@ struct OrderBufs {
   uint32_t* rank;    // n
   uint32_t* hist;    // (C + 1) x row blocks of 256, then the first working row of each
   uint32_t* totals;  // C + 1: rows per center, then the first working row of each
-  float* qcen;       // query blocks x dp
+  double* qcen;      // query blocks x dp: unit cone centers
   double* qang;      // query blocks x 3: cone angle, its cosine, its sine
-  float* tcen;       // candidate tiles x dp
+  double* tcen;      // candidate tiles x dp
   double* tang;      // candidate tiles x 3
   uint32_t* seq;     // query blocks x tiles: tile ids in decreasing bound order
   float* sb;         // query blocks x tiles: the bounds in that order
   // the same for the blocks of a rescan's row list (RESCAN_PRUNE_ROWS / qb blocks at most)
-  float* rcen;
+  double* rcen;
   double* rang;
   uint32_t* rseq;
   float* rsb;

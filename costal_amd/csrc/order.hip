@@ -272,12 +272,13 @@ __device__ __forceinline__ double wave_min(double x) {
 // Cone of positions [g*G, min(n, g*G + G)) of a row list (rows[w]: the original row at position w,
 // e.g. the working order perm) (one wave): center = the normalized f64 sum of the rows, rounded to
 // f32; angle = the largest angle between that f32 center and any row, in f64 with both vectors'
-// own norms, plus CONE_MARGIN; pi when the sum is zero (the cone covers everything).  ang[3g ..]
+// own norms, plus CONE_MARGIN; pi when the sum is zero (the cone covers everything).  cen holds the
+// center as an f64 unit vector (the f32 center over its f64 norm: the same direction), ang[3g ..]
 // = (angle, its cosine, its sine).
 template <int DPM>
 __global__ __launch_bounds__(64) void cone_kernel(const float* __restrict__ xs32, uint32_t dp,
                                                   const uint32_t* __restrict__ rows, uint32_t n, uint32_t G,
-                                                  float* __restrict__ cen, double* __restrict__ ang) {
+                                                  double* __restrict__ cen, double* __restrict__ ang) {
   __shared__ float s_c[DPM];
   const uint32_t g = bid_x(), l = tid_x();
   const uint32_t w0 = g * G, w1 = std::min(n, w0 + G);
@@ -298,7 +299,7 @@ __global__ __launch_bounds__(64) void cone_kernel(const float* __restrict__ xs32
   }
   const double nrm = sqrt(nn);
   if (!(nrm > 0.0)) {
-    if (l < dp) cen[(size_t)g * dp + l] = 0.0f;
+    if (l < dp) cen[(size_t)g * dp + l] = 0.0;
     if (l == 0) ang[3 * g] = M_PI, ang[3 * g + 1] = -1.0, ang[3 * g + 2] = 0.0;
     return;
   }
@@ -306,15 +307,14 @@ __global__ __launch_bounds__(64) void cone_kernel(const float* __restrict__ xs32
     double mine = 0.0;
 #pragma unroll
     for (int f = 0; f < DPM; ++f) mine = (uint32_t)f == l ? acc[f] : mine;
-    const float c = (float)(mine / nrm);
-    s_c[l] = c;
-    if (l < dp) cen[(size_t)g * dp + l] = c;
+    s_c[l] = (float)(mine / nrm);
   }
   __syncthreads();
   double cn = 0.0;
 #pragma unroll
   for (int f = 0; f < DPM; ++f) cn += (double)s_c[f] * (double)s_c[f];
   cn = sqrt(cn);
+  if (l < dp) cen[(size_t)g * dp + l] = (double)s_c[l] / cn;
   double m = 2.0;
   for (uint32_t w = w0 + l; w < w1; w += 64) {
     float v[DPM];
@@ -426,19 +426,16 @@ __device__ __forceinline__ float key_float(uint32_t k) {
 // Bound of every candidate tile for query block bid_x(), sorted in decreasing order (ties: lower
 // tile first): seq[b*npt + r] = tile of rank r, sb[b*npt + r] = its bound, an f32 rounded up.
 // 1024 threads; a value is (bound key << 32) | ~tile, so that one comparison orders both.
-__global__ __launch_bounds__(1024) void bound_sort_kernel(const float* __restrict__ qcen, const double* __restrict__ qang,
-                                                          const float* __restrict__ tcen, const double* __restrict__ tang,
+__global__ __launch_bounds__(1024) void bound_sort_kernel(const double* __restrict__ qcen, const double* __restrict__ qang,
+                                                          const double* __restrict__ tcen, const double* __restrict__ tang,
                                                           uint32_t dp, uint32_t npt, uint32_t p2, uint32_t* __restrict__ seq,
                                                           float* __restrict__ sb) {
   constexpr int EMAX = PRUNE_MAX_TILES / 1024;
   __shared__ uint64_t s_buf[PRUNE_MAX_TILES];
   __shared__ double s_q[64];
   const uint32_t b = bid_x(), t0 = tid_x();
-  if (t0 < 64) s_q[t0] = t0 < dp ? (double)qcen[(size_t)b * dp + t0] : 0.0;
+  if (t0 < 64) s_q[t0] = t0 < dp ? qcen[(size_t)b * dp + t0] : 0.0;
   __syncthreads();
-  double qn = 0.0;
-  for (uint32_t f = 0; f < dp; ++f) qn += s_q[f] * s_q[f];
-  qn = sqrt(qn);
   const double aq = qang[3 * b], cq = qang[3 * b + 1], sq = qang[3 * b + 2];
   const int E = (int)(p2 > 1024 ? p2 / 1024 : 1);
   uint64_t v[EMAX];
@@ -447,20 +444,17 @@ __global__ __launch_bounds__(1024) void bound_sort_kernel(const float* __restric
     v[s] = 0;  // padding: below every real value, sorted last
     const uint32_t t = (uint32_t)s * 1024 + t0;
     if (s < E && t < npt) {
-      const float* tc = tcen + (size_t)t * dp;
-      double dot = 0.0, tn = 0.0;
-      for (uint32_t f = 0; f < dp; ++f) {
-        const double c = (double)tc[f];
-        dot += s_q[f] * c;
-        tn += c * c;
-      }
+      const double* tc = tcen + (size_t)t * dp;
+      double dot = 0.0;
+      for (uint32_t f = 0; f < dp; ++f) dot += s_q[f] * tc[f];
       // cos(max(0, th - a)) with a = aq + at, from the cosines and sines of the angles: th > a iff
-      // cos th < cos a (both in [0, pi]); then cos(th - a) = cos th cos a + sin th sin a
+      // cos th < cos a (both in [0, pi]); then cos(th - a) = cos th cos a + sin th sin a (a cone of
+      // angle pi, a zero sum, covers everything)
       const double at = tang[3 * t], ct = tang[3 * t + 1], st = tang[3 * t + 2];
       double bound = 1.0;
-      if (aq + at < M_PI && qn > 0.0 && tn > 0.0) {
+      if (aq + at < M_PI) {
         const double ca = cq * ct - sq * st, sa = sq * ct + cq * st;
-        const double cth = fmax(-1.0, fmin(1.0, dot / (qn * sqrt(tn))));
+        const double cth = fmax(-1.0, fmin(1.0, dot));
         if (cth < ca) bound = cth * ca + sqrt(fmax(0.0, 1.0 - cth * cth)) * sa;
       }
       bound += BOUND_MARGIN;
@@ -564,14 +558,14 @@ OrderBufs carve(Carver& c, const OrderShape& o) {
   b.rank = c.take<uint32_t>(o.n);
   b.hist = c.take<uint32_t>((size_t)(o.C + 1) * nb);
   b.totals = c.take<uint32_t>(o.C + 1);
-  b.qcen = c.take<float>((size_t)o.nblk * o.dp);
+  b.qcen = c.take<double>((size_t)o.nblk * o.dp);
   b.qang = c.take<double>(3 * (size_t)o.nblk);
-  b.tcen = c.take<float>((size_t)o.npt * o.dp);
+  b.tcen = c.take<double>((size_t)o.npt * o.dp);
   b.tang = c.take<double>(3 * (size_t)o.npt);
   b.seq = c.take<uint32_t>((size_t)o.nblk * o.npt);
   b.sb = c.take<float>((size_t)o.nblk * o.npt);
   const size_t rb = (RESCAN_PRUNE_ROWS + o.qb - 1) / o.qb;
-  b.rcen = c.take<float>(rb * o.dp);
+  b.rcen = c.take<double>(rb * o.dp);
   b.rang = c.take<double>(3 * rb);
   b.rseq = c.take<uint32_t>(rb * o.npt);
   b.rsb = c.take<float>(rb * o.npt);

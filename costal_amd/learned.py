@@ -4,8 +4,10 @@ A 3-layer MLP (d -> 64 -> 64 -> 1, ReLU) on L2-normalized rows predicts u = log(
 for a row's k-th neighbor cosine s_k.  Inside a kNN call the library turns it into a threshold
 t = 1 - exp(u + offset) - 2 eps; candidates scoring at or below t skip the candidate-list insert.
 ``offset`` is calibrated on held-out rows so that the threshold lies below s_k for a chosen fraction
-of rows; every other row is caught by the scan's certificate and recomputed by the plain scan, so
-the result is exact whatever the model predicts.
+of rows; every other row is caught by the scan's certificate and recomputed, so the result is exact
+whatever the model predicts.  ``offset2`` (the largest held-out residual plus a margin) gives the
+rows left with fewer than k candidates a second, lower threshold for their rescan, so that it
+prunes like the first pass; rows short again are recomputed by the exact row scan.
 
 Training data come from the exact method itself: the exact kNN of a training sample gives the
 labels.  When the sample is smaller than the data the model will serve, the k-th neighbor of the
@@ -55,14 +57,17 @@ class ThresholdModel:
     params: np.ndarray
     offset: float
     meta: dict = field(default_factory=dict)
+    offset2: float | None = None  # second-chance offset (None: rescan short rows without threshold)
 
     def save(self, path: str) -> None:
-        np.savez(path, d=self.d, k=self.k, params=self.params, offset=self.offset)
+        np.savez(path, d=self.d, k=self.k, params=self.params, offset=self.offset,
+                 offset2=self.offset if self.offset2 is None else self.offset2)
 
     @classmethod
     def load(cls, path: str) -> "ThresholdModel":
         f = np.load(path, allow_pickle=False)
-        return cls(int(f["d"]), int(f["k"]), f["params"].astype(np.float32), float(f["offset"]))
+        o2 = float(f["offset2"]) if "offset2" in f.files else None
+        return cls(int(f["d"]), int(f["k"]), f["params"].astype(np.float32), float(f["offset"]), offset2=o2)
 
 
 def _torch():
@@ -193,6 +198,8 @@ def train(x_train: np.ndarray, k: int, *, n_target: int | None = None, epochs: i
     pred = predict_raw(params, x_train[hold], bf16=True)
     resid = u[hold].astype(np.float64) - pred.astype(np.float64)
     offset = float(np.quantile(resid, quantile))
+    # second chance: above every held-out residual by a margin of 0.1 in log(1 - s) (10% in 1 - s)
+    offset2 = max(offset, float(resid.max())) + 0.1
     meta = {
         "n_train": n,
         "k_label": k_eff,
@@ -204,4 +211,4 @@ def train(x_train: np.ndarray, k: int, *, n_target: int | None = None, epochs: i
         "fit_seconds": t_fit,
         "losses": losses,
     }
-    return ThresholdModel(d=d, k=k, params=params, offset=offset, meta=meta)
+    return ThresholdModel(d=d, k=k, params=params, offset=offset, meta=meta, offset2=offset2)

@@ -92,7 +92,10 @@ typedef struct costal_knn_params {
   float mlp_offset;
   uint32_t mlp_bf16;     /* 1 = BF16 inference (products of bf16-rounded values, f32 sums) */
   uint32_t prune_clusters; /* k-means centers of the locality order; 0 = about n/3072 in [16, 256] */
-  uint32_t reserved;
+  /* Second-chance offset of the learned thresholds (0 = none): rows left with fewer than k
+   * candidates rescan with 1 - exp(model(row) + mlp_offset2) - margin when mlp_offset2 is above
+   * mlp_offset (the rows short again go to the exact row scan); otherwise without a threshold. */
+  float mlp_offset2;
 } costal_knn_params;
 
 /* KnnStats (knn.rs:344-357) plus GPU-side counters */

@@ -41,6 +41,13 @@ struct OrderBufs {
   uint32_t* rank;    // n
   uint32_t* hist;    // (C + 1) x row blocks of 256, then the first working row of each
   uint32_t* totals;  // C + 1: rows per center, then the first working row of each
+  uint32_t* sub_cnt; // C + 1: sub-centers of each first-level center (the zero rows: one)
+  uint32_t* sub_off; // C + 1: the first sub-center of each
+  float* sub_cen;    // G2 x dp
+  uint32_t* key2;    // n: the sub-center of each working row, then sorted
+  uint32_t* key2s;   // n
+  uint32_t* perm2;   // n: radix-sort buffers
+  uint32_t* rank2;   // n
   double* qcen;      // query blocks x dp: unit cone centers
   double* qang;      // query blocks x 3: cone angle, its cosine, its sine
   double* tcen;      // candidate tiles x dp
@@ -59,6 +66,11 @@ struct OrderShape {
   uint32_t C;          // k-means centers
   uint32_t sample;     // rows of the k-means sample (spread evenly over all rows)
   uint32_t iters;      // k-means iterations on the sample
+  uint32_t G2;         // sub-centers at most, the zero rows' one included
+  uint32_t sub_cap;    // sub-centers of one first-level center at most
+  uint32_t sub_sample; // rows of a first-level center's sub-center fit at most
+  uint32_t sub_iters;
+  uint32_t key_bits;   // bits of a sub-center id
   uint32_t qb;         // rows of a query block
   uint32_t pt;         // rows of a candidate tile (PRUNE_TILE or twice that)
   uint32_t nblk, npt;  // query blocks, candidate tiles
@@ -74,6 +86,10 @@ size_t order_bytes(const OrderShape& o);
 OrderBufs order_carve(void* base, const OrderShape& o);
 
 // Working order of the rows of xs32 (normalized f32 rows, dp per row, original order): perm/iperm.
+// Two levels: rows sorted by their nearest first-level center, then, within each first-level
+// center's rows, by their nearest of its sub-centers (about one per candidate tile of rows,
+// spherical k-means on a sample of those rows), so that consecutive rows, and so every query block
+// and candidate tile, are tight groups.
 hipError_t order_rows(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b);
 // Working copies in the working order, rows [n, n_pad) zero: xs32w (f32, dp32) and/or xs16w
 // (FP16, dp16 per row, or hi then lo rows of dp16 when split16).  Either may be null.

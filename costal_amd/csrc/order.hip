@@ -6,6 +6,7 @@
 
 #include <math.h>
 
+
 #include <algorithm>
 #include <type_traits>
 
@@ -179,17 +180,6 @@ __global__ __launch_bounds__(256) void km_assign_kernel(const float* __restrict_
   for (uint32_t c = tid_x(); c <= C; c += 256) {
     hist[(size_t)c * nb + bid_x()] = s_h[c];
     if (s_h[c]) atomicAdd(&totals[c], s_h[c]);
-  }
-}
-
-// First working row of every center: exclusive prefix sum of the C + 1 totals, in place.
-__global__ void center_start_kernel(uint32_t* __restrict__ totals, uint32_t m) {
-  if (tid_x() != 0) return;
-  uint32_t run = 0;
-  for (uint32_t c = 0; c < m; ++c) {
-    const uint32_t v = totals[c];
-    totals[c] = run;
-    run += v;
   }
 }
 
@@ -497,6 +487,206 @@ __global__ __launch_bounds__(1024) void sort_rows_kernel(uint32_t* __restrict__ 
   }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Second level of the working order.
+
+// Sub-centers of every first-level center: about one per `rows` of its rows, at least one, at most
+// cap (the zero rows' bucket C: one), and their offsets (one workgroup, C + 1 <= 1024).
+__global__ __launch_bounds__(1024) void sub_plan_kernel(const uint32_t* __restrict__ starts, uint32_t C, uint32_t n,
+                                                        uint32_t rows, uint32_t cap, uint32_t* __restrict__ sub_cnt,
+                                                        uint32_t* __restrict__ sub_off) {
+  __shared__ uint32_t s[1024];
+  const uint32_t c = tid_x();
+  uint32_t m = 0;
+  if (c <= C) {
+    const uint32_t size = (c < C ? starts[c + 1] : n) - starts[c];
+    m = c < C ? std::max<uint32_t>(1, std::min<uint32_t>(cap, (size + rows / 2) / rows)) : 1;
+    sub_cnt[c] = m;
+  }
+  s[c] = m;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024; off <<= 1) {
+    const uint32_t add = c >= off ? s[c - off] : 0u;
+    __syncthreads();
+    s[c] += add;
+    __syncthreads();
+  }
+  if (c <= C) sub_off[c] = s[c] - m;
+}
+
+// Spherical k-means of the sub-centers of first-level center bid_x() on a sample of its rows
+// (working rows [starts[c], starts[c+1]) at a stride), deterministic as the first level.
+template <int DPM>
+__global__ __launch_bounds__(256) void sub_fit_kernel(const float* __restrict__ xs32, uint32_t dp,
+                                                      const uint32_t* __restrict__ perm,
+                                                      const uint32_t* __restrict__ starts, uint32_t C,
+                                                      const uint32_t* __restrict__ sub_cnt,
+                                                      const uint32_t* __restrict__ sub_off, uint32_t sample_max,
+                                                      uint32_t iters, float* __restrict__ sub_cen) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t km_lds[];
+  const uint32_t c = bid_x();
+  const uint32_t m = sub_cnt[c];
+  if (m <= 1) return;
+  const uint32_t s0 = starts[c], size = starts[c + 1] - s0;
+  const uint32_t S = std::min(size, sample_max), stride = std::max<uint32_t>(1, size / S);
+  long long* s_sum = reinterpret_cast<long long*>(km_lds);
+  float* s_cen = reinterpret_cast<float*>(s_sum + (size_t)m * DPM);
+  uint32_t* s_cnt = reinterpret_cast<uint32_t*>(s_cen + (size_t)m * DPM);
+  for (uint32_t e = tid_x(); e < m * DPM; e += 256) {
+    const uint32_t j = e / DPM, f = e % DPM;
+    const uint32_t row = perm[s0 + (uint64_t)j * S / m * stride];
+    s_cen[e] = f < dp ? xs32[(size_t)row * dp + f] : 0.0f;
+  }
+  for (uint32_t it = 0; it < iters; ++it) {
+    for (uint32_t e = tid_x(); e < m * DPM; e += 256) s_sum[e] = 0;
+    for (uint32_t j = tid_x(); j < m; j += 256) s_cnt[j] = 0;
+    __syncthreads();
+    for (uint32_t r = tid_x(); r < S; r += 256) {
+      float v[DPM];
+      load_row<DPM>(xs32 + (size_t)perm[s0 + r * stride] * dp, dp, v);
+      const uint32_t j = nearest<DPM>(v, s_cen, m);
+      if (j != NO_CENTER) {
+#pragma unroll
+        for (int f = 0; f < DPM; ++f)
+          if (v[f] != 0.0f)
+            atomicAdd(reinterpret_cast<unsigned long long*>(&s_sum[j * DPM + f]),
+                      (unsigned long long)(long long)rint((double)v[f] * KM_SCALE));
+        atomicAdd(&s_cnt[j], 1u);
+      }
+    }
+    __syncthreads();
+    for (uint32_t j = tid_x(); j < m; j += 256) {
+      if (s_cnt[j] == 0) continue;
+      double nn = 0.0;
+      for (uint32_t f = 0; f < dp; ++f) nn += (double)s_sum[j * DPM + f] * (double)s_sum[j * DPM + f];
+      if (nn > 0.0) {
+        const double inv = 1.0 / sqrt(nn);
+        for (uint32_t f = 0; f < dp; ++f) s_cen[j * DPM + f] = (float)((double)s_sum[j * DPM + f] * inv);
+      }
+    }
+    __syncthreads();
+  }
+  for (uint32_t e = tid_x(); e < m * DPM; e += 256) {
+    const uint32_t j = e / DPM, f = e % DPM;
+    if (f < dp) sub_cen[((size_t)sub_off[c] + j) * dp + f] = s_cen[e];
+  }
+}
+
+// key2[w] = the global sub-center id of working row w (its first-level center's first sub-center
+// plus its nearest sub-center there; ties to the lowest).
+template <int DPM>
+__global__ __launch_bounds__(256) void sub_assign_kernel(const float* __restrict__ xs32, uint32_t dp,
+                                                         const uint32_t* __restrict__ perm,
+                                                         const uint32_t* __restrict__ assign, uint32_t n, uint32_t C,
+                                                         const uint32_t* __restrict__ sub_cnt,
+                                                         const uint32_t* __restrict__ sub_off,
+                                                         const float* __restrict__ sub_cen, uint32_t* __restrict__ key2) {
+  const uint32_t w = bid_x() * 256 + tid_x();
+  if (w >= n) return;
+  const uint32_t row = perm[w];
+  const uint32_t c = assign[row];
+  const uint32_t m = sub_cnt[c], base = sub_off[c];
+  uint32_t g = base;
+  if (c < C && m > 1) {
+    float v[DPM];
+    load_row<DPM>(xs32 + (size_t)row * dp, dp, v);
+    float best = -INFINITY;
+    for (uint32_t j = 0; j < m; ++j) {
+      float u[DPM];
+      load_row<DPM>(sub_cen + ((size_t)base + j) * dp, dp, u);
+      float sc = 0.0f;
+#pragma unroll
+      for (int f = 0; f < DPM; ++f) sc = __builtin_fmaf(v[f], u[f], sc);
+      if (sc > best) {
+        best = sc;
+        g = base + j;
+      }
+    }
+  }
+  key2[w] = g;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Stable LSD radix sort of (key, value) pairs, RADIX_BITS per pass, by counting: per 256-row block
+// the digit histogram (digit-major, hist[digit * nb + block]) and every row's rank among the
+// block's rows of the same digit (wave ballots: lanes of equal digit, then the earlier waves'
+// counts), the digits' first slots (exclusive scan of the totals), the blocks' first slots within
+// each digit (block_offsets_kernel), then the scatter.
+constexpr uint32_t RADIX_BITS = 8, RADIX_B = 1u << RADIX_BITS;
+
+__global__ __launch_bounds__(256) void digit_count_kernel(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift,
+                                                          uint32_t* __restrict__ rank, uint32_t* __restrict__ hist,
+                                                          uint32_t* __restrict__ totals) {
+  __shared__ uint32_t s_cnt[4][RADIX_B];
+  const uint32_t t = tid_x(), wv = t >> 6, lane = t & 63;
+  for (uint32_t e = t; e < 4 * RADIX_B; e += 256) (&s_cnt[0][0])[e] = 0;
+  __syncthreads();
+  const uint32_t i = bid_x() * 256 + t;
+  const bool valid = i < n;
+  const uint32_t dg = valid ? (keys[i] >> shift) & (RADIX_B - 1) : 0;
+  // lanes of this wave with the same digit (valid rows only)
+  uint64_t same = __builtin_amdgcn_ballot_w64(valid);
+#pragma unroll
+  for (uint32_t b = 0; b < RADIX_BITS; ++b) {
+    const bool bit = (dg >> b) & 1;
+    const uint64_t bal = __builtin_amdgcn_ballot_w64(bit);
+    same &= bit ? bal : ~bal;
+  }
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const uint32_t r_in = (uint32_t)__builtin_popcountll(same & below);
+  if (valid && r_in == 0) s_cnt[wv][dg] = (uint32_t)__builtin_popcountll(same);
+  __syncthreads();
+  if (valid) {
+    uint32_t r = r_in;
+    for (uint32_t v = 0; v < wv; ++v) r += s_cnt[v][dg];
+    rank[i] = r;
+  }
+  const uint32_t nb = gdim_x();
+  for (uint32_t d = t; d < RADIX_B; d += 256) {
+    const uint32_t c = s_cnt[0][d] + s_cnt[1][d] + s_cnt[2][d] + s_cnt[3][d];
+    hist[(size_t)d * nb + bid_x()] = c;
+    if (c) atomicAdd(&totals[d], c);
+  }
+}
+
+__global__ void digit_scatter_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint32_t n,
+                                     uint32_t shift, const uint32_t* __restrict__ rank,
+                                     const uint32_t* __restrict__ first, uint32_t* __restrict__ keys_out,
+                                     uint32_t* __restrict__ vals_out) {
+  const uint32_t i = bid_x() * bdim_x() + tid_x();
+  if (i >= n) return;
+  const uint32_t k = keys[i];
+  const uint32_t pos = first[(size_t)((k >> shift) & (RADIX_B - 1)) * gdim_x() + i / 256] + rank[i];
+  keys_out[pos] = k;
+  vals_out[pos] = vals[i];
+}
+
+__global__ void clear_kernel(uint32_t* __restrict__ v, uint32_t m) {
+  const uint32_t e = bid_x() * bdim_x() + tid_x();
+  if (e < m) v[e] = 0;
+}
+
+// In-place exclusive prefix sum of m <= 1024 counts (one workgroup).
+__global__ __launch_bounds__(1024) void small_scan_kernel(uint32_t* __restrict__ v, uint32_t m) {
+  __shared__ uint32_t s[1024];
+  const uint32_t t = tid_x();
+  const uint32_t x = t < m ? v[t] : 0u;
+  s[t] = x;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024; off <<= 1) {
+    const uint32_t add = t >= off ? s[t - off] : 0u;
+    __syncthreads();
+    s[t] += add;
+    __syncthreads();
+  }
+  if (t < m) v[t] = s[t] - x;
+}
+
+__global__ void invert_kernel(const uint32_t* __restrict__ perm, uint32_t n, uint32_t* __restrict__ iperm) {
+  const uint32_t w = bid_x() * bdim_x() + tid_x();
+  if (w < n) iperm[perm[w]] = w;
+}
+
 template <class F>
 hipError_t by_dpm(uint32_t dp, F f) {
   if (dp <= 16) return f(std::integral_constant<int, 16>{});
@@ -524,14 +714,27 @@ OrderShape order_shape(uint32_t n, uint32_t dp, uint32_t qb, uint32_t clusters) 
   o.C = std::max<uint32_t>(1, std::min<uint32_t>({C, cmax, std::max<uint32_t>(1, n / 4)}));
   o.sample = std::min<uint32_t>(n, 32768);
   o.iters = 5;
+  // second level: one sub-center per candidate tile of rows (set below), fitted on at most 2048 rows
+  o.sub_cap = std::min<uint32_t>(1024, KM_LDS_MAX / (KM_LDS_BYTES * dpm_of(dp)) - 1);
+  // (the fit's effort barely matters: 0, 1 or 4 iterations, 512 or 2048 rows scan within 16.1-16.3%
+  // of the tile pairs on 396,460 x 15)
+  o.sub_sample = 512;
+  o.sub_iters = 1;
   o.qb = qb;
   o.nblk = (n + qb - 1) / qb;
   o.pt = (n + PRUNE_TILE - 1) / PRUNE_TILE <= PRUNE_MAX_TILES ? PRUNE_TILE : 2 * PRUNE_TILE;
   o.npt = (n + o.pt - 1) / o.pt;
+  // sub-centers: at most round(size / pt) per center (<= size / pt + 1/2), summed over C centers,
+  // plus one for the zero rows
+  o.G2 = n / o.pt + o.C + 2;
+  o.key_bits = 1;
+  while ((1ull << o.key_bits) < o.G2) ++o.key_bits;
   return o;
 }
 
-bool order_supported(const OrderShape& o) { return o.dp <= 64 && o.npt >= 8 && o.npt <= PRUNE_MAX_TILES; }
+bool order_supported(const OrderShape& o) {
+  return o.dp <= 64 && o.npt >= 8 && o.npt <= PRUNE_MAX_TILES && o.key_bits <= 2 * 8;
+}
 
 namespace {
 struct Carver {
@@ -556,8 +759,15 @@ OrderBufs carve(Carver& c, const OrderShape& o) {
   b.counts = c.take<uint32_t>(o.C);
   b.assign = c.take<uint32_t>(o.n);
   b.rank = c.take<uint32_t>(o.n);
-  b.hist = c.take<uint32_t>((size_t)(o.C + 1) * nb);
-  b.totals = c.take<uint32_t>(o.C + 1);
+  b.hist = c.take<uint32_t>((size_t)std::max<uint32_t>(o.C + 1, RADIX_B) * nb);
+  b.totals = c.take<uint32_t>(std::max<uint32_t>(o.C + 1, RADIX_B));
+  b.sub_cnt = c.take<uint32_t>(o.C + 1);
+  b.sub_off = c.take<uint32_t>(o.C + 1);
+  b.sub_cen = c.take<float>((size_t)o.G2 * o.dp);
+  b.key2 = c.take<uint32_t>(o.n);
+  b.key2s = c.take<uint32_t>(o.n);
+  b.perm2 = c.take<uint32_t>(o.n);
+  b.rank2 = c.take<uint32_t>(o.n);
   b.qcen = c.take<double>((size_t)o.nblk * o.dp);
   b.qang = c.take<double>(3 * (size_t)o.nblk);
   b.tcen = c.take<double>((size_t)o.npt * o.dp);
@@ -604,9 +814,34 @@ hipError_t order_rows(hipStream_t s, const float* xs32, const OrderShape& o, Ord
     const size_t lds_assign = (size_t)o.C * DPM * 4 + (o.C + 1) * 4;
     km_assign_kernel<DPM><<<nb, 256, lds_assign, s>>>(xs32, o.dp, o.n, o.C, b.centers, b.assign, b.rank, b.hist,
                                                        b.totals);
-    center_start_kernel<<<1, 64, 0, s>>>(b.totals, o.C + 1);
+    small_scan_kernel<<<1, 1024, 0, s>>>(b.totals, o.C + 1);
     block_offsets_kernel<<<o.C + 1, 256, 0, s>>>(b.hist, nb, b.totals);
     km_scatter_kernel<<<nb, 256, 0, s>>>(b.assign, b.rank, b.hist, o.n, nb, b.perm, b.iperm);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // second level: after the scatter, totals holds every center's first working row
+    sub_plan_kernel<<<1, 1024, 0, s>>>(b.totals, o.C, o.n, o.pt, o.sub_cap, b.sub_cnt, b.sub_off);
+    const size_t lds_sub = (size_t)o.sub_cap * DPM * 12 + o.sub_cap * 4;
+    sub_fit_kernel<DPM><<<o.C, 256, lds_sub, s>>>(xs32, o.dp, b.perm, b.totals, o.C, b.sub_cnt, b.sub_off,
+                                                  o.sub_sample, o.sub_iters, b.sub_cen);
+    sub_assign_kernel<DPM><<<nb, 256, 0, s>>>(xs32, o.dp, b.perm, b.assign, o.n, o.C, b.sub_cnt, b.sub_off, b.sub_cen,
+                                              b.key2);
+    // stable: rows of one sub-center keep the first level's order.  Two passes of RADIX_BITS (the
+    // sub-center ids have at most 2 * RADIX_BITS bits), (key2, perm) -> (key2s, perm2) -> back.
+    uint32_t* kin = b.key2;
+    uint32_t* vin = b.perm;
+    uint32_t* kout = b.key2s;
+    uint32_t* vout = b.perm2;
+    for (uint32_t shift = 0; shift < 2 * RADIX_BITS; shift += RADIX_BITS) {
+      clear_kernel<<<1, RADIX_B, 0, s>>>(b.totals, RADIX_B);
+      digit_count_kernel<<<nb, 256, 0, s>>>(kin, o.n, shift, b.rank2, b.hist, b.totals);
+      small_scan_kernel<<<1, 1024, 0, s>>>(b.totals, RADIX_B);
+      block_offsets_kernel<<<RADIX_B, 256, 0, s>>>(b.hist, nb, b.totals);
+      digit_scatter_kernel<<<nb, 256, 0, s>>>(kin, vin, o.n, shift, b.rank2, b.hist, kout, vout);
+      std::swap(kin, kout);
+      std::swap(vin, vout);
+    }
+    invert_kernel<<<nb, 256, 0, s>>>(b.perm, o.n, b.iperm);
     return hipGetLastError();
   });
 }

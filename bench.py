@@ -45,10 +45,14 @@ FORM_DESC = {
 }
 PRECISION_OF = {"fp32": "fp32", "fp16": "fp16", "fp16x3": "fp16x3", "fp16x3+learned": "fp16x3"}
 # HBM bytes per launch of the dominant kernel: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB), the scale
-# factors from a 1 GiB device copy under the same counters; two rocprofv3 --pmc runs of this script
-# (--forms fp16x3+learned --steps 2), mean over the two steps' first passes.
+# factors from a 1 GiB device copy under the same counters; rocprofv3 --pmc runs of this script,
+# mean over the first passes of the run (profiles/, one summary per kernel and workload).
 MEASURED_TRAFFIC = {
-    ("rare_396460x15", "fp16x3+learned"): ((1_285_230 + 1_278_944) / 2 * 2.0 + 68_493) * 1024,
+    ("rare_396460x15", "fp16x3+learned"): 8438.4e6,
+    ("learned_1000000x40", "fp16x3+learned"): 95592.9e6,
+    ("cytometry_86864x39", "fp16x3"): 1842.7e6,
+    ("counts_3605x2000", "fp32"): 1014.4e6 + 57.3e6,
+    ("batched_64", "fp16x3"): 2804.8e6,
 }
 WORKLOADS = {
     # name: (k, forms, train the threshold model on this many other cells)

@@ -1,10 +1,13 @@
-"""Autotuner for the fused scan: coarse-tile precision, candidate-list width and candidate splits.
+"""Autotuner for the fused scan: coarse-tile precision, candidate-list width, candidate splits and
+the locality order of the pruned scan (its k-means center count, or no pruning).
 
 For a shape bucket (feature count rounded up to 8, row count rounded to a power of two, k) it times
-the library on synthetic data of that shape (an arcsinh Gaussian mixture like cytometry data) over
-a small grid of configurations and keeps the fastest.  Results are cached in a JSON file
+the whole call (ordering, scan, rescoring, fallback) on synthetic data of that shape (an arcsinh
+Gaussian mixture like cytometry data) over a small grid of configurations and keeps the fastest:
+first precision x list margin x splits under the default order, then, for the best of those, other
+k-means center counts and the unpruned scan.  Results are cached in a JSON file
 (``$COSTAL_TUNE_CACHE``, default ``~/.cache/costal_amd/tune.json``).  None of these knobs can change
-the output; they only trade candidate-list inserts, rescoring work and parallelism.
+the output; they only trade candidate-list inserts, rescoring work, skipped tiles and parallelism.
 """
 
 from __future__ import annotations
@@ -38,13 +41,24 @@ def _synthetic(n: int, d: int, seed: int = 0) -> np.ndarray:
     return np.ascontiguousarray(np.arcsinh(x / 5.0), dtype=np.float32)
 
 
+KNOBS = ("precision", "margin", "col_splits", "prune_clusters", "prune")
+
+
 def candidates(k: int):
     margins = sorted({max(0, k // 2), k, min(2 * k, 64 - k)})
     margins = [m for m in margins if k + m <= 64]
     for prec in ("fp16x3", "fp32"):
         for m in margins:
             for ns in (0, 1, 4):
-                yield {"precision": prec, "margin": m, "col_splits": ns}
+                yield {"precision": prec, "margin": m, "col_splits": ns, "prune_clusters": 0, "prune": True}
+
+
+def order_candidates(best: dict):
+    """The order knobs for the best configuration of the first stage: k-means centers (0 = the
+    library's default, about n/3072) and the unpruned scan."""
+    for c in (32, 128):
+        yield {**best, "prune_clusters": c, "prune": True}
+    yield {**best, "prune_clusters": 0, "prune": False}
 
 
 def time_config(x_dev, n: int, d: int, k: int, cfg: dict, outs, reps: int = 3) -> float:
@@ -54,13 +68,15 @@ def time_config(x_dev, n: int, d: int, k: int, cfg: dict, outs, reps: int = 3) -
     p.margin = cfg["margin"]
     p.precision = _lib.PRECISION[cfg["precision"]]
     p.col_splits = cfg["col_splits"]
+    p.prune_clusters = cfg.get("prune_clusters", 0)
+    p.no_prune = 0 if cfg.get("prune", True) else 1
     st = _lib.KnnStats()
     times = []
     for r in range(reps + 1):
         _lib.check(L.costal_knn_exact_device(C.c_void_p(x_dev.data_ptr()), n, d, C.byref(p),
                                              *[C.c_void_p(o.data_ptr()) for o in outs], C.byref(st), None))
         if r:
-            times.append(st.coarse_ms + st.rescore_ms + st.fallback_ms)
+            times.append(st.device_ms)
     return float(np.median(times))
 
 
@@ -71,7 +87,8 @@ def choose(results: list, tie: float = 0.03) -> dict:
     noise of one timing run."""
     fastest = min(r["ms"] for r in results)
     near = [r for r in results if r["ms"] <= (1.0 + tie) * fastest]
-    return min(near, key=lambda r: (r["margin"], r["precision"] != "fp16x3", r["col_splits"] != 0, r["col_splits"]))
+    return min(near, key=lambda r: (r["margin"], r["precision"] != "fp16x3", r["col_splits"] != 0, r["col_splits"],
+                                    not r.get("prune", True), r.get("prune_clusters", 0) != 0))
 
 
 def tune_shape(n: int, d: int, k: int, *, reps: int = 3, seed: int = 0) -> dict:
@@ -87,8 +104,11 @@ def tune_shape(n: int, d: int, k: int, *, reps: int = 3, seed: int = 0) -> dict:
         ms = time_config(xd, n, d, k, cfg, outs, reps)
         results.append({**cfg, "ms": ms})
     best = choose(results)
-    return {"best": {kk: best[kk] for kk in ("precision", "margin", "col_splits")}, "ms": best["ms"],
-            "trials": results, "shape": [n, d, k]}
+    for cfg in order_candidates({kk: best[kk] for kk in KNOBS}):
+        ms = time_config(xd, n, d, k, cfg, outs, reps)
+        results.append({**cfg, "ms": ms})
+    best = choose(results)
+    return {"best": {kk: best[kk] for kk in KNOBS}, "ms": best["ms"], "trials": results, "shape": [n, d, k]}
 
 
 class Tuner:
@@ -111,13 +131,15 @@ class Tuner:
         os.replace(tmp, self.path)
 
     def get(self, n: int, d: int, k: int) -> dict:
+        """The kept configuration of the bucket of (n, d, k), every knob present (entries cached
+        before the order knobs existed get the defaults: pruned, default center count)."""
         key = bucket(n, d, k)
         if key not in self.table:
             nb = int(key.split(":")[1][1:])
             db = int(key.split(":")[0][1:])
             self.table[key] = tune_shape(max(nb, k + 2), db, k)
             self.save()
-        return self.table[key]["best"]
+        return {"prune_clusters": 0, "prune": True, **self.table[key]["best"]}
 
     def plan_batch(self, shapes, k: int, fill_blocks: int = 2048):
         """Per-sample (margin, col_splits) and the batch precision: the precision chosen for the
@@ -134,3 +156,10 @@ class Tuner:
         if sum(-(-n // 128) for n, _ in shapes) >= fill_blocks:
             splits = [1] * len(shapes)
         return prec, [pl["margin"] for pl in plans], splits
+
+
+def knn_kwargs(best: dict) -> dict:
+    """Keyword arguments of ``_costal._knn_exact`` for a kept configuration."""
+    matmul = {"fp16x3": "mfma-f16x3", "fp32": "mfma-f32", "fp16": "mfma-f16"}[best["precision"]]
+    return {"matmul": matmul, "margin": best["margin"], "col_splits": best["col_splits"],
+            "prune_clusters": best.get("prune_clusters", 0), "prune": best.get("prune", True)}

@@ -177,6 +177,31 @@ def test_tuner_choice_is_stable_under_timing_noise():
     assert tuner.choose(res) == res[0]
 
 
+def test_tuner_grid_covers_the_order_knobs(tmp_path):
+    """The second stage times the best first-stage configuration under other k-means center counts
+    and unpruned; ties keep the pruned scan with the default center count; cached entries from
+    before the order knobs read as the defaults."""
+    from costal_amd import tuner
+
+    first = list(tuner.candidates(10))
+    assert all(c["prune"] and c["prune_clusters"] == 0 for c in first)
+    assert {c["precision"] for c in first} == {"fp16x3", "fp32"}
+    best = {"precision": "fp16x3", "margin": 5, "col_splits": 0, "prune_clusters": 0, "prune": True}
+    second = list(tuner.order_candidates(best))
+    assert {(c["prune"], c["prune_clusters"]) for c in second} == {(True, 32), (True, 128), (False, 0)}
+    assert all(c["margin"] == 5 and c["precision"] == "fp16x3" for c in second)
+    res = [{**best, "ms": 1.0}] + [{**c, "ms": 0.99} for c in second]
+    assert tuner.choose(res) == res[0]
+    res[3]["ms"] = 0.5
+    assert tuner.choose(res) == res[3]
+    tn = tuner.Tuner(str(tmp_path / "tune.json"))
+    tn.table[tuner.bucket(5000, 40, 10)] = {"best": {"precision": "fp32", "margin": 10, "col_splits": 1}}
+    got = tn.get(5000, 40, 10)
+    assert got["prune"] and got["prune_clusters"] == 0
+    assert tuner.knn_kwargs(got) == {"matmul": "mfma-f32", "margin": 10, "col_splits": 1, "prune_clusters": 0,
+                                     "prune": True}
+
+
 def test_batch_plan_drops_splits_when_samples_fill_the_gpu(tmp_path):
     from costal_amd import tuner
 

@@ -126,15 +126,17 @@ def test_learned_1000000x40_k30(c5_model, oracle):
 
 
 def test_tuner_plan_of_one_bucket_is_exact(oracle):
-    """The autotuner times its grid on the GPU for one shape bucket; the plan it keeps gives the
-    oracle's result (tuning knobs never change the output)."""
+    """The autotuner times its grid on the GPU for one shape bucket (precision, list margin,
+    splits, then k-means center counts and the unpruned scan); the plan it keeps gives the oracle's
+    result (tuning knobs never change the output)."""
     tn = tuner.Tuner(None)
     x = synth.cytometry_86864x39(n=6000)
     best = tn.get(*x.shape, 10)
     print("tuned:", best, flush=True)
-    assert set(best) >= {"precision", "margin", "col_splits"}
-    matmul = {"fp16x3": "mfma-f16x3", "fp32": "mfma-f32", "fp16": "mfma-f16"}[best["precision"]]
-    got, _ = ext._knn_exact(x, 10, matmul=matmul, margin=best["margin"], col_splits=best["col_splits"])
+    assert set(best) >= set(tuner.KNOBS)
+    trials = tn.table[tuner.bucket(*x.shape, 10)]["trials"]
+    assert any(not t["prune"] for t in trials) and any(t["prune_clusters"] for t in trials)
+    got, _ = ext._knn_exact(x, 10, **tuner.knn_kwargs(best))
     want = oracle.knn_bruteforce(x, 10)
     for a, b in zip(got, want):
         np.testing.assert_array_equal(a, b)

@@ -87,6 +87,13 @@ __device__ __forceinline__ double tani_from_cos(double a, double b, double c) {
   return d / (a * a + b * b - d);
 }
 
+// The selection score of a pair from its f64 dot product: the cosine, or the Tanimoto score of
+// that cosine (Selector::rescore, knn.rs:152-153, 257-260)
+__device__ __forceinline__ double score64(double dot, double ni, double nj, bool tani) {
+  const double c = cos64(dot, ni, nj);
+  return tani ? tani_from_cos(ni, nj, c) : c;
+}
+
 
 // (score desc, index asc): the order of sort_cands (knn.rs:589-591).  Scores are never NaN and
 // -0.0 never occurs (cos64 normalizes it), so '>' agrees with total_cmp.
@@ -446,21 +453,24 @@ __device__ __forceinline__ void wave_err_max(double err, unsigned long long* __r
 }
 
 // One query's rescoring (64 threads).  Failed rows are recorded as fb_tag in fb_rows.
-// err_max collects the largest |list score - f64 cosine| over the row's admitted candidates: the
-// measured counterpart of the error bound eps_list that the certificate relies on.
+// err_max collects the largest |list score - f64 score| over the row's admitted candidates: the
+// measured counterpart of the error bound eps_list that the certificate relies on.  Scores are
+// cosines, or with Tanimoto selection Tanimoto scores (ranked by them; the cosines of the k kept
+// are computed again for the output).
 template <int CMAX>
 __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_id, const float* __restrict__ x,
                             uint32_t d, const double* __restrict__ norms, uint32_t k, int kp, uint32_t nsplit,
                             uint32_t qq, uint32_t i, const uint32_t* __restrict__ cand_idx,
                             const float* __restrict__ cand_sc, const uint32_t* __restrict__ cand_cnt,
                             const float* __restrict__ cand_floor, int all_admitted, double eps_list, double eps32,
-                            uint32_t* __restrict__ idx, double* __restrict__ cos_out, double* __restrict__ wts,
-                            uint32_t fb_tag, uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo,
-                            uint32_t* __restrict__ fb_count, uint32_t* __restrict__ short_count,
-                            unsigned long long* __restrict__ err_max) {
+                            uint32_t select, uint32_t* __restrict__ idx, double* __restrict__ cos_out,
+                            double* __restrict__ wts, uint32_t fb_tag, uint32_t* __restrict__ fb_rows,
+                            double* __restrict__ fb_lo, uint32_t* __restrict__ fb_count,
+                            uint32_t* __restrict__ short_count, unsigned long long* __restrict__ err_max) {
   const double ni = norms[i];
   const float* xi = x + (size_t)i * d;
   const int tot_slots = (int)nsplit * kp;
+  const bool tani = select == COSTAL_SELECT_TANIMOTO;
 
   uint32_t total = 0;
   double floor_ = -INFINITY;
@@ -480,7 +490,7 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
     if (r < cand_cnt[slot]) {
       const uint32_t j = cand_idx[slot * kp + r];
       const uint32_t pos = atomicAdd(&s_fill, 1u);
-      const double c = cos64(dot64(xi, x + (size_t)j * d, d), ni, norms[j]);
+      const double c = score64(dot64(xi, x + (size_t)j * d, d), ni, norms[j], tani);
       err = fmax(err, fabs((double)cand_sc[slot * kp + r] - c));
       s_sc[pos] = c;
       s_id[pos] = j;
@@ -528,7 +538,7 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
   if (ok) {
     for (uint32_t t = tid_x(); t < k; t += 64) {
       const uint32_t j = s_id[t];
-      const double c = s_sc[t];
+      const double c = tani ? cos64(dot64(xi, x + (size_t)j * d, d), ni, norms[j]) : s_sc[t];
       idx[(size_t)i * k + t] = j;
       cos_out[(size_t)i * k + t] = c;
       if (wts) wts[(size_t)i * k + t] = tani_from_cos(ni, norms[j], c);
@@ -550,11 +560,13 @@ __device__ void rescore_half(double* __restrict__ s_sc, uint32_t* __restrict__ s
                              const double* __restrict__ norms, uint32_t k, int kp, uint32_t nsplit, uint32_t qq,
                              uint32_t i, const uint32_t* __restrict__ cand_idx, const float* __restrict__ cand_sc,
                              const uint32_t* __restrict__ cand_cnt, const float* __restrict__ cand_floor,
-                             int all_admitted, double eps_list, double eps32, uint32_t* __restrict__ idx,
-                             double* __restrict__ cos_out, double* __restrict__ wts, uint32_t fb_tag,
+                             int all_admitted, double eps_list, double eps32, uint32_t select,
+                             uint32_t* __restrict__ idx, double* __restrict__ cos_out, double* __restrict__ wts,
+                             uint32_t fb_tag,
                              uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo, uint32_t* __restrict__ fb_count,
                              uint32_t* __restrict__ short_count, unsigned long long* __restrict__ err_max) {
   const int tot_slots = (int)nsplit * kp;  // <= 32
+  const bool tani = select == COSTAL_SELECT_TANIMOTO;
   double ni = 0.0, floor_ = -INFINITY;
   uint32_t total = 0;
   if (valid) {
@@ -573,7 +585,7 @@ __device__ void rescore_half(double* __restrict__ s_sc, uint32_t* __restrict__ s
     if (r < cand_cnt[slot]) {
       const uint32_t j = cand_idx[slot * kp + r];
       const uint32_t pos = atomicAdd(s_fill, 1u);
-      const double c = cos64(dot64(x + (size_t)i * d, x + (size_t)j * d, d), ni, norms[j]);
+      const double c = score64(dot64(x + (size_t)i * d, x + (size_t)j * d, d), ni, norms[j], tani);
       err = fmax(err, fabs((double)cand_sc[slot * kp + r] - c));
       s_sc[pos] = c;
       s_id[pos] = j;
@@ -603,7 +615,7 @@ __device__ void rescore_half(double* __restrict__ s_sc, uint32_t* __restrict__ s
   if (ok) {
     for (uint32_t t = lane; t < k; t += 32) {
       const uint32_t j = s_id[t];
-      const double c = s_sc[t];
+      const double c = tani ? cos64(dot64(x + (size_t)i * d, x + (size_t)j * d, d), ni, norms[j]) : s_sc[t];
       idx[(size_t)i * k + t] = j;
       cos_out[(size_t)i * k + t] = c;
       if (wts) wts[(size_t)i * k + t] = tani_from_cos(ni, norms[j], c);
@@ -620,8 +632,8 @@ __global__ __launch_bounds__(64) void rescore_pair_kernel(
     const float* __restrict__ x, uint32_t d, const double* __restrict__ norms, uint32_t k, int kp,
     uint32_t nsplit, const uint32_t* __restrict__ rowlist, uint32_t nq, const uint32_t* __restrict__ cand_idx,
     const float* __restrict__ cand_sc, const uint32_t* __restrict__ cand_cnt, const float* __restrict__ cand_floor,
-    int all_admitted, double eps_list, double eps32, uint32_t* __restrict__ idx, double* __restrict__ cos_out,
-    double* __restrict__ wts, uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo,
+    int all_admitted, double eps_list, double eps32, uint32_t select, uint32_t* __restrict__ idx,
+    double* __restrict__ cos_out, double* __restrict__ wts, uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo,
     uint32_t* __restrict__ fb_count, uint32_t* __restrict__ short_count, unsigned long long* __restrict__ err_max) {
   __shared__ double s_sc[2][32];
   __shared__ uint32_t s_id[2][32];
@@ -631,8 +643,8 @@ __global__ __launch_bounds__(64) void rescore_pair_kernel(
   const bool valid = qq < nq;
   const uint32_t i = valid ? (rowlist ? rowlist[qq] : qq) : 0;
   rescore_half(s_sc[half], s_id[half], &s_fill[half], valid, lane, x, d, norms, k, kp, nsplit, qq, i, cand_idx,
-               cand_sc, cand_cnt, cand_floor, all_admitted, eps_list, eps32, idx, cos_out, wts, i, fb_rows, fb_lo,
-               fb_count, short_count, err_max);
+               cand_sc, cand_cnt, cand_floor, all_admitted, eps_list, eps32, select, idx, cos_out, wts, i, fb_rows,
+               fb_lo, fb_count, short_count, err_max);
 }
 
 template <int CMAX>
@@ -640,7 +652,7 @@ __global__ __launch_bounds__(64) void rescore_kernel(
     const float* __restrict__ x, uint32_t d, const double* __restrict__ norms, uint32_t n,
     uint32_t k, int kp, uint32_t nsplit, const uint32_t* __restrict__ rowlist, uint32_t nq,
     const uint32_t* __restrict__ cand_idx, const float* __restrict__ cand_sc, const uint32_t* __restrict__ cand_cnt,
-    const float* __restrict__ cand_floor, int all_admitted, double eps_list, double eps32,
+    const float* __restrict__ cand_floor, int all_admitted, double eps_list, double eps32, uint32_t select,
     uint32_t* __restrict__ idx, double* __restrict__ cos_out, double* __restrict__ wts,
     uint32_t* __restrict__ fb_rows, double* __restrict__ fb_lo, uint32_t* __restrict__ fb_count,
     uint32_t* __restrict__ short_count, unsigned long long* __restrict__ err_max) {
@@ -649,8 +661,8 @@ __global__ __launch_bounds__(64) void rescore_kernel(
   for (uint32_t qq = bid_x(); qq < nq; qq += gdim_x()) {
     const uint32_t i = rowlist ? rowlist[qq] : qq;
     rescore_row<CMAX>(s_sc, s_id, x, d, norms, k, kp, nsplit, qq, i, cand_idx, cand_sc, cand_cnt, cand_floor,
-                      all_admitted, eps_list, eps32, idx, cos_out, wts, i, fb_rows, fb_lo, fb_count, short_count,
-                      err_max);
+                      all_admitted, eps_list, eps32, select, idx, cos_out, wts, i, fb_rows, fb_lo, fb_count,
+                      short_count, err_max);
     __syncthreads();  // the LDS arrays are refilled by the next row
   }
 }
@@ -723,7 +735,13 @@ __device__ void rowscan_row(RowscanSmem<CAP>& S, const float* __restrict__ x, ui
     if (j < n && j != i) {
       bool pass = true;
       if (COARSE) {
+        // the f32 score of the normalized rows (for Tanimoto selection, of their cosine and the
+        // raw norms in f32, as the fused scan scores it)
         float c32 = dot32(xs32 + (size_t)i * dp32, xs32 + (size_t)j * dp32, dp32);
+        if (select == COSTAL_SELECT_TANIMOTO) {
+          const float a = (float)ni;
+          c32 = tani32(c32, (float)norms[j], a > 0.0f ? __builtin_amdgcn_rcpf(a) : 0.0f);
+        }
         pass = (double)c32 >= S.lo;
       }
       if (pass) {
@@ -895,7 +913,8 @@ __global__ __launch_bounds__(64) void rescore_batch_kernel(
   const int all_admitted = (S.nsplit == 1 && (uint32_t)S.kp >= S.n - 1) ? 1 : 0;
   rescore_row<CMAX>(s_sc, s_id, S.x, S.d, S.norms, k, S.kp, nlists, i, i, cand_idx + S.entry_base,
                     cand_sc + S.entry_base, cand_cnt + S.slot_base, cand_floor + S.slot_base, all_admitted, eps_list,
-                    eps32, S.idx, S.cos, S.wts, g, fb_rows, fb_lo, fb_count, short_count, err_max);
+                    eps32, COSTAL_SELECT_COSINE, S.idx, S.cos, S.wts, g, fb_rows, fb_lo, fb_count, short_count,
+                    err_max);
 }
 
 // rescore_batch_kernel for batches whose rows hold at most 32 list entries: two rows per wave
@@ -918,7 +937,8 @@ __global__ __launch_bounds__(64) void rescore_batch_pair_kernel(
   const int all_admitted = (S.nsplit == 1 && (uint32_t)S.kp >= S.n - 1) ? 1 : 0;
   rescore_half(s_sc[half], s_id[half], &s_fill[half], valid, lane, S.x, S.d, S.norms, k, S.kp, 2 * S.nsplit, i, i,
                cand_idx + S.entry_base, cand_sc + S.entry_base, cand_cnt + S.slot_base, cand_floor + S.slot_base,
-               all_admitted, eps_list, eps32, S.idx, S.cos, S.wts, g, fb_rows, fb_lo, fb_count, short_count, err_max);
+               all_admitted, eps_list, eps32, COSTAL_SELECT_COSINE, S.idx, S.cos, S.wts, g, fb_rows, fb_lo, fb_count,
+               short_count, err_max);
 }
 
 template <int CAP>
@@ -935,6 +955,64 @@ __global__ __launch_bounds__(256) void rowscan_batch_kernel(
     const BatchSample& S = samples[sidx];
     rowscan_row<CAP, true>(Sm, S.x, S.d, xs32 + (size_t)S.row_pad * dp32, dp32, S.norms, S.n, k,
                            COSTAL_SELECT_COSINE, g - S.row_base, lo_arr[r], eps32, S.idx, S.cos, S.wts);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Tanimoto selection inputs of the fused scan.
+
+// The largest row norm (ext[0..1], f64 bits) and the smallest positive one (ext[2..3], the
+// complement of its bits, so that both are maxima from zero), one atomic per wave.
+__global__ __launch_bounds__(256) void norm_extremes_kernel(const double* __restrict__ norms, uint32_t n,
+                                                            unsigned long long* __restrict__ ext) {
+  const uint32_t i = bid_x() * 256 + tid_x();
+  const double v = i < n ? norms[i] : 0.0;
+  unsigned long long hi = (unsigned long long)__double_as_longlong(v);
+  unsigned long long lo = v > 0.0 ? ~(unsigned long long)__double_as_longlong(v) : 0ull;
+  for (int off = 32; off > 0; off >>= 1) {
+    hi = std::max(hi, (unsigned long long)__shfl_xor((long long)hi, off));
+    lo = std::max(lo, (unsigned long long)__shfl_xor((long long)lo, off));
+  }
+  if ((tid_x() & 63) == 0) {
+    if (hi) atomicMax(&ext[0], hi);
+    if (lo) atomicMax(&ext[1], lo);
+  }
+}
+
+// The raw norms of the working rows times `scale` (a power of two that keeps them in f32 range;
+// Tanimoto scores depend on norm ratios only) in f32, rows past n zero, and per block of `half`
+// working rows the smallest and largest scaled norm rounded outward to f32 ((0, 0) past n): one
+// wave per block.
+__global__ __launch_bounds__(64) void tani_norms_kernel(const double* __restrict__ norms,
+                                                        const uint32_t* __restrict__ perm, uint32_t n, uint32_t half,
+                                                        double scale, float* __restrict__ tnorm,
+                                                        float2* __restrict__ trange) {
+  const uint32_t b = bid_x(), l = tid_x();
+  double lo = INFINITY, hi = -INFINITY;
+  for (uint32_t r = l; r < half; r += 64) {
+    const uint32_t w = b * half + r;
+    float f = 0.0f;
+    if (w < n) {
+      const double v = norms[perm ? perm[w] : w] * scale;
+      lo = fmin(lo, v);
+      hi = fmax(hi, v);
+      f = (float)v;
+    }
+    tnorm[w] = f;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, off));
+    hi = fmax(hi, __shfl_xor(hi, off));
+  }
+  if (l == 0) {
+    float2 r = make_float2(0.0f, 0.0f);
+    if (hi >= lo) {
+      r.x = (float)lo;
+      if ((double)r.x > lo) r.x = nextafterf(r.x, -INFINITY);
+      r.y = (float)hi;
+      if ((double)r.y < hi) r.y = nextafterf(r.y, INFINITY);
+    }
+    trange[b] = r;
   }
 }
 
@@ -1232,6 +1310,7 @@ constexpr uint32_t ROWSCAN_K_MAX = 2048;
 
 struct Plan {
   bool fused;
+  bool tani;        // Tanimoto selection (the fused scan of split FP16 rows with Tanimoto lists)
   bool wide;        // wide rows: score blocks (wide_gemm_kernel) + wide_scan_kernel
   uint32_t wide_qb; // query rows per score block (wide)
   int prec;
@@ -1248,8 +1327,9 @@ struct Plan {
 
 using costal::CoarseFn;
 
-CoarseFn coarse_select(int prec, uint32_t J, int KP) {
+CoarseFn coarse_select(int prec, uint32_t J, int KP, bool tani = false) {
   using namespace costal;
+  if (tani) return KP == 16 ? scan_tani_fn_k16(J) : KP == 32 ? scan_tani_fn_k32(J) : scan_tani_fn_k64(J);
   if (prec == PREC_F32) return KP == 16 ? scan_fn_p0_k16(J) : KP == 32 ? scan_fn_p0_k32(J) : scan_fn_p0_k64(J);
   if (prec == PREC_F16) return KP == 16 ? scan_fn_p1_k16(J) : KP == 32 ? scan_fn_p1_k32(J) : scan_fn_p1_k64(J);
   return KP == 16 ? scan_fn_p2_k16(J) : KP == 32 ? scan_fn_p2_k32(J) : scan_fn_p2_k64(J);
@@ -1271,6 +1351,11 @@ double fp16_eps(uint32_t d) {
 }
 
 double fp32_eps(uint32_t d) { return ((double)d + 4.0) * (double)std::ldexp(1.0, -23); }
+
+// Bound on |f32 Tanimoto score - f64 Tanimoto score| for list scores computed from cosines within e
+// of the f64 cosine: the slope of T = c / (s - c) is s / (s - c)^2 <= 2 / (1 - e)^2 for s >= 2 and
+// c <= 1 + e, and the f32 evaluation (tani32: about 16 roundings of a value <= 1 + 3e) adds < 2^-19.
+double tani_eps(double e) { return 2.0 * e / ((1.0 - e) * (1.0 - e)) + std::ldexp(1.0, -19); }
 
 // Split-FP16 bound.  v = f32(x/|x|) (relative error 2^-24), hi = fp16(v), lo = fp16(v - hi): the
 // represented value hi + lo is within a|x| + eta of x per component, a = 2^-22 (1 + 2^-24) + 2^-24.
@@ -1302,11 +1387,14 @@ Plan make_plan(const Device& D, uint64_t n, uint64_t d, uint64_t k, const costal
   Plan P{};
   const uint64_t margin = p.margin < 0 ? k : (uint64_t)p.margin;
   P.kp = list_capacity(n, k, margin);
-  P.prec = (int)p.precision;
+  // Tanimoto selection scans split FP16 rows whatever the precision asked for (the coarse form
+  // never changes the output)
+  P.tani = p.select == COSTAL_SELECT_TANIMOTO;
+  P.prec = P.tani ? PREC_F16X3 : (int)p.precision;
   P.dp32 = round_up((uint32_t)std::max<uint64_t>(d, 1), 8);
   P.dp16 = round_up((uint32_t)std::max<uint64_t>(d, 1), 16);
   const uint32_t dmax = P.prec == PREC_F16 ? 128 : 64;
-  P.fused = p.select == COSTAL_SELECT_COSINE && d <= dmax && P.kp <= FUSED_KP_MAX;
+  P.fused = d <= dmax && P.kp <= FUSED_KP_MAX;
   P.J = P.prec == PREC_F16 ? P.dp16 / 16 : P.prec == PREC_F16X3 ? P.dp16 / 8 : P.dp32 / 8;
   P.KP = list_template(P.kp);
   P.NW = 4;
@@ -1331,7 +1419,7 @@ Plan make_plan(const Device& D, uint64_t n, uint64_t d, uint64_t k, const costal
   // Pruned first pass: the splits take turns over each block's tiles in bound order (at most one
   // split per candidate tile of PRUNE_TILE rows).
   P.prune = false;
-  if (P.fused && !p.no_prune) {
+  if (P.fused && !p.no_prune && !P.tani) {
     P.order = costal::order_shape((uint32_t)n, P.dp32, P.QB, p.prune_clusters);
     if (costal::order_supported(P.order)) {
       P.prune = true;
@@ -1419,6 +1507,8 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   const uint32_t n = (uint32_t)n64, d = (uint32_t)d64, k = (uint32_t)p.k;
   if (p.precision > COSTAL_PREC_FP16X3) return fail(COSTAL_E_INVALID, "unknown precision %u", p.precision);
   if (p.select > COSTAL_SELECT_TANIMOTO) return fail(COSTAL_E_INVALID, "unknown select %u", p.select);
+  if (p.select == COSTAL_SELECT_TANIMOTO && !brute && (p.row_threshold || p.mlp_params))
+    return fail(COSTAL_E_UNSUPPORTED, "admission thresholds are cosine similarities: Tanimoto selection takes none");
   Plan P = make_plan(D, n, d, k, p);
   if (brute) P.fused = P.wide = false;
   // Balanced first pass (tuned splits only): with at least one query block per resident workgroup
@@ -1430,7 +1520,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   // k = 10 lists were slower).
   uint32_t units_per_wg = 0, grid_bal = 0;
   if (P.fused && !P.prune && p.col_splits == 0 && (P.prec == PREC_F32 || (P.prec == PREC_F16X3 && P.KP >= 32))) {
-    CoarseFn f0 = coarse_select(P.prec, P.J, P.KP);
+    CoarseFn f0 = coarse_select(P.prec, P.J, P.KP, P.tani);
     int per_cu = 0;
     if (f0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(f0),
                                                            64 * P.NW, 0) == hipSuccess && per_cu > 0) {
@@ -1481,6 +1571,10 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     if (learned) c.take<float>(n);
     if (P.fused) c.take<float>(n);
     c.take<float>(score_floats);
+    if (P.fused && P.tani) {
+      c.take<float>(P.n_pad);
+      c.take<float2>(P.n_pad / (P.TC / 2));
+    }
     return c.off + 256;
   };
   size_t bytes = workspace_bytes();
@@ -1538,6 +1632,8 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   float* th_ws = learned ? c.take<float>(n) : nullptr;
   float* th_rescan = P.fused ? c.take<float>(n) : nullptr;
   float* scores = P.wide ? c.take<float>(score_floats) : nullptr;
+  float* tnorm = (P.fused && P.tani) ? c.take<float>(P.n_pad) : nullptr;
+  float2* trange = (P.fused && P.tani) ? c.take<float2>(P.n_pad / (P.TC / 2)) : nullptr;
   double* norms = norms_out ? norms_out : norms_ws;
 
   HIP_TRY(hipEventRecord(D.ev0, s));
@@ -1551,9 +1647,14 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     prep_kernel<<<(P.n_pad + 255) / 256, 256, 0, s>>>(x, n, d, P.n_pad, P.dp32, P.dp16, split16, norms, xs32,
                                                       P.prune ? nullptr : xs16, counters);
   HIP_TRY(hipGetLastError());
+  // Tanimoto: the norm extremes come back with the non-finite check (counters[14..17])
+  if (P.fused && P.tani) {
+    norm_extremes_kernel<<<(n + 255) / 256, 256, 0, s>>>(norms, n, reinterpret_cast<unsigned long long*>(counters + 14));
+    HIP_TRY(hipGetLastError());
+  }
   HIP_TRY(hipEventRecord(D.evk[1], s));
-  uint32_t bad = NONE;
-  HIP_TRY(hipMemcpyAsync(&bad, counters, sizeof bad, hipMemcpyDeviceToHost, s));
+  uint32_t head[18] = {};
+  HIP_TRY(hipMemcpyAsync(head, counters, (P.fused && P.tani ? 18 : 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipEventRecord(D.evcopy, s));
   // The learned thresholds go behind the copy, so that the device works on them while the host
   // waits for the non-finite check (the model only reads x; its output is unused if x is bad).
@@ -1567,6 +1668,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     if (rc) return fail(rc, "threshold model: %s", costal_mlp_last_error());
   }
   HIP_TRY(hipEventSynchronize(D.evcopy));
+  const uint32_t bad = head[0];
   if (bad != NONE) {
     if (learned_early) HIP_TRY(hipStreamSynchronize(s));  // nothing still reads x when we return
     if (st) st->nonfinite_row = bad;
@@ -1580,8 +1682,31 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     HIP_TRY(costal::order_bounds(s, xs32, P.order, ob));
     HIP_TRY(hipEventRecord(D.evk[7], s));
   }
+  if (P.fused && P.tani) {
+    // Norms scaled by a power of two that puts the largest in [1, 2); a spread of norms beyond
+    // 2^100 (the smallest would lose bits in f32) takes the exact row scan instead.
+    const uint64_t hib = (uint64_t)head[14] | ((uint64_t)head[15] << 32);
+    const uint64_t lob = (uint64_t)head[16] | ((uint64_t)head[17] << 32);
+    double scale = 1.0;
+    if (hib) {
+      double hi, lo;
+      memcpy(&hi, &hib, sizeof hi);
+      const uint64_t lbits = ~lob;
+      memcpy(&lo, &lbits, sizeof lo);
+      scale = std::ldexp(1.0, -std::ilogb(hi));
+      if (lo * scale < std::ldexp(1.0, -100)) P.fused = false;
+    }
+    if (P.fused) {
+      tani_norms_kernel<<<P.n_pad / (P.TC / 2), 64, 0, s>>>(norms, P.prune ? ob.perm : nullptr, n, P.TC / 2, scale,
+                                                           tnorm, trange);
+      HIP_TRY(hipGetLastError());
+    }
+  }
 
   const double eps32 = fp32_eps(d);
+  // the row scan's f32 survivor test: cosines, or Tanimoto scores of them
+  const double eps_row = P.tani ? tani_eps(eps32) : eps32;
+  const uint32_t sel = P.fused && P.tani ? COSTAL_SELECT_TANIMOTO : COSTAL_SELECT_COSINE;
   unsigned long long* inserts = reinterpret_cast<unsigned long long*>(counters + 8);
   unsigned long long* err_max = reinterpret_cast<unsigned long long*>(counters + 64);  // ERR_SLOTS
   // f64 rescoring of the candidate lists of the queries `rows` (null: every row) with the
@@ -1592,12 +1717,12 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     const uint32_t slots = nlists * kp;
 #define RESCORE(CM)                                                                                    \
   rescore_kernel<CM><<<nq, 64, 0, s>>>(x, d, norms, n, k, (int)kp, nlists, rows, nq, cand_idx, cand_sc,       \
-                                       cand_cnt, cand_floor, all_admitted, eps_list, eps32, idx, cos, wts, fbr, \
-                                       fbl, fbc, shc, err_max)
+                                       cand_cnt, cand_floor, all_admitted, eps_list, eps_row, sel, idx, cos,  \
+                                       wts, fbr, fbl, fbc, shc, err_max)
     if (slots <= 32) {
       rescore_pair_kernel<<<(nq + 1) / 2, 64, 0, s>>>(x, d, norms, k, (int)kp, nlists, rows, nq, cand_idx, cand_sc,
-                                                      cand_cnt, cand_floor, all_admitted, eps_list, eps32, idx, cos,
-                                                      wts, fbr, fbl, fbc, shc, err_max);
+                                                      cand_cnt, cand_floor, all_admitted, eps_list, eps_row, sel, idx,
+                                                      cos, wts, fbr, fbl, fbc, shc, err_max);
     } else if (slots <= 64) RESCORE(64);
     else if (slots <= 128) RESCORE(128);
     else if (slots <= 256) RESCORE(256);
@@ -1610,16 +1735,16 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   };
   bool third = false;  // rows left short by a second-chance rescan were rescanned once more
   if (P.fused) {
-    CoarseFn fn = coarse_select(P.prec, P.J, P.KP);
+    CoarseFn fn = coarse_select(P.prec, P.J, P.KP, P.tani);
     if (!fn) return fail(COSTAL_E_UNSUPPORTED, "no coarse kernel for d=%u", d);
     const float eps_pre = P.prec == PREC_F16 ? (float)(fp16_eps(d) + eps32 + std::ldexp(1.0, -22)) : 0.0f;
     const uint8_t* xsq = need16 ? (const uint8_t*)xs16 : (const uint8_t*)(xs32w ? xs32w : xs32);
     const float* xs32q = xs32w ? xs32w : xs32;  // the f32 rows the FP16 form re-scores, in scan order
-    const double eps_list = P.prec == PREC_F16X3 ? fp16x3_eps(d) : eps32;
+    const double eps_list = P.tani ? tani_eps(fp16x3_eps(d)) : P.prec == PREC_F16X3 ? fp16x3_eps(d) : eps32;
     const float* row_th = p.row_threshold;
     unsigned long long* scanned = reinterpret_cast<unsigned long long*>(counters + 10);
     // the first pass prunes; the rescans of failed rows read the same working copies, unpruned
-    PruneArgs pr_first{}, pr_rescan{};
+    ScanArgs pr_first{}, pr_rescan{};
     if (P.prune) {
       pr_rescan.perm = ob.perm;
       pr_rescan.iperm = ob.iperm;
@@ -1630,13 +1755,15 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
       pr_first.pt = P.order.pt;
       pr_first.scanned = scanned;
     }
+    pr_first.tnorm = pr_rescan.tnorm = tnorm;
+    pr_first.trange = pr_rescan.trange = trange;
     if (learned) row_th = th_ws;  // predicted behind the non-finite check
     const uint32_t ntile = (n + P.TC - 1) / P.TC;
     // one scan pass over the queries `rows` (null: every row), then rescoring; rows that fail the
     // certificate are appended to (fbr, fbl) and counted in *fbc
     auto scan_pass = [&](CoarseFn f, uint32_t kp, const uint32_t* rows, uint32_t nq, const float* th,
                          uint32_t nsplit, uint32_t cols_per, uint32_t units, uint32_t* fbr, double* fbl,
-                         uint32_t* fbc, uint32_t* shc, const PruneArgs& pr) -> int {
+                         uint32_t* fbc, uint32_t* shc, const ScanArgs& pr) -> int {
       const dim3 grid = units ? dim3(grid_bal, 1) : dim3((nq + P.QB - 1) / P.QB, nsplit);
       f<<<grid, 64 * P.NW, 0, s>>>(xsq, xs32q, P.dp32, n, (int)kp, rows, nq, th, cols_per, units, eps_pre,
                                    cand_idx, cand_sc, cand_cnt, cand_floor, inserts, pr);
@@ -1662,7 +1789,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     HIP_TRY(hipStreamSynchronize(s));
     const uint32_t nfb = nfb_short[0], nwith = nfb_short[0] - nfb_short[1];
     if (nfb > 0) {
-      CoarseFn fn2 = coarse_select(P.prec, P.J, list_template(kp2));
+      CoarseFn fn2 = coarse_select(P.prec, P.J, list_template(kp2), P.tani);
       if (!fn2) return fail(COSTAL_E_UNSUPPORTED, "no coarse kernel for d=%u", d);
       // A row that failed only the certificate has its k-th exact score s_k among the first
       // pass's candidates (fb_lo = s_k - eps32), and its true k-th is at least that: its rescan
@@ -1699,7 +1826,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
         ns = (ntile + tiles_per - 1) / tiles_per;
         // In the locality order the rows are sorted into working order, so that a block holds
         // similar rows, and get cones and bounds of their own: the rescan prunes like the first pass.
-        PruneArgs pr = pr_rescan;
+        ScanArgs pr = pr_rescan;
         if (pruned) {
           HIP_TRY(costal::order_rescan(s, xs32, P.order, ob, rows, nr));
           pr.seq = ob.rseq;
@@ -1749,14 +1876,14 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
       }
     }
     if (third) {
-      rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb_rows, counters + 5,
-                          0, fb_lo, eps32, idx, cos, wts, grid_fb);
+      rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, sel, fb_rows, counters + 5, 0, fb_lo, eps_row,
+                          idx, cos, wts, grid_fb);
       if (!rc)
-        rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb2_rows, counters + 7,
-                            0, fb2_lo, eps32, idx, cos, wts, grid_fb);
+        rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, sel, fb2_rows, counters + 7, 0, fb2_lo,
+                            eps_row, idx, cos, wts, grid_fb);
     } else {
-      rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, COSTAL_SELECT_COSINE, fb2_rows, counters + 3,
-                          0, fb2_lo, eps32, idx, cos, wts, grid_fb);
+      rc = launch_rowscan(D, s, true, x, d, xs32, P.dp32, norms, n, k, sel, fb2_rows, counters + 3, 0, fb2_lo, eps_row,
+                          idx, cos, wts, grid_fb);
     }
     if (rc) return rc;
     HIP_TRY(hipEventRecord(D.evk[5], s));
@@ -1833,7 +1960,8 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     st->fallback_ms = span(4, 5);
     st->order_ms = P.prune ? span(6, 7) : 0.0;
     st->coarse_err_max = slots_max(cnt + 64);
-    st->coarse_eps = P.fused ? (P.prec == PREC_F16X3 ? fp16x3_eps(d) : fp32_eps(d)) : P.wide ? fp32_eps(d) : 0.0;
+    st->coarse_eps = P.fused ? (P.tani ? tani_eps(fp16x3_eps(d)) : P.prec == PREC_F16X3 ? fp16x3_eps(d) : fp32_eps(d))
+                     : P.wide ? fp32_eps(d) : 0.0;
   }
   return 0;
 }
@@ -2215,7 +2343,8 @@ int costal_knn_plan(uint64_t n, uint64_t d, const costal_knn_params* params, cos
   stats->col_splits = (P.fused || P.wide) ? P.nsplit : 1;
   stats->blocks_total = P.prune ? (uint64_t)P.order.nblk * P.order.npt
                        : (P.fused || P.wide) ? (uint64_t)((n + P.QB - 1) / P.QB) * ((n + P.TC - 1) / P.TC) : n;
-  stats->coarse_eps = P.fused ? (P.prec == PREC_F16X3 ? fp16x3_eps((uint32_t)d) : fp32_eps((uint32_t)d))
+  stats->coarse_eps = P.fused ? (P.tani ? tani_eps(fp16x3_eps((uint32_t)d))
+                                       : P.prec == PREC_F16X3 ? fp16x3_eps((uint32_t)d) : fp32_eps((uint32_t)d))
                               : P.wide ? fp32_eps((uint32_t)d) : 0.0;
   return 0;
 }

@@ -33,27 +33,32 @@ struct BatchSample {
   double* wts;
 };
 
-// The pruned scan's inputs (order.h): the working order of the rows and, per query block, the
-// candidate tiles in decreasing bound order.  All null: candidates in row order, no pruning.
-struct PruneArgs {
+// The scan's inputs beyond the rows.  The pruned scan (order.h): the working order of the rows
+// and, per query block, the candidate tiles in decreasing bound order; all null: candidates in row
+// order, no pruning.  Tanimoto selection: the rows' raw norms and their ranges per half LDS tile.
+struct ScanArgs {
   const uint32_t* perm;          // working row -> original row (null: the identity)
   const uint32_t* iperm;         // original row -> working row (for a row list)
   const uint32_t* seq;           // query block b: tiles seq[b*npt ..] in decreasing bound order
-  const float* sb;               // their bounds (upper bounds of every query-candidate cosine)
+  const float* sb;               // their bounds (upper bounds of every query-candidate score)
   uint32_t npt;                  // candidate tiles
   uint32_t pt;                   // rows of a candidate tile (a multiple of the LDS tile)
   unsigned long long* scanned;   // candidate tiles scanned (summed over blocks and splits)
+  const float* tnorm;            // Tanimoto: raw norm of every working row (f32; 0 past n)
+  const float2* trange;          // Tanimoto: (smallest, largest) raw norm of every block of TC/2
+                                 // working rows, rounded outward to f32
 };
 
 typedef void (*CoarseFn)(const uint8_t*, const float*, uint32_t, uint32_t, int, const uint32_t*,
                          uint32_t, const float*, uint32_t, uint32_t, float, uint32_t*, float*, uint32_t*,
-                         float*, unsigned long long*, PruneArgs);
+                         float*, unsigned long long*, ScanArgs);
 typedef void (*CoarseBatchFn)(const BatchSample*, const uint32_t*, uint32_t, uint32_t*, const uint8_t*,
                               const float*, uint32_t, float, uint32_t*, float*, uint32_t*, float*,
                               unsigned long long*, const unsigned long long*);
 
 // One instantiation unit per (precision form, list capacity): the kernel for J 16-byte row chunks
-// per lane half, or null when the form has none for that J.
+// per lane half, or null when the form has none for that J.  Tanimoto selection has units of its
+// own, split FP16 only (the coarse form never changes the output).
 #define COSTAL_SCAN_DECL(P, K)                       \
   CoarseFn scan_fn_p##P##_k##K(uint32_t J);          \
   CoarseBatchFn scan_batch_fn_p##P##_k##K(uint32_t J);
@@ -61,17 +66,55 @@ COSTAL_SCAN_DECL(0, 16) COSTAL_SCAN_DECL(0, 32) COSTAL_SCAN_DECL(0, 64)
 COSTAL_SCAN_DECL(1, 16) COSTAL_SCAN_DECL(1, 32) COSTAL_SCAN_DECL(1, 64)
 COSTAL_SCAN_DECL(2, 16) COSTAL_SCAN_DECL(2, 32) COSTAL_SCAN_DECL(2, 64)
 #undef COSTAL_SCAN_DECL
+CoarseFn scan_tani_fn_k16(uint32_t J);
+CoarseFn scan_tani_fn_k32(uint32_t J);
+CoarseFn scan_tani_fn_k64(uint32_t J);
 
 }  // namespace costal
 
 namespace {
 
 using costal::BatchSample;
-using costal::PruneArgs;
+using costal::ScanArgs;
 
 constexpr int PREC_F32 = COSTAL_PREC_FP32;
 constexpr int PREC_F16 = COSTAL_PREC_FP16;
 constexpr uint32_t NONE = 0xffffffffu;
+// selection metric of a scan instance
+constexpr int SEL_COS = 0, SEL_TANI = 1;
+// Tanimoto cut shave: the computed cosine cut c_tau * s_min is at most ~10 f32 roundings above the
+// exact one; scaling it by 1 - 2^-18 puts it below.
+constexpr float TANI_SHAVE = 1.0f - 0x1p-18f;
+
+// Tanimoto selection on normalized rows.  With a, b the raw norms of the query and the candidate,
+// rho = b / a and s = rho + 1/rho >= 2, the Tanimoto score of cosine c is T = c a b / (a^2 + b^2 -
+// c a b) = c / (s - c) (weight.rs:28-37), increasing in c with slope s / (s - c)^2 <= 2 for c <= 1:
+// a list score off by e in the cosine is off by at most 2e / (1 - e)^2 in T.  f32 evaluation from
+// the list score v: rho from b and 1/a (v_rcp_f32, 1 ulp), s, then v / (s - v) by a reciprocal:
+// at most ~16 roundings of a value <= 1 + 3e.  A zero query (1/a taken as 0) or a zero candidate
+// gives s = inf and T = 0, as the reference scores zero rows.
+__device__ __forceinline__ float tani32(float v, float b, float ia) {
+  const float rho = b * ia;
+  const float s = rho + __builtin_amdgcn_rcpf(rho);
+  return v * __builtin_amdgcn_rcpf(s - v);
+}
+
+// The smallest s over candidates with raw norms in [r.x, r.y] for a query with 1/a = ia: s(rho) is
+// smallest at rho = 1 and grows away from it (a zero query: inf).
+__device__ __forceinline__ float tani_smin(float2 r, float ia) {
+  const float lo = r.x * ia, hi = r.y * ia;
+  if (hi < 1.0f) return hi + __builtin_amdgcn_rcpf(hi);
+  if (lo > 1.0f) return lo + __builtin_amdgcn_rcpf(lo);
+  return 2.0f;
+}
+
+// The cosine cut of a Tanimoto threshold tau > 0: T > tau needs c > tau / (1 + tau) * s (T >= -1/3,
+// so 1 + tau > 0), and s >= smin; shaved below the exact value.  No cut for tau <= 0 (T of a
+// negative cosine rises with s).
+__device__ __forceinline__ float tani_cut(float tau, float smin) {
+  if (!(tau > 0.0f)) return -INFINITY;
+  return tau * __builtin_amdgcn_rcpf(1.0f + tau) * smin * TANI_SHAVE;
+}
 
 // f32 inner product of two zero-padded normalized rows, sequential fma: |err| <= gamma_d.
 __device__ __forceinline__ float dot32(const float* __restrict__ a, const float* __restrict__ b,
@@ -203,6 +246,8 @@ template <int J>
 struct TileShape {
   static constexpr int TC = J <= 2 ? 512 : J <= 4 ? 256 : 128;  // candidates per tile: 32 KB tiles
   static constexpr int TILEB = TC * 32 * J;
+  // Tanimoto: a tile's raw norms (TC floats) and its halves' norm ranges (two float2), in LDS
+  static constexpr int TANIF = TC + 4;
   static constexpr int SUBT = TC / 32;                         // 32-candidate sub-tiles per tile
   static constexpr int GS = (SUBT >= 4 && J <= 4) ? 2 : 1;     // sub-tiles per scan step
   static constexpr int NSTEP = SUBT / GS;                      // scan steps per tile
@@ -220,7 +265,7 @@ struct TileShape {
 // parity is the step's parity within its tile).  The next tile is stored to the other LDS buffer,
 // and the workgroup meets at the barrier, at step NSTEP-3 of each tile, before the first read of
 // that tile.
-template <int PREC, int J, int KPR, int NW>
+template <int PREC, int J, int KPR, int NW, int SEL>
 __device__ __forceinline__ void scan_block(
     uint8_t* __restrict__ tile0, uint8_t* __restrict__ tile1,
     const uint8_t* __restrict__ xsq,   // n_pad rows of 32*J bytes (the working copy of this form)
@@ -233,7 +278,7 @@ __device__ __forceinline__ void scan_block(
     float eps_pre,                     // FP16 form: bound on |FP16 score - f32 score|
     uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc, uint32_t* __restrict__ cand_cnt,
     float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts,
-    const PruneArgs& pr, float* __restrict__ s_thr) {
+    const ScanArgs& pr, float* __restrict__ s_thr, float* __restrict__ s_nrm) {
   constexpr int QB = 32 * NW;
   constexpr int TC = TileShape<J>::TC;
   constexpr int ROWB = 32 * J;
@@ -245,6 +290,9 @@ __device__ __forceinline__ void scan_block(
   constexpr int SUBT = TileShape<J>::SUBT;
   static_assert(NSTEP >= 2 && NSTEP % 2 == 0, "pipeline steps per tile");
   static_assert(GS * NSTEP * 32 == TC, "tile shape");
+  static_assert((SUBT / 2) % GS == 0, "the sub-tiles of a scan step lie in one half of the tile");
+  static_assert(SEL == SEL_COS || PREC != PREC_F16, "Tanimoto scans have no re-scoring form");
+  constexpr bool TANI = SEL == SEL_TANI;
 
   const int tid = tid_x();
   const int w = tid >> 6, l = tid & 63, h = l >> 5, l32 = l & 31;
@@ -287,6 +335,17 @@ __device__ __forceinline__ void scan_block(
   float th = ls[0];
   float thp = -INFINITY;  // the partner list's threshold, refreshed once per tile
   uint32_t my_inserts = 0;
+  // Tanimoto: list scores are f32 Tanimoto scores (tani32); a candidate whose cosine is at most the
+  // cut of the lane's threshold for its half of the tile (tani_cut with the half's smallest s) can
+  // not enter, so the max3 test stays on the cosines.  1/a of the query (0 for a zero row), the
+  // halves' smallest s and their cuts.
+  float qia = 0.0f, smin0 = 2.0f, smin1 = 2.0f, cut0 = -INFINITY, cut1 = -INFINITY;
+  if constexpr (TANI) {
+    if (qvalid) {
+      const float qa = pr.tnorm[qg];
+      qia = qa > 0.0f ? __builtin_amdgcn_rcpf(qa) : 0.0f;
+    }
+  }
 
   // Candidate tiles.  Without pruning: the range [c_begin, c_end) in row order, one LDS tile after
   // another, or (a row list in the working order) every nsplit-th LDS tile of all rows, so that a
@@ -345,6 +404,17 @@ __device__ __forceinline__ void scan_block(
   constexpr bool SPLIT_HALVES = TC == 512;
   static_assert(!SPLIT_HALVES || (CHUNKS / 2) % NT == 0, "halves of a tile in whole chunks per thread");
   uint4 ra0 = {}, ra1 = {}, ra2 = {}, ra3 = {}, ra4 = {}, ra5 = {}, ra6 = {}, ra7 = {};
+  // Tanimoto staging: NPER raw norms of tile rows tid*NPER.., and (threads 0, 1) a half's range.
+  // The norms live in three LDS slots, tile t in slot t % 3: unlike the rows, which every wave has
+  // read into registers before the barrier of their tile, the norms are read by the insert path
+  // until the end of the tile, after that barrier, so the slot of tile t is refilled (with tile
+  // t + 3, at tile t + 2) only once every wave has met at the barrier of tile t + 1, after its last
+  // read of tile t.
+  constexpr int NPER = TC >= NT ? TC / NT : 1;
+  constexpr int TANIF = TileShape<J>::TANIF;
+  static_assert(NPER <= 2, "norm staging");
+  float2 rn = {}, rr = {};
+  int ns_cur = 0;  // the norm slot of tile t
   // Thread e fills LDS slot e (consecutive lanes, consecutive 16-byte slots: conflict free); slot
   // e = ((sub*J + j)*2 + half)*32 + lane32 holds chunk 2j + half of tile row sub*32 + lane32.
 #define LOAD_ONE(R, C)                                                                            \
@@ -364,15 +434,37 @@ __device__ __forceinline__ void scan_block(
     const uint8_t* srcb_ = SPLIT_HALVES ? xsq + (size_t)((HB).y < n ? (HB).y : (HB).x) * ROWB : srca_; \
     LOAD_ONE(R, 0) LOAD_ONE(R, 1) LOAD_ONE(R, 2) LOAD_ONE(R, 3)                                   \
     LOAD_ONE(R, 4) LOAD_ONE(R, 5) LOAD_ONE(R, 6) LOAD_ONE(R, 7)                                   \
+    if constexpr (TANI) LOAD_TANI(HB)                                                             \
+  }
+  // the second half's first row (a contiguous tile: the first half's plus TC/2; a second half past
+  // the split's last position: the first half's rows again, never admitted)
+#define HALF1(HB) (SPLIT_HALVES ? ((HB).y < n ? (HB).y : (HB).x) : (HB).x + TC / 2)
+#define LOAD_TANI(HB)                                                                             \
+  {                                                                                               \
+    const int r0_ = tid * NPER;                                                                   \
+    if (r0_ < TC) {                                                                               \
+      const uint32_t row_ = r0_ < TC / 2 ? (HB).x + r0_ : HALF1(HB) + (r0_ - TC / 2);              \
+      if constexpr (NPER == 2) rn = *reinterpret_cast<const float2*>(pr.tnorm + row_);             \
+      else rn.x = pr.tnorm[row_];                                                                 \
+    }                                                                                             \
+    if (tid < 2) rr = pr.trange[(tid ? HALF1(HB) : (HB).x) / (TC / 2)];                            \
   }
 #define STORE_ONE(R, C, BUF)                                                                      \
   if constexpr (C < CPT) {                                                                        \
     const int e_ = tid + C * NT;                                                                  \
     if (CHUNKS % NT == 0 || e_ < CHUNKS) *reinterpret_cast<uint4*>(&BUF[e_ * 16]) = R##C;         \
   }
-#define STORE_REGS(R, BUF) \
+#define STORE_REGS(R, BUF, NSLOT) \
   { STORE_ONE(R, 0, BUF) STORE_ONE(R, 1, BUF) STORE_ONE(R, 2, BUF) STORE_ONE(R, 3, BUF)         \
-    STORE_ONE(R, 4, BUF) STORE_ONE(R, 5, BUF) STORE_ONE(R, 6, BUF) STORE_ONE(R, 7, BUF) }
+    STORE_ONE(R, 4, BUF) STORE_ONE(R, 5, BUF) STORE_ONE(R, 6, BUF) STORE_ONE(R, 7, BUF)         \
+    if constexpr (TANI) {                                                                         \
+      float* nb_ = s_nrm + (NSLOT) * TANIF;                                                       \
+      if (tid * NPER < TC) {                                                                      \
+        if constexpr (NPER == 2) *reinterpret_cast<float2*>(nb_ + tid * 2) = rn;                  \
+        else nb_[tid] = rn.x;                                                                     \
+      }                                                                                           \
+      if (tid < 2) *reinterpret_cast<float2*>(nb_ + TC + 2 * tid) = rr;                           \
+    } }
   // LDS writes of this wave done, then everyone meets; no wait on the global loads in flight
 #define TILE_BARRIER()                                                                            \
   {                                                                                               \
@@ -427,7 +519,8 @@ __device__ __forceinline__ void scan_block(
     float m = mu[0];
 #pragma unroll
     for (int u = 1; u < GS; ++u) m = fmaxf(m, mu[u]);
-    float the = (PREC == PREC_F16) ? fmaxf(th, thp) - eps_pre : fmaxf(th, thp);
+    const bool half1 = p * GS >= SUBT / 2;  // the step's half of the tile
+    float the = TANI ? (half1 ? cut1 : cut0) : (PREC == PREC_F16) ? fmaxf(th, thp) - eps_pre : fmaxf(th, thp);
     if (__builtin_amdgcn_ballot_w64(m > the) != 0) {
 #pragma unroll
       for (int u = 0; u < GS; ++u) {
@@ -453,9 +546,20 @@ __device__ __forceinline__ void scan_block(
                 v = dot32(xq32, xs32 + (size_t)cand * dp32, dp32);
                 if (!(v > th)) continue;
               }
+              if constexpr (TANI) {
+                const float b = s_nrm[ns_cur * TANIF + sub * 32 + (int)lane_off + 8 * (i >> 2) + (i & 3)];
+                v = tani32(v, b, qia);
+                if (!(v > fmaxf(th, thp))) continue;
+              }
               list_push<KPR>(ls, li, v, cand);
               th = ls[0];
-              the = (PREC == PREC_F16) ? fmaxf(th, thp) - eps_pre : fmaxf(th, thp);
+              if constexpr (TANI) {
+                cut0 = tani_cut(fmaxf(th, thp), smin0);
+                cut1 = tani_cut(fmaxf(th, thp), smin1);
+                the = half1 ? cut1 : cut0;
+              } else {
+                the = (PREC == PREC_F16) ? fmaxf(th, thp) - eps_pre : fmaxf(th, thp);
+              }
               ++my_inserts;
             }
           }
@@ -467,9 +571,18 @@ __device__ __forceinline__ void scan_block(
 #define RUN_TILE(CUR, OTH)                                                                        \
   _Pragma("unroll") for (int p = 0; p < NSTEP; ++p) {                                             \
     MFMA_STEP((p + 1) & 1)                                                                        \
-    if (p == 0) thp = partner32(th);                                                              \
+    if (p == 0) {                                                                                 \
+      thp = partner32(th);                                                                        \
+      if constexpr (TANI) {                                                                       \
+        const f32x4 rg_ = *reinterpret_cast<const f32x4*>(s_nrm + ns_cur * TANIF + TC);           \
+        smin0 = tani_smin(make_float2(rg_[0], rg_[1]), qia);                                      \
+        smin1 = tani_smin(make_float2(rg_[2], rg_[3]), qia);                                      \
+        cut0 = tani_cut(fmaxf(th, thp), smin0);                                                   \
+        cut1 = tani_cut(fmaxf(th, thp), smin1);                                                   \
+      }                                                                                           \
+    }                                                                                             \
     if (p == NSTEP - RD) {                                                                        \
-      STORE_REGS(ra, OTH);                                                                        \
+      STORE_REGS(ra, OTH, ns_cur == 2 ? 0 : ns_cur + 1);                                          \
       if (seq_b) wave_threshold();                                                                \
       TILE_BARRIER();                                                                             \
       if (seq_b) prune_next();                                                                    \
@@ -489,7 +602,7 @@ __device__ __forceinline__ void scan_block(
     uint8_t* oth = tile1;
     b_cur = tile_base(0);
     LOAD_REGS(ra, b_cur);
-    STORE_REGS(ra, cur);
+    STORE_REGS(ra, cur, 0);
     TILE_BARRIER();
     b_nx = tile_base(ntile > 1 ? 1 : 0);
     LOAD_REGS(ra, b_nx);
@@ -506,6 +619,7 @@ __device__ __forceinline__ void scan_block(
       oth = tmp;
       b_cur = b_nx;
       b_nx = b_nn;
+      ns_cur = ns_cur == 2 ? 0 : ns_cur + 1;
     }
   }
   if (seq_b && pr.scanned && tid == 0) {
@@ -520,6 +634,8 @@ __device__ __forceinline__ void scan_block(
 #undef STORE_REGS
 #undef LOAD_ONE
 #undef STORE_ONE
+#undef LOAD_TANI
+#undef HALF1
 
   if (inserts) {
     // one atomic per wave
@@ -558,21 +674,24 @@ __device__ __forceinline__ void scan_block(
 // (the compiler otherwise lets such instances grow past 256 registers by a few, which halves the
 // occupancy: the split-FP16 headline instance measured 12.3 ms unbounded against 6.2 ms bounded);
 // left to the compiler where two would spill: J of 3 or 4, lists of 64, and lists of 32 with J of
-// 2, 7 or 8.
-template <int J, int KPR>
+// 2, 7 or 8.  The Tanimoto scan holds a few more registers (1/a, the halves' cuts, norm staging):
+// two waves only for J = 6 with lists of 16 (the others spill at two).
+template <int J, int KPR, int SEL = SEL_COS>
 constexpr int scan_waves() {
+  if constexpr (SEL == SEL_TANI) return (J == 6 && KPR == 16) ? 2 : 1;
   return (KPR >= 64 || J == 3 || J == 4 || (KPR >= 32 && (J == 2 || J >= 7))) ? 1 : 2;
 }
 
-template <int PREC, int J, int KPR, int NW>
-__global__ __launch_bounds__(64 * NW, (scan_waves<J, KPR>())) void coarse_kernel(
+template <int PREC, int J, int KPR, int NW, int SEL = SEL_COS>
+__global__ __launch_bounds__(64 * NW, (scan_waves<J, KPR, SEL>())) void coarse_kernel(
     const uint8_t* __restrict__ xsq, const float* __restrict__ xs32, uint32_t dp32, uint32_t n, int kp,
     const uint32_t* __restrict__ rowlist, uint32_t nq, const float* __restrict__ tinit, uint32_t cols_per_split,
     uint32_t units_per_wg, float eps_pre, uint32_t* __restrict__ cand_idx, float* __restrict__ cand_sc,
     uint32_t* __restrict__ cand_cnt, float* __restrict__ cand_floor, unsigned long long* __restrict__ inserts,
-    PruneArgs pr) {
+    ScanArgs pr) {
   __shared__ __attribute__((aligned(16))) uint8_t tile0[TileShape<J>::TILEB];
   __shared__ __attribute__((aligned(16))) uint8_t tile1[TileShape<J>::TILEB];
+  __shared__ __attribute__((aligned(16))) float s_nrm[SEL == SEL_TANI ? 3 * TileShape<J>::TANIF : 4];
   __shared__ float s_thr[2 * NW];
   constexpr uint32_t TC = TileShape<J>::TC;
   constexpr uint32_t QB = 32 * NW;
@@ -604,8 +723,8 @@ __global__ __launch_bounds__(64 * NW, (scan_waves<J, KPR>())) void coarse_kernel
       len = 1;
     }
     if (!first) __syncthreads();  // every wave is done with the previous piece's tiles
-    scan_block<PREC, J, KPR, NW>(tile0, tile1, xsq, xs32, dp32, n, kp, rowlist, nq, qb, tinit, split, nspl, c0, c1,
-                                 eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor, inserts, pr, s_thr);
+    scan_block<PREC, J, KPR, NW, SEL>(tile0, tile1, xsq, xs32, dp32, n, kp, rowlist, nq, qb, tinit, split, nspl, c0, c1,
+                                 eps_pre, cand_idx, cand_sc, cand_cnt, cand_floor, inserts, pr, s_thr, s_nrm);
     if (whole) {
       const uint32_t qq = qb * QB + (tid_x() >> 6) * 32 + (tid_x() & 31);
       if (qq < nq) {
@@ -643,10 +762,10 @@ __global__ __launch_bounds__(64 * NW) void coarse_batch_kernel(
     const size_t rowb = (size_t)S.row_pad;
     const uint32_t c0 = sp * S.cols_per_split;
     const uint32_t c1 = c0 + S.cols_per_split < S.n ? c0 + S.cols_per_split : S.n;
-    scan_block<PREC, J, KPR, NW>(tile0, tile1, xsq + rowb * 32 * J, xs32 + rowb * dp32, dp32, S.n, S.kp, nullptr,
+    scan_block<PREC, J, KPR, NW, SEL_COS>(tile0, tile1, xsq + rowb * 32 * J, xs32 + rowb * dp32, dp32, S.n, S.kp, nullptr,
                                  S.n, qb, nullptr, sp, S.nsplit, c0, c1, eps_pre,
                                  cand_idx + S.entry_base, cand_sc + S.entry_base, cand_cnt + S.slot_base,
-                                 cand_floor + S.slot_base, inserts, PruneArgs{}, s_thr);
+                                 cand_floor + S.slot_base, inserts, ScanArgs{}, s_thr, nullptr);
     __syncthreads();
   }
 }

@@ -1,6 +1,7 @@
 // scan_inst.hip -- instantiates the fused scan kernels of one precision form and list capacity.
 // Built once per (SCAN_PREC, SCAN_KP) pair (see the Makefile), so the nine kernel families compile
 // in parallel; the host code picks one through the scan_fn_p<P>_k<K> selectors declared in scan.h.
+// With SCAN_TANI, the Tanimoto-selection scan of split FP16 rows instead (scan_tani_fn_k<K>).
 #include "scan.h"
 
 #include <type_traits>
@@ -14,10 +15,17 @@
 
 namespace {
 
+#ifdef SCAN_TANI
+static_assert(SCAN_PREC == PREC_F16X3, "Tanimoto scans run on split FP16 rows");
+constexpr int SCAN_SEL = SEL_TANI;
+#else
+constexpr int SCAN_SEL = SEL_COS;
+#endif
+
 template <int J>
 costal::CoarseFn one() {
   if constexpr (SCAN_PREC == PREC_F16X3 && J % 2) return nullptr;  // split rows hold hi and lo halves
-  else return coarse_kernel<SCAN_PREC, J, SCAN_KP, 4>;
+  else return coarse_kernel<SCAN_PREC, J, SCAN_KP, 4, SCAN_SEL>;
 }
 
 template <int J>
@@ -48,6 +56,11 @@ F by_j(uint32_t J, G pick) {
 
 namespace costal {
 
+#ifdef SCAN_TANI
+CoarseFn COSTAL_NAME(scan_tani_fn, _k, SCAN_KP, )(uint32_t J) {
+  return by_j<CoarseFn>(J, [](auto j) { return one<decltype(j)::value>(); });
+}
+#else
 CoarseFn COSTAL_NAME(scan_fn_p, SCAN_PREC, _k, SCAN_KP)(uint32_t J) {
   return by_j<CoarseFn>(J, [](auto j) { return one<decltype(j)::value>(); });
 }
@@ -55,5 +68,6 @@ CoarseFn COSTAL_NAME(scan_fn_p, SCAN_PREC, _k, SCAN_KP)(uint32_t J) {
 CoarseBatchFn COSTAL_NAME(scan_batch_fn_p, SCAN_PREC, _k, SCAN_KP)(uint32_t J) {
   return by_j<CoarseBatchFn>(J, [](auto j) { return one_batch<decltype(j)::value>(); });
 }
+#endif
 
 }  // namespace costal

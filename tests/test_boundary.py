@@ -272,7 +272,12 @@ def test_plan_paths_and_bounds():
     assert st32.coarse_eps == (15 + 4) * 2.0 ** -23  # the reference's bound, knn.rs:185-187
     assert 0 < eps16x3 < 2 * st32.coarse_eps
     assert _plan(3605, 2000, 10, precision="fp32").path == 2  # wide rows
-    assert _plan(5000, 15, 10, select="tanimoto").path == 1  # exact row scan
+    tani = _plan(5000, 15, 10, select="tanimoto")  # the fused scan with Tanimoto lists
+    assert tani.path == 0
+    # the list error of a Tanimoto score: twice the split-FP16 cosine bound and then some
+    assert 2 * eps16x3 < tani.coarse_eps < 2 * eps16x3 * 1.01 + 2.0 ** -19
+    assert _plan(5000, 15, 100, select="tanimoto").path == 1  # lists beyond 64: exact row scan
+    assert _plan(5000, 80, 10, select="tanimoto").path == 1  # rows beyond 64 features
 
 
 def test_plan_rejects_what_the_reference_rejects():

@@ -1419,8 +1419,8 @@ Plan make_plan(const Device& D, uint64_t n, uint64_t d, uint64_t k, const costal
   // Pruned first pass: the splits take turns over each block's tiles in bound order (at most one
   // split per candidate tile of PRUNE_TILE rows).
   P.prune = false;
-  if (P.fused && !p.no_prune && !P.tani) {
-    P.order = costal::order_shape((uint32_t)n, P.dp32, P.QB, p.prune_clusters);
+  if (P.fused && !p.no_prune) {
+    P.order = costal::order_shape((uint32_t)n, P.dp32, P.QB, p.prune_clusters, P.tani);
     if (costal::order_supported(P.order)) {
       P.prune = true;
       P.nsplit = std::min<uint32_t>(P.nsplit, P.order.npt);
@@ -1674,33 +1674,32 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     if (st) st->nonfinite_row = bad;
     return fail(COSTAL_E_NON_FINITE, "row %u contains NaN or inf", bad);
   }
+  // Tanimoto: norms scaled by a power of two that puts the largest in [1, 2); a spread of norms
+  // beyond 2^100 (the smallest would lose bits in f32) takes the exact row scan instead
+  double tani_scale = 1.0, norm_lo = 0.0, norm_hi = 0.0;
+  if (P.fused && P.tani) {
+    const uint64_t hib = (uint64_t)head[14] | ((uint64_t)head[15] << 32);
+    const uint64_t lob = (uint64_t)head[16] | ((uint64_t)head[17] << 32);
+    if (hib) {
+      const uint64_t lbits = ~lob;
+      memcpy(&norm_hi, &hib, sizeof norm_hi);
+      memcpy(&norm_lo, &lbits, sizeof norm_lo);
+      tani_scale = std::ldexp(1.0, -std::ilogb(norm_hi));
+      if (norm_lo * tani_scale < std::ldexp(1.0, -100)) P.fused = P.prune = false;
+    }
+  }
   if (P.prune) {
     // the locality order, the working copies in it, and every block's tiles in bound order
     HIP_TRY(hipEventRecord(D.evk[6], s));
-    HIP_TRY(costal::order_rows(s, xs32, P.order, ob));
+    HIP_TRY(costal::order_rows(s, xs32, P.order, ob, norms, norm_lo, norm_hi));
     HIP_TRY(costal::order_permute(s, xs32, P.order, ob, P.n_pad, P.dp16, split16, xs32w, xs16));
-    HIP_TRY(costal::order_bounds(s, xs32, P.order, ob));
+    HIP_TRY(costal::order_bounds(s, xs32, P.order, ob, norms));
     HIP_TRY(hipEventRecord(D.evk[7], s));
   }
   if (P.fused && P.tani) {
-    // Norms scaled by a power of two that puts the largest in [1, 2); a spread of norms beyond
-    // 2^100 (the smallest would lose bits in f32) takes the exact row scan instead.
-    const uint64_t hib = (uint64_t)head[14] | ((uint64_t)head[15] << 32);
-    const uint64_t lob = (uint64_t)head[16] | ((uint64_t)head[17] << 32);
-    double scale = 1.0;
-    if (hib) {
-      double hi, lo;
-      memcpy(&hi, &hib, sizeof hi);
-      const uint64_t lbits = ~lob;
-      memcpy(&lo, &lbits, sizeof lo);
-      scale = std::ldexp(1.0, -std::ilogb(hi));
-      if (lo * scale < std::ldexp(1.0, -100)) P.fused = false;
-    }
-    if (P.fused) {
-      tani_norms_kernel<<<P.n_pad / (P.TC / 2), 64, 0, s>>>(norms, P.prune ? ob.perm : nullptr, n, P.TC / 2, scale,
-                                                           tnorm, trange);
-      HIP_TRY(hipGetLastError());
-    }
+    tani_norms_kernel<<<P.n_pad / (P.TC / 2), 64, 0, s>>>(norms, P.prune ? ob.perm : nullptr, n, P.TC / 2, tani_scale,
+                                                         tnorm, trange);
+    HIP_TRY(hipGetLastError());
   }
 
   const double eps32 = fp32_eps(d);
@@ -1828,7 +1827,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
         // similar rows, and get cones and bounds of their own: the rescan prunes like the first pass.
         ScanArgs pr = pr_rescan;
         if (pruned) {
-          HIP_TRY(costal::order_rescan(s, xs32, P.order, ob, rows, nr));
+          HIP_TRY(costal::order_rescan(s, xs32, P.order, ob, rows, nr, norms));
           pr.seq = ob.rseq;
           pr.sb = ob.rsb;
           pr.npt = P.order.npt;

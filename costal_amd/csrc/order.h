@@ -12,6 +12,11 @@
 // tiles in decreasing bound order and stops at the first bound below every query's admission
 // threshold: the candidates left out score below every list floor, so the certificate of the
 // rescoring (knn.rs:704-714) holds unchanged and the result is the brute force bit for bit.
+//
+// Tanimoto selection (T = c / (s - c), s = rho + 1/rho for the norm ratio rho) orders the rows of
+// every direction group by norm as well, and bounds a pair of groups by c_max / (s_min - c_max),
+// with c_max the cone bound and s_min the smallest s over the two groups' norm ranges (0 when
+// c_max <= 0): the reference's norm window (knn.rs:246-282) and the cosine cones in one bound.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,6 +35,10 @@ constexpr uint32_t RESCAN_PRUNE_ROWS = 8192;
 // Bytes of LDS the k-means kernels hold per center feature (f32 center + i64 sum).
 constexpr uint32_t KM_LDS_BYTES = 12;
 constexpr uint32_t KM_LDS_MAX = 60 * 1024;
+// Tanimoto: candidate tiles of rows per direction sub-center (whose rows are then sorted by norm),
+// and the bits of the norm key (log-spaced over the rows' norm range)
+constexpr uint32_t TANI_SUB_TILES = 4;
+constexpr uint32_t TANI_NORM_BITS = 8;
 
 struct OrderBufs {
   uint32_t* perm;    // n: working row -> original row
@@ -52,11 +61,14 @@ struct OrderBufs {
   double* qang;      // query blocks x 3: cone angle, its cosine, its sine
   double* tcen;      // candidate tiles x dp
   double* tang;      // candidate tiles x 3
+  double* qnr;       // Tanimoto: query blocks x 2: smallest and largest raw norm
+  double* tnr;       // Tanimoto: candidate tiles x 2
   uint32_t* seq;     // query blocks x tiles: tile ids in decreasing bound order
   float* sb;         // query blocks x tiles: the bounds in that order
   // the same for the blocks of a rescan's row list (RESCAN_PRUNE_ROWS / qb blocks at most)
   double* rcen;
   double* rang;
+  double* rnr;
   uint32_t* rseq;
   float* rsb;
 };
@@ -70,14 +82,17 @@ struct OrderShape {
   uint32_t sub_cap;    // sub-centers of one first-level center at most
   uint32_t sub_sample; // rows of a first-level center's sub-center fit at most
   uint32_t sub_iters;
-  uint32_t key_bits;   // bits of a sub-center id
+  uint32_t key_bits;   // bits of a sort key: sub-center id, then (Tanimoto) the norm key
+  uint32_t norm_bits;  // bits of the norm key (0: cosine selection)
+  uint32_t tani;       // Tanimoto bounds
   uint32_t qb;         // rows of a query block
   uint32_t pt;         // rows of a candidate tile (PRUNE_TILE or twice that)
   uint32_t nblk, npt;  // query blocks, candidate tiles
 };
 
-// The default shape for n rows of dp features with query blocks of qb rows.
-OrderShape order_shape(uint32_t n, uint32_t dp, uint32_t qb, uint32_t clusters);
+// The default shape for n rows of dp features with query blocks of qb rows (tani: for Tanimoto
+// selection).
+OrderShape order_shape(uint32_t n, uint32_t dp, uint32_t qb, uint32_t clusters, bool tani = false);
 // Whether the bound matrix fits the sort (and the scan has something to gain).
 bool order_supported(const OrderShape& o);
 // Device bytes of the buffers (each carved on 256-byte boundaries).
@@ -89,18 +104,22 @@ OrderBufs order_carve(void* base, const OrderShape& o);
 // Two levels: rows sorted by their nearest first-level center, then, within each first-level
 // center's rows, by their nearest of its sub-centers (about one per candidate tile of rows,
 // spherical k-means on a sample of those rows), so that consecutive rows, and so every query block
-// and candidate tile, are tight groups.
-hipError_t order_rows(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b);
+// and candidate tile, are tight groups.  Tanimoto: about one sub-center per TANI_SUB_TILES tiles,
+// and within each, rows by norm (norms: the f64 row norms, in [nlo, nhi] where positive).
+hipError_t order_rows(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b,
+                      const double* norms = nullptr, double nlo = 0.0, double nhi = 0.0);
 // Working copies in the working order, rows [n, n_pad) zero: xs32w (f32, dp32) and/or xs16w
 // (FP16, dp16 per row, or hi then lo rows of dp16 when split16).  Either may be null.
 hipError_t order_permute(hipStream_t s, const float* xs32, const OrderShape& o, const OrderBufs& b, uint32_t n_pad,
                          uint32_t dp16, int split16, float* xs32w, _Float16* xs16w);
-// Cones of the query blocks and candidate tiles, the bound matrix and its per-block order.
-hipError_t order_bounds(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b);
+// Cones (and for Tanimoto norm ranges, from norms) of the query blocks and candidate tiles, the
+// bound matrix and its per-block order.
+hipError_t order_bounds(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b,
+                        const double* norms = nullptr);
 // A rescan's row list (original rows, nr <= RESCAN_PRUNE_ROWS): sorted in place into working
 // order (so that its blocks of qb rows are groups of similar rows), their cones, and every block's
 // tiles in bound order in rseq / rsb.  Needs order_bounds first (the tile cones).
 hipError_t order_rescan(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b, uint32_t* rows,
-                        uint32_t nr);
+                        uint32_t nr, const double* norms = nullptr);
 
 }  // namespace costal

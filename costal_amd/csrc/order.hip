@@ -264,22 +264,35 @@ __device__ __forceinline__ double wave_min(double x) {
 // f32; angle = the largest angle between that f32 center and any row, in f64 with both vectors'
 // own norms, plus CONE_MARGIN; pi when the sum is zero (the cone covers everything).  cen holds the
 // center as an f64 unit vector (the f32 center over its f64 norm: the same direction), ang[3g ..]
-// = (angle, its cosine, its sine).
+// = (angle, its cosine, its sine).  With norms (Tanimoto), nr[2g ..] = the smallest and largest
+// f64 row norm of the group.
 template <int DPM>
 __global__ __launch_bounds__(64) void cone_kernel(const float* __restrict__ xs32, uint32_t dp,
                                                   const uint32_t* __restrict__ rows, uint32_t n, uint32_t G,
-                                                  double* __restrict__ cen, double* __restrict__ ang) {
+                                                  double* __restrict__ cen, double* __restrict__ ang,
+                                                  const double* __restrict__ norms, double* __restrict__ nr) {
   __shared__ float s_c[DPM];
   const uint32_t g = bid_x(), l = tid_x();
   const uint32_t w0 = g * G, w1 = std::min(n, w0 + G);
   double acc[DPM];
 #pragma unroll
   for (int f = 0; f < DPM; ++f) acc[f] = 0.0;
+  double nmin = INFINITY, nmax = 0.0;
   for (uint32_t w = w0 + l; w < w1; w += 64) {
     float v[DPM];
     load_row<DPM>(xs32 + (size_t)rows[w] * dp, dp, v);
 #pragma unroll
     for (int f = 0; f < DPM; ++f) acc[f] += (double)v[f];
+    if (norms) {
+      const double nv = norms[rows[w]];
+      nmin = fmin(nmin, nv);
+      nmax = fmax(nmax, nv);
+    }
+  }
+  if (norms) {
+    nmin = wave_min(nmin);
+    nmax = -wave_min(-nmax);
+    if (l == 0) nr[2 * g] = nmin, nr[2 * g + 1] = nmax;
   }
   double nn = 0.0;
 #pragma unroll
@@ -413,13 +426,29 @@ __device__ __forceinline__ float key_float(uint32_t k) {
   return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
 
+// The largest Tanimoto score of a pair with cosine at most c and raw norms a in [amin, amax], b in
+// [bmin, bmax]: T = c / (s - c) rises with c and, for c > 0, falls with s = rho + 1/rho (rho = b/a),
+// which is smallest at the ratio nearest 1; a cosine <= 0 gives T <= 0.
+__device__ __forceinline__ double tani_bound(double c, double amin, double amax, double bmin, double bmax) {
+  if (!(c > 0.0) || amax == 0.0 || bmax == 0.0) return 0.0;
+  const double lo = bmin / amax;
+  const double hi = amin > 0.0 ? bmax / amin : INFINITY;
+  double smin = 2.0;
+  if (hi < 1.0) smin = hi + 1.0 / hi;
+  else if (lo > 1.0) smin = lo + 1.0 / lo;
+  const double cc = fmin(c, 1.0);
+  return cc / (smin - cc);
+}
+
 // Bound of every candidate tile for query block bid_x(), sorted in decreasing order (ties: lower
 // tile first): seq[b*npt + r] = tile of rank r, sb[b*npt + r] = its bound, an f32 rounded up.
-// 1024 threads; a value is (bound key << 32) | ~tile, so that one comparison orders both.
+// 1024 threads; a value is (bound key << 32) | ~tile, so that one comparison orders both.  With
+// norm ranges (qnr, tnr), the bound is on Tanimoto scores (tani_bound of the cosine bound).
 __global__ __launch_bounds__(1024) void bound_sort_kernel(const double* __restrict__ qcen, const double* __restrict__ qang,
                                                           const double* __restrict__ tcen, const double* __restrict__ tang,
                                                           uint32_t dp, uint32_t npt, uint32_t p2, uint32_t* __restrict__ seq,
-                                                          float* __restrict__ sb) {
+                                                          float* __restrict__ sb, const double* __restrict__ qnr,
+                                                          const double* __restrict__ tnr) {
   constexpr int EMAX = PRUNE_MAX_TILES / 1024;
   __shared__ uint64_t s_buf[PRUNE_MAX_TILES];
   __shared__ double s_q[64];
@@ -447,6 +476,7 @@ __global__ __launch_bounds__(1024) void bound_sort_kernel(const double* __restri
         const double cth = fmax(-1.0, fmin(1.0, dot));
         if (cth < ca) bound = cth * ca + sqrt(fmax(0.0, 1.0 - cth * cth)) * sa;
       }
+      if (qnr) bound = tani_bound(bound, qnr[2 * b], qnr[2 * b + 1], tnr[2 * t], tnr[2 * t + 1]);
       bound += BOUND_MARGIN;
       float key = (float)bound;
       if ((double)key < bound) key = nextafterf(key, INFINITY);
@@ -573,14 +603,18 @@ __global__ __launch_bounds__(256) void sub_fit_kernel(const float* __restrict__ 
 }
 
 // key2[w] = the global sub-center id of working row w (its first-level center's first sub-center
-// plus its nearest sub-center there; ties to the lowest).
+// plus its nearest sub-center there; ties to the lowest), shifted left by nbits and, with norms,
+// or-ed with the row's norm key: floor((log2(norm) - l0) * lscale) clamped to [0, 2^nbits) (0 for
+// a zero row).
 template <int DPM>
 __global__ __launch_bounds__(256) void sub_assign_kernel(const float* __restrict__ xs32, uint32_t dp,
                                                          const uint32_t* __restrict__ perm,
                                                          const uint32_t* __restrict__ assign, uint32_t n, uint32_t C,
                                                          const uint32_t* __restrict__ sub_cnt,
                                                          const uint32_t* __restrict__ sub_off,
-                                                         const float* __restrict__ sub_cen, uint32_t* __restrict__ key2) {
+                                                         const float* __restrict__ sub_cen, const double* __restrict__ norms,
+                                                         double l0, double lscale, uint32_t nbits,
+                                                         uint32_t* __restrict__ key2) {
   const uint32_t w = bid_x() * 256 + tid_x();
   if (w >= n) return;
   const uint32_t row = perm[w];
@@ -603,7 +637,15 @@ __global__ __launch_bounds__(256) void sub_assign_kernel(const float* __restrict
       }
     }
   }
-  key2[w] = g;
+  uint32_t nk = 0;
+  if (norms) {
+    const double v = norms[row];
+    if (v > 0.0) {
+      const double q = floor((log2(v) - l0) * lscale);
+      nk = q <= 0.0 ? 0u : q >= (double)((1u << nbits) - 1) ? (1u << nbits) - 1 : (uint32_t)q;
+    }
+  }
+  key2[w] = (g << nbits) | nk;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -698,10 +740,15 @@ hipError_t by_dpm(uint32_t dp, F f) {
 
 inline uint32_t dpm_of(uint32_t dp) { return dp <= 16 ? 16 : dp <= 32 ? 32 : dp <= 48 ? 48 : 64; }
 
+// rows per sub-center: one candidate tile (cosine), TANI_SUB_TILES of them (Tanimoto)
+inline uint32_t sub_rows(const OrderShape& o) { return o.tani ? o.pt * TANI_SUB_TILES : o.pt; }
+
 }  // namespace
 
-OrderShape order_shape(uint32_t n, uint32_t dp, uint32_t qb, uint32_t clusters) {
+OrderShape order_shape(uint32_t n, uint32_t dp, uint32_t qb, uint32_t clusters, bool tani) {
   OrderShape o{};
+  o.tani = tani ? 1 : 0;
+  o.norm_bits = tani ? TANI_NORM_BITS : 0;
   o.n = n;
   o.dp = dp;
   uint32_t C = clusters;
@@ -724,16 +771,17 @@ OrderShape order_shape(uint32_t n, uint32_t dp, uint32_t qb, uint32_t clusters) 
   o.nblk = (n + qb - 1) / qb;
   o.pt = (n + PRUNE_TILE - 1) / PRUNE_TILE <= PRUNE_MAX_TILES ? PRUNE_TILE : 2 * PRUNE_TILE;
   o.npt = (n + o.pt - 1) / o.pt;
-  // sub-centers: at most round(size / pt) per center (<= size / pt + 1/2), summed over C centers,
-  // plus one for the zero rows
-  o.G2 = n / o.pt + o.C + 2;
+  // sub-centers: at most round(size / rows) per center (<= size / rows + 1/2), summed over C
+  // centers, plus one for the zero rows
+  o.G2 = n / sub_rows(o) + o.C + 2;
   o.key_bits = 1;
   while ((1ull << o.key_bits) < o.G2) ++o.key_bits;
+  o.key_bits += o.norm_bits;
   return o;
 }
 
 bool order_supported(const OrderShape& o) {
-  return o.dp <= 64 && o.npt >= 8 && o.npt <= PRUNE_MAX_TILES && o.key_bits <= 2 * 8;
+  return o.dp <= 64 && o.npt >= 8 && o.npt <= PRUNE_MAX_TILES && o.key_bits <= 3 * 8;
 }
 
 namespace {
@@ -772,11 +820,14 @@ OrderBufs carve(Carver& c, const OrderShape& o) {
   b.qang = c.take<double>(3 * (size_t)o.nblk);
   b.tcen = c.take<double>((size_t)o.npt * o.dp);
   b.tang = c.take<double>(3 * (size_t)o.npt);
+  b.qnr = c.take<double>(2 * (size_t)o.nblk);
+  b.tnr = c.take<double>(2 * (size_t)o.npt);
   b.seq = c.take<uint32_t>((size_t)o.nblk * o.npt);
   b.sb = c.take<float>((size_t)o.nblk * o.npt);
   const size_t rb = (RESCAN_PRUNE_ROWS + o.qb - 1) / o.qb;
   b.rcen = c.take<double>(rb * o.dp);
   b.rang = c.take<double>(3 * rb);
+  b.rnr = c.take<double>(2 * rb);
   b.rseq = c.take<uint32_t>(rb * o.npt);
   b.rsb = c.take<float>(rb * o.npt);
   return b;
@@ -794,7 +845,8 @@ OrderBufs order_carve(void* base, const OrderShape& o) {
   return carve(c, o);
 }
 
-hipError_t order_rows(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b) {
+hipError_t order_rows(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b, const double* norms,
+                      double nlo, double nhi) {
   const uint32_t stride = std::max<uint32_t>(1, o.n / o.sample);
   const uint32_t nb = (o.n + 255) / 256;
   return by_dpm(o.dp, [&](auto dpm) -> hipError_t {
@@ -820,19 +872,24 @@ hipError_t order_rows(hipStream_t s, const float* xs32, const OrderShape& o, Ord
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     // second level: after the scatter, totals holds every center's first working row
-    sub_plan_kernel<<<1, 1024, 0, s>>>(b.totals, o.C, o.n, o.pt, o.sub_cap, b.sub_cnt, b.sub_off);
+    sub_plan_kernel<<<1, 1024, 0, s>>>(b.totals, o.C, o.n, sub_rows(o), o.sub_cap, b.sub_cnt, b.sub_off);
     const size_t lds_sub = (size_t)o.sub_cap * DPM * 12 + o.sub_cap * 4;
     sub_fit_kernel<DPM><<<o.C, 256, lds_sub, s>>>(xs32, o.dp, b.perm, b.totals, o.C, b.sub_cnt, b.sub_off,
                                                   o.sub_sample, o.sub_iters, b.sub_cen);
+    // Tanimoto: norm keys log-spaced over [nlo, nhi] (the smallest positive and the largest norm)
+    const bool nk = o.tani && norms;
+    const double l0 = nk && nlo > 0.0 ? log2(nlo) : 0.0;
+    const double lscale = nk && nhi > nlo && nlo > 0.0 ? (double)(1u << o.norm_bits) / (log2(nhi) - l0) : 0.0;
     sub_assign_kernel<DPM><<<nb, 256, 0, s>>>(xs32, o.dp, b.perm, b.assign, o.n, o.C, b.sub_cnt, b.sub_off, b.sub_cen,
-                                              b.key2);
-    // stable: rows of one sub-center keep the first level's order.  Two passes of RADIX_BITS (the
-    // sub-center ids have at most 2 * RADIX_BITS bits), (key2, perm) -> (key2s, perm2) -> back.
+                                              nk ? norms : nullptr, l0, lscale, nk ? o.norm_bits : 0u, b.key2);
+    // stable: rows of one key keep the first level's order.  Passes of RADIX_BITS over the key's
+    // bits (at least two), (key2, perm) -> (key2s, perm2) -> back ...; an odd count ends in perm2.
+    const uint32_t passes = std::max<uint32_t>(2, (o.key_bits + RADIX_BITS - 1) / RADIX_BITS);
     uint32_t* kin = b.key2;
     uint32_t* vin = b.perm;
     uint32_t* kout = b.key2s;
     uint32_t* vout = b.perm2;
-    for (uint32_t shift = 0; shift < 2 * RADIX_BITS; shift += RADIX_BITS) {
+    for (uint32_t shift = 0; shift < passes * RADIX_BITS; shift += RADIX_BITS) {
       clear_kernel<<<1, RADIX_B, 0, s>>>(b.totals, RADIX_B);
       digit_count_kernel<<<nb, 256, 0, s>>>(kin, o.n, shift, b.rank2, b.hist, b.totals);
       small_scan_kernel<<<1, 1024, 0, s>>>(b.totals, RADIX_B);
@@ -840,6 +897,10 @@ hipError_t order_rows(hipStream_t s, const float* xs32, const OrderShape& o, Ord
       digit_scatter_kernel<<<nb, 256, 0, s>>>(kin, vin, o.n, shift, b.rank2, b.hist, kout, vout);
       std::swap(kin, kout);
       std::swap(vin, vout);
+    }
+    if (vin != b.perm) {
+      e = hipMemcpyAsync(b.perm, vin, (size_t)o.n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) return e;
     }
     invert_kernel<<<nb, 256, 0, s>>>(b.perm, o.n, b.iperm);
     return hipGetLastError();
@@ -852,20 +913,23 @@ hipError_t order_permute(hipStream_t s, const float* xs32, const OrderShape& o, 
   return hipGetLastError();
 }
 
-hipError_t order_bounds(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b) {
+hipError_t order_bounds(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b, const double* norms) {
+  const double* nv = o.tani ? norms : nullptr;
   return by_dpm(o.dp, [&](auto dpm) -> hipError_t {
     constexpr int DPM = decltype(dpm)::value;
-    cone_kernel<DPM><<<o.nblk, 64, 0, s>>>(xs32, o.dp, b.perm, o.n, o.qb, b.qcen, b.qang);
-    cone_kernel<DPM><<<o.npt, 64, 0, s>>>(xs32, o.dp, b.perm, o.n, o.pt, b.tcen, b.tang);
+    cone_kernel<DPM><<<o.nblk, 64, 0, s>>>(xs32, o.dp, b.perm, o.n, o.qb, b.qcen, b.qang, nv, b.qnr);
+    cone_kernel<DPM><<<o.npt, 64, 0, s>>>(xs32, o.dp, b.perm, o.n, o.pt, b.tcen, b.tang, nv, b.tnr);
     uint32_t p2 = 1;
     while (p2 < o.npt) p2 <<= 1;
-    bound_sort_kernel<<<o.nblk, 1024, 0, s>>>(b.qcen, b.qang, b.tcen, b.tang, o.dp, o.npt, p2, b.seq, b.sb);
+    bound_sort_kernel<<<o.nblk, 1024, 0, s>>>(b.qcen, b.qang, b.tcen, b.tang, o.dp, o.npt, p2, b.seq, b.sb,
+                                              nv ? b.qnr : nullptr, nv ? b.tnr : nullptr);
     return hipGetLastError();
   });
 }
 
 hipError_t order_rescan(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b, uint32_t* rows,
-                        uint32_t nr) {
+                        uint32_t nr, const double* norms) {
+  const double* nv = o.tani ? norms : nullptr;
   if (nr == 0 || nr > RESCAN_PRUNE_ROWS) return hipErrorInvalidValue;
   uint32_t p2 = 1;
   while (p2 < nr) p2 <<= 1;
@@ -873,10 +937,11 @@ hipError_t order_rescan(hipStream_t s, const float* xs32, const OrderShape& o, O
   const uint32_t nblk = (nr + o.qb - 1) / o.qb;
   return by_dpm(o.dp, [&](auto dpm) -> hipError_t {
     constexpr int DPM = decltype(dpm)::value;
-    cone_kernel<DPM><<<nblk, 64, 0, s>>>(xs32, o.dp, rows, nr, o.qb, b.rcen, b.rang);
+    cone_kernel<DPM><<<nblk, 64, 0, s>>>(xs32, o.dp, rows, nr, o.qb, b.rcen, b.rang, nv, b.rnr);
     uint32_t q2 = 1;
     while (q2 < o.npt) q2 <<= 1;
-    bound_sort_kernel<<<nblk, 1024, 0, s>>>(b.rcen, b.rang, b.tcen, b.tang, o.dp, o.npt, q2, b.rseq, b.rsb);
+    bound_sort_kernel<<<nblk, 1024, 0, s>>>(b.rcen, b.rang, b.tcen, b.tang, o.dp, o.npt, q2, b.rseq, b.rsb,
+                                            nv ? b.rnr : nullptr, nv ? b.tnr : nullptr);
     return hipGetLastError();
   });
 }

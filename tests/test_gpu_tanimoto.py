@@ -4,8 +4,9 @@ The reference ranks by T = a b c / (a^2 + b^2 - a b c) on raw rows (ike12th/CosT
 crates/costal-core/src/knn.rs:197-284) and its exact path equals the brute force bit for bit
 (knn.rs:1076-1147, tests/test_knn.py:118-148).  Here the scan scores normalized rows (T depends on
 the cosine and the norm ratio only), keeps f32 Tanimoto scores in its lists, skips candidates
-whose cosine is below the cut of the list's Tanimoto threshold for the tile's norm range, and the
-f64 rescoring ranks by the f64 Tanimoto score.  Every case is checked against the oracle's f64
+whose cosine is below the cut of the list's Tanimoto threshold for the tile's norm range, skips
+tile pairs whose Tanimoto bound (cone cosine bound and norm ranges) is below every list's
+threshold, and the f64 rescoring ranks by the f64 Tanimoto score.  Every case is checked against the oracle's f64
 brute force (idx, cos, norms bitwise; weights to 1e-12 of the oracle's reweight, which the
 Tanimoto score equals).  Cases follow the reference's tests: rows rescaled so that magnitude
 matters, small signed integers with ties and zero rows, norms spanning 50x, plus the benchmark
@@ -85,10 +86,36 @@ def test_signed_integers_ties_and_zero_rows(oracle):
 
 
 def test_norms_spanning_50x(oracle):
-    """knn.rs:1119-1137: norms spanning about 50x (the reference's norm-window case)."""
-    x = synth.clustered(3000, 8, 9)
-    x *= (1.0 + (np.arange(3000) % 50)).astype(np.float32)[:, None]
-    check(np.ascontiguousarray(x), 10, oracle, "norms 50x")
+    """knn.rs:1119-1137: norms spanning about 50x, so that whole tiles fall outside every row's
+    norm window: the pruned scan skips tile pairs and stays exact."""
+    x = synth.clustered(12000, 8, 9)
+    x = np.ascontiguousarray(x * (1.0 + (np.arange(12000) % 50)).astype(np.float32)[:, None])
+    skipped = {}
+    for splits, clusters in ((0, 0), (1, 0), (1, 2)):
+        st = check(x, 10, oracle, f"norms 50x, splits {splits}, centers {clusters}", col_splits=splits,
+                   prune_clusters=clusters)
+        skipped[splits, clusters] = st["blocks_skipped"] / st["blocks_total"]
+        print(f"  splits {st['col_splits']}, centers {clusters}: skipped {st['blocks_skipped']} of "
+              f"{st['blocks_total']} tile pairs", flush=True)
+    # One candidate range per row (the reference's test has one; every split fills its own lists and
+    # so prunes with lower thresholds).  The centers of this data lie in one orthant, so direction
+    # groups barely separate rows: few groups leave long norm-sorted runs, the reference's layout.
+    assert skipped[1, 0] > 0.10
+    assert skipped[1, 2] > 0.25
+
+
+def test_prune_off_equals_on(oracle):
+    x = scaled(synth.rare_396460x15(n=20_000), 8)
+    st = check(x, 10, oracle, "pruned")
+    assert st["blocks_skipped"] > 0
+    st = check(x, 10, oracle, "unpruned", prune=False)
+    assert st["blocks_skipped"] == 0
+
+
+@pytest.mark.parametrize("clusters", [1, 7, 64, 1000])
+def test_center_count_never_changes_output(clusters, oracle):
+    x = scaled(synth.cytometry_86864x39(n=10_000), 6)
+    check(x, 10, oracle, f"{clusters} centers", prune_clusters=clusters)
 
 
 def test_duplicates_at_other_scales(oracle):

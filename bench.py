@@ -18,7 +18,7 @@ OpenMP) on a bounded sample of query rows of the same input, on this host's core
 Other workloads (--workload): cytometry_86864x39, learned_1000000x40 (k=30), batched_64 (64 uneven
 samples in one batched hipGraph call, k=15), counts_3605x2000 (wide rows: the exact row-scan path).
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--workload NAME] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--workload NAME] [--no-cpu] [--no-tanimoto]
 """
 
 from __future__ import annotations
@@ -42,8 +42,10 @@ FORM_DESC = {
     "fp16": "fp16 MFMA candidates + f32 re-score + f64 rescoring",
     "fp16x3": "split-fp16 MFMA candidates (hi.hi + hi.lo + lo.hi) + f64 rescoring",
     "fp16x3+learned": "split-fp16 MFMA candidates + learned BF16 admission thresholds + f64 rescoring",
+    "tanimoto-select": "neighbors selected by Tanimoto: split-fp16 MFMA candidates + f64 rescoring",
 }
-PRECISION_OF = {"fp32": "fp32", "fp16": "fp16", "fp16x3": "fp16x3", "fp16x3+learned": "fp16x3"}
+PRECISION_OF = {"fp32": "fp32", "fp16": "fp16", "fp16x3": "fp16x3", "fp16x3+learned": "fp16x3",
+                "tanimoto-select": "fp16x3"}
 # HBM bytes per launch of the dominant kernel: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (KiB), the scale
 # factors from a 1 GiB device copy under the same counters; rocprofv3 --pmc runs of this script,
 # mean over the first passes of the run (profiles/, one summary per kernel and workload).
@@ -72,6 +74,7 @@ def parse():
     ap.add_argument("--workload", default="rare_396460x15", choices=sorted(WORKLOADS))
     ap.add_argument("--k", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-tanimoto", action="store_true", help="skip the Tanimoto-selection timing")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline duration")
     ap.add_argument("--forms", default=None, help="forms to time (headline = last)")
     ap.add_argument("--train-cells", type=int, default=None, help="threshold-model training cells")
@@ -125,11 +128,12 @@ class DeviceRun:
         self.model = model
         self.model_params = self.torch.from_numpy(model.params).cuda()
 
-    def step(self, form: str, threshold=None):
+    def step(self, form: str, threshold=None, prune=True):
         p = self.lib.KnnParams()
         p.k = self.k
         p.margin = -1
-        p.select = 0
+        p.select = 1 if form == "tanimoto-select" else 0
+        p.no_prune = 0 if prune else 1
         p.precision = self.lib.PRECISION[PRECISION_OF[form]]
         if threshold is not None:
             p.row_threshold = threshold.data_ptr()
@@ -338,6 +342,20 @@ def main():
     if not batched:
         h2h = host_to_host(data, k, head, run.model)
 
+    # Selection by Tanimoto (select="tanimoto") on the same input: its own kNN, timed like the
+    # forms above, and checked equal to the unpruned scan of the same selection.
+    tani = None
+    if not batched and not args.no_tanimoto:
+        ms, kern, st = time_form(run, "tanimoto-select", args.steps, args.warmup)
+        got = run.outputs()
+        run.step("tanimoto-select", prune=False)
+        want = run.outputs()
+        tani = {"ms_per_step": ms, "cells_per_s": n / (ms / 1000.0), "kernel_ms": kern,
+                "fallback_rows": int(st.fallback_rows), "coarse_err_max": float(st.coarse_err_max),
+                "coarse_eps": float(st.coarse_eps), "tile_pairs": int(st.blocks_total),
+                "tile_pairs_skipped": int(st.blocks_skipped),
+                "equals_unpruned": bool(np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]))}
+
     cpu = None
     if not args.no_cpu and not batched:
         cpu, (rows, cidx, ccos) = cpu_baseline(data, k, args.cpu_seconds)
@@ -366,6 +384,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "host_to_host": h2h,
+        "tanimoto_selection": tani,
         "threshold_model": model_info,
         "tuner": tuner_info,
         "device": _lib.device_info(),

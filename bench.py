@@ -203,6 +203,8 @@ def host_to_host(x: np.ndarray, k: int, form: str, model, reps: int = 3):
     from costal_amd import _costal
 
     kw = {"matmul": {"fp32": "mfma-f32", "fp16": "mfma-f16"}.get(PRECISION_OF[form], "mfma-f16x3"), "weights": True}
+    if form == "tanimoto-select":
+        kw["select"] = "tanimoto"
     if form.endswith("+learned"):
         kw["threshold_model"] = model
     _costal._knn_exact(x, k, **kw)

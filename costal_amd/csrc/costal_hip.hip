@@ -962,18 +962,26 @@ __global__ __launch_bounds__(256) void rowscan_batch_kernel(
 // Tanimoto selection inputs of the fused scan.
 
 // The largest row norm (ext[0..1], f64 bits) and the smallest positive one (ext[2..3], the
-// complement of its bits, so that both are maxima from zero), one atomic per wave.
+// complement of its bits, so that both are maxima from zero): a grid-stride pass, one pair of
+// atomics per workgroup (per wave they queued on the one address: 78 us on 396,460 rows).
 __global__ __launch_bounds__(256) void norm_extremes_kernel(const double* __restrict__ norms, uint32_t n,
                                                             unsigned long long* __restrict__ ext) {
-  const uint32_t i = bid_x() * 256 + tid_x();
-  const double v = i < n ? norms[i] : 0.0;
-  unsigned long long hi = (unsigned long long)__double_as_longlong(v);
-  unsigned long long lo = v > 0.0 ? ~(unsigned long long)__double_as_longlong(v) : 0ull;
+  __shared__ unsigned long long s_hi[4], s_lo[4];
+  unsigned long long hi = 0, lo = 0;
+  for (uint32_t i = bid_x() * 256 + tid_x(); i < n; i += gdim_x() * 256) {
+    const double v = norms[i];
+    hi = std::max(hi, (unsigned long long)__double_as_longlong(v));
+    if (v > 0.0) lo = std::max(lo, ~(unsigned long long)__double_as_longlong(v));
+  }
   for (int off = 32; off > 0; off >>= 1) {
     hi = std::max(hi, (unsigned long long)__shfl_xor((long long)hi, off));
     lo = std::max(lo, (unsigned long long)__shfl_xor((long long)lo, off));
   }
-  if ((tid_x() & 63) == 0) {
+  if ((tid_x() & 63) == 0) s_hi[tid_x() >> 6] = hi, s_lo[tid_x() >> 6] = lo;
+  __syncthreads();
+  if (tid_x() == 0) {
+    hi = std::max(std::max(s_hi[0], s_hi[1]), std::max(s_hi[2], s_hi[3]));
+    lo = std::max(std::max(s_lo[0], s_lo[1]), std::max(s_lo[2], s_lo[3]));
     if (hi) atomicMax(&ext[0], hi);
     if (lo) atomicMax(&ext[1], lo);
   }
@@ -1649,7 +1657,8 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   HIP_TRY(hipGetLastError());
   // Tanimoto: the norm extremes come back with the non-finite check (counters[14..17])
   if (P.fused && P.tani) {
-    norm_extremes_kernel<<<(n + 255) / 256, 256, 0, s>>>(norms, n, reinterpret_cast<unsigned long long*>(counters + 14));
+    norm_extremes_kernel<<<std::min<uint32_t>((n + 255) / 256, 256), 256, 0, s>>>(
+        norms, n, reinterpret_cast<unsigned long long*>(counters + 14));
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipEventRecord(D.evk[1], s));

@@ -108,12 +108,12 @@ __device__ __forceinline__ float tani_smin(float2 r, float ia) {
   return 2.0f;
 }
 
-// The cosine cut of a Tanimoto threshold tau > 0: T > tau needs c > tau / (1 + tau) * s (T >= -1/3,
-// so 1 + tau > 0), and s >= smin; shaved below the exact value.  No cut for tau <= 0 (T of a
-// negative cosine rises with s).
-__device__ __forceinline__ float tani_cut(float tau, float smin) {
+// The cosine cut of a Tanimoto threshold tau > 0 is c_tau * smin: T > tau needs c > tau / (1 + tau)
+// * s (T >= -1/3, so 1 + tau > 0), and s >= smin.  c_tau is shaved below the exact value; -inf
+// (no cut) for tau <= 0 (T of a negative cosine rises with s).  smin is 2 or more, or inf.
+__device__ __forceinline__ float tani_ctau(float tau) {
   if (!(tau > 0.0f)) return -INFINITY;
-  return tau * __builtin_amdgcn_rcpf(1.0f + tau) * smin * TANI_SHAVE;
+  return tau * __builtin_amdgcn_rcpf(1.0f + tau) * TANI_SHAVE;
 }
 
 // f32 inner product of two zero-padded normalized rows, sequential fma: |err| <= gamma_d.
@@ -242,14 +242,23 @@ __device__ __forceinline__ f32x16 tile_mfma(const uint4 (&b)[J], const uint4 (&q
   return acc;
 }
 
-template <int J>
+// Tanimoto instances that fit two waves per SIMD without scratch when a scan step holds one
+// sub-tile: lists of 16 with J <= 6, lists of 32 with J = 2.
+template <int J, int KPR>
+constexpr bool tani_two_waves() {
+  return (KPR == 16 && J <= 6) || (KPR == 32 && J == 2);
+}
+
+template <int J, bool ONE_SUB = false>
 struct TileShape {
   static constexpr int TC = J <= 2 ? 512 : J <= 4 ? 256 : 128;  // candidates per tile: 32 KB tiles
   static constexpr int TILEB = TC * 32 * J;
   // Tanimoto: a tile's raw norms (TC floats) and its halves' norm ranges (two float2), in LDS
   static constexpr int TANIF = TC + 4;
   static constexpr int SUBT = TC / 32;                         // 32-candidate sub-tiles per tile
-  static constexpr int GS = (SUBT >= 4 && J <= 4) ? 2 : 1;     // sub-tiles per scan step
+  // sub-tiles per scan step (ONE_SUB: one, which frees the registers of a second step's
+  // accumulators and operands)
+  static constexpr int GS = (!ONE_SUB && SUBT >= 4 && J <= 4) ? 2 : 1;
   static constexpr int NSTEP = SUBT / GS;                      // scan steps per tile
 };
 
@@ -285,8 +294,9 @@ __device__ __forceinline__ void scan_block(
   constexpr int NT = 64 * NW;
   constexpr int CHUNKS = TC * 2 * J;        // 16-byte chunks per tile
   constexpr int CPT = (CHUNKS + NT - 1) / NT;
-  constexpr int GS = TileShape<J>::GS;
-  constexpr int NSTEP = TileShape<J>::NSTEP;
+  constexpr bool ONE_SUB = SEL == SEL_TANI && tani_two_waves<J, KPR>();
+  constexpr int GS = TileShape<J, ONE_SUB>::GS;
+  constexpr int NSTEP = TileShape<J, ONE_SUB>::NSTEP;
   constexpr int SUBT = TileShape<J>::SUBT;
   static_assert(NSTEP >= 2 && NSTEP % 2 == 0, "pipeline steps per tile");
   static_assert(GS * NSTEP * 32 == TC, "tile shape");
@@ -336,10 +346,10 @@ __device__ __forceinline__ void scan_block(
   float thp = -INFINITY;  // the partner list's threshold, refreshed once per tile
   uint32_t my_inserts = 0;
   // Tanimoto: list scores are f32 Tanimoto scores (tani32); a candidate whose cosine is at most the
-  // cut of the lane's threshold for its half of the tile (tani_cut with the half's smallest s) can
-  // not enter, so the max3 test stays on the cosines.  1/a of the query (0 for a zero row), the
-  // halves' smallest s and their cuts.
-  float qia = 0.0f, smin0 = 2.0f, smin1 = 2.0f, cut0 = -INFINITY, cut1 = -INFINITY;
+  // cut of the lane's threshold for its half of the tile (c_tau times the half's smallest s) can
+  // not enter, so the max3 test stays on the cosines.  1/a of the query (0 for a zero row), c_tau
+  // of the lane's threshold, the halves' smallest s.
+  float qia = 0.0f, ctau = -INFINITY, smin0 = 2.0f, smin1 = 2.0f;
   if constexpr (TANI) {
     if (qvalid) {
       const float qa = pr.tnorm[qg];
@@ -520,7 +530,7 @@ __device__ __forceinline__ void scan_block(
 #pragma unroll
     for (int u = 1; u < GS; ++u) m = fmaxf(m, mu[u]);
     const bool half1 = p * GS >= SUBT / 2;  // the step's half of the tile
-    float the = TANI ? (half1 ? cut1 : cut0) : (PREC == PREC_F16) ? fmaxf(th, thp) - eps_pre : fmaxf(th, thp);
+    float the = TANI ? ctau * (half1 ? smin1 : smin0) : (PREC == PREC_F16) ? fmaxf(th, thp) - eps_pre : fmaxf(th, thp);
     if (__builtin_amdgcn_ballot_w64(m > the) != 0) {
 #pragma unroll
       for (int u = 0; u < GS; ++u) {
@@ -554,9 +564,8 @@ __device__ __forceinline__ void scan_block(
               list_push<KPR>(ls, li, v, cand);
               th = ls[0];
               if constexpr (TANI) {
-                cut0 = tani_cut(fmaxf(th, thp), smin0);
-                cut1 = tani_cut(fmaxf(th, thp), smin1);
-                the = half1 ? cut1 : cut0;
+                ctau = tani_ctau(fmaxf(th, thp));
+                the = ctau * (half1 ? smin1 : smin0);
               } else {
                 the = (PREC == PREC_F16) ? fmaxf(th, thp) - eps_pre : fmaxf(th, thp);
               }
@@ -577,8 +586,7 @@ __device__ __forceinline__ void scan_block(
         const f32x4 rg_ = *reinterpret_cast<const f32x4*>(s_nrm + ns_cur * TANIF + TC);           \
         smin0 = tani_smin(make_float2(rg_[0], rg_[1]), qia);                                      \
         smin1 = tani_smin(make_float2(rg_[2], rg_[3]), qia);                                      \
-        cut0 = tani_cut(fmaxf(th, thp), smin0);                                                   \
-        cut1 = tani_cut(fmaxf(th, thp), smin1);                                                   \
+        ctau = tani_ctau(fmaxf(th, thp));                                                         \
       }                                                                                           \
     }                                                                                             \
     if (p == NSTEP - RD) {                                                                        \
@@ -675,10 +683,10 @@ __device__ __forceinline__ void scan_block(
 // occupancy: the split-FP16 headline instance measured 12.3 ms unbounded against 6.2 ms bounded);
 // left to the compiler where two would spill: J of 3 or 4, lists of 64, and lists of 32 with J of
 // 2, 7 or 8.  The Tanimoto scan holds a few more registers (1/a, the halves' cuts, norm staging):
-// two waves only for J = 6 with lists of 16 (the others spill at two).
+// two waves where one sub-tile per scan step makes room (tani_two_waves), else one.
 template <int J, int KPR, int SEL = SEL_COS>
 constexpr int scan_waves() {
-  if constexpr (SEL == SEL_TANI) return (J == 6 && KPR == 16) ? 2 : 1;
+  if constexpr (SEL == SEL_TANI) return tani_two_waves<J, KPR>() ? 2 : 1;
   return (KPR >= 64 || J == 3 || J == 4 || (KPR >= 32 && (J == 2 || J >= 7))) ? 1 : 2;
 }
 

@@ -93,7 +93,7 @@ def _call(fn, *args, stats=None):
 def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None, matmul=None,
                sort_by_norm=True, prune=True, n_threads=None, row_threshold=None,
                weights=False, col_splits=None, threshold_model=None, threshold_bf16=True,
-               prune_clusters=None):
+               prune_clusters=None, prune_tile=None):
     """Exact kNN: ``((idx int32[n,k], cos f64[n,k], norms f64[n]), stats)``.
 
     Extensions beyond the reference signature (keyword-only, default off):
@@ -103,7 +103,8 @@ def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None
     ``threshold_model`` -- a :class:`costal_amd.learned.ThresholdModel`; its thresholds are computed
     on the GPU inside the call (BF16 unless ``threshold_bf16=False``); the result stays exact.
     ``prune_clusters`` -- k-means centers of the locality order of the pruned scan (0/None = about
-    n/3072).  ``prune`` keeps the reference's meaning of "skip what provably cannot qualify": here
+    n/3072); ``prune_tile`` -- rows of a candidate tile of that order, 256 or 512 (0/None = 256, or
+    512 past 4096 tiles).  ``prune`` keeps the reference's meaning of "skip what provably cannot qualify": here
     candidate tiles whose cosine bound is below every list's threshold (speed only).
     """
     x = _matrix_f32(x)
@@ -126,6 +127,7 @@ def _knn_exact(x, k, *, select=None, margin=None, row_block=None, col_block=None
     p.col_splits = int(col_splits or 0)
     p.no_prune = 0 if prune else 1
     p.prune_clusters = int(prune_clusters or 0)
+    p.prune_tile = int(prune_tile or 0)
     th = None
     if row_threshold is not None:
         th = np.ascontiguousarray(row_threshold, dtype=np.float32)

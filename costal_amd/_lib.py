@@ -61,6 +61,7 @@ class KnnParams(C.Structure):
         ("mlp_bf16", C.c_uint32),
         ("prune_clusters", C.c_uint32),
         ("mlp_offset2", C.c_float),
+        ("prune_tile", C.c_uint32),
     ]
 
 
@@ -122,6 +123,27 @@ class CostalError(ValueError):
 _lib = None
 
 
+def _one_hip_runtime() -> None:
+    """Load the HIP runtime that PyTorch ships, when PyTorch is installed, before the library.
+
+    The library needs ``libamdhip64.so.7`` and binds to an already loaded runtime of that soname;
+    PyTorch's extensions need ``libamdhip64.so`` from their own directory, a copy with the same
+    soname.  Loaded in the other order (the library first), the process would hold two HIP (and
+    HSA) runtimes and PyTorch would find no GPU.  With PyTorch's copy loaded first by path, both
+    bind to it, whichever of the two touches the GPU first.  Nothing is initialised here."""
+    import importlib.util
+
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    for d in (spec.submodule_search_locations or []) if spec else []:
+        path = os.path.join(d, "lib", "libamdhip64.so")
+        if os.path.exists(path):
+            C.CDLL(path, mode=C.RTLD_GLOBAL)
+            return
+
+
 def load():
     """Load the library (once). Raises ImportError when it has not been built."""
     global _lib
@@ -132,6 +154,7 @@ def load():
             f"{LIB_PATH} is not built; build it with `python -c 'import __graft_entry__ as g; g.build()'` "
             "(hipcc --offload-arch=gfx950)"
         )
+    _one_hip_runtime()
     L = C.CDLL(LIB_PATH)
     P, U64, U32 = C.c_void_p, C.c_uint64, C.c_uint32
     L.costal_last_error.restype = C.c_char_p

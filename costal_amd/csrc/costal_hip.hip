@@ -1428,7 +1428,7 @@ Plan make_plan(const Device& D, uint64_t n, uint64_t d, uint64_t k, const costal
   // split per candidate tile of PRUNE_TILE rows).
   P.prune = false;
   if (P.fused && !p.no_prune) {
-    P.order = costal::order_shape((uint32_t)n, P.dp32, P.QB, p.prune_clusters, P.tani);
+    P.order = costal::order_shape((uint32_t)n, P.dp32, P.QB, p.prune_clusters, P.tani, p.prune_tile);
     if (costal::order_supported(P.order)) {
       P.prune = true;
       P.nsplit = std::min<uint32_t>(P.nsplit, P.order.npt);
@@ -1515,6 +1515,8 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   const uint32_t n = (uint32_t)n64, d = (uint32_t)d64, k = (uint32_t)p.k;
   if (p.precision > COSTAL_PREC_FP16X3) return fail(COSTAL_E_INVALID, "unknown precision %u", p.precision);
   if (p.select > COSTAL_SELECT_TANIMOTO) return fail(COSTAL_E_INVALID, "unknown select %u", p.select);
+  if (p.prune_tile != 0 && p.prune_tile != 256 && p.prune_tile != 512)
+    return fail(COSTAL_E_INVALID, "prune_tile=%u: need 0, 256 or 512", p.prune_tile);
   if (p.select == COSTAL_SELECT_TANIMOTO && !brute && (p.row_threshold || p.mlp_params))
     return fail(COSTAL_E_UNSUPPORTED, "admission thresholds are cosine similarities: Tanimoto selection takes none");
   Plan P = make_plan(D, n, d, k, p);

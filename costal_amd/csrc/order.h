@@ -91,8 +91,10 @@ struct OrderShape {
 };
 
 // The default shape for n rows of dp features with query blocks of qb rows (tani: for Tanimoto
-// selection).
-OrderShape order_shape(uint32_t n, uint32_t dp, uint32_t qb, uint32_t clusters, bool tani = false);
+// selection; tile: rows of a candidate tile, 0 = PRUNE_TILE unless that makes more than
+// PRUNE_MAX_TILES tiles).
+OrderShape order_shape(uint32_t n, uint32_t dp, uint32_t qb, uint32_t clusters, bool tani = false,
+                       uint32_t tile = 0);
 // Whether the bound matrix fits the sort (and the scan has something to gain).
 bool order_supported(const OrderShape& o);
 // Device bytes of the buffers (each carved on 256-byte boundaries).

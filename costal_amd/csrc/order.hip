@@ -745,7 +745,7 @@ inline uint32_t sub_rows(const OrderShape& o) { return o.tani ? o.pt * TANI_SUB_
 
 }  // namespace
 
-OrderShape order_shape(uint32_t n, uint32_t dp, uint32_t qb, uint32_t clusters, bool tani) {
+OrderShape order_shape(uint32_t n, uint32_t dp, uint32_t qb, uint32_t clusters, bool tani, uint32_t tile) {
   OrderShape o{};
   o.tani = tani ? 1 : 0;
   o.norm_bits = tani ? TANI_NORM_BITS : 0;
@@ -769,7 +769,7 @@ OrderShape order_shape(uint32_t n, uint32_t dp, uint32_t qb, uint32_t clusters, 
   o.sub_iters = 1;
   o.qb = qb;
   o.nblk = (n + qb - 1) / qb;
-  o.pt = (n + PRUNE_TILE - 1) / PRUNE_TILE <= PRUNE_MAX_TILES ? PRUNE_TILE : 2 * PRUNE_TILE;
+  o.pt = tile == 2 * PRUNE_TILE || (n + PRUNE_TILE - 1) / PRUNE_TILE > PRUNE_MAX_TILES ? 2 * PRUNE_TILE : PRUNE_TILE;
   o.npt = (n + o.pt - 1) / o.pt;
   // sub-centers: at most round(size / rows) per center (<= size / rows + 1/2), summed over C
   // centers, plus one for the zero rows

@@ -41,7 +41,7 @@ def _synthetic(n: int, d: int, seed: int = 0) -> np.ndarray:
     return np.ascontiguousarray(np.arcsinh(x / 5.0), dtype=np.float32)
 
 
-KNOBS = ("precision", "margin", "col_splits", "prune_clusters", "prune")
+KNOBS = ("precision", "margin", "col_splits", "prune_clusters", "prune_tile", "prune")
 
 
 def candidates(k: int):
@@ -50,15 +50,18 @@ def candidates(k: int):
     for prec in ("fp16x3", "fp32"):
         for m in margins:
             for ns in (0, 1, 4):
-                yield {"precision": prec, "margin": m, "col_splits": ns, "prune_clusters": 0, "prune": True}
+                yield {"precision": prec, "margin": m, "col_splits": ns, "prune_clusters": 0, "prune_tile": 0,
+                       "prune": True}
 
 
 def order_candidates(best: dict):
     """The order knobs for the best configuration of the first stage: k-means centers (0 = the
-    library's default, about n/3072) and the unpruned scan."""
+    library's default, about n/3072), candidate tiles of 512 rows instead of 256 (the tile shape
+    of the pruned scan), and the unpruned scan."""
     for c in (32, 128):
-        yield {**best, "prune_clusters": c, "prune": True}
-    yield {**best, "prune_clusters": 0, "prune": False}
+        yield {**best, "prune_clusters": c, "prune_tile": 0, "prune": True}
+    yield {**best, "prune_clusters": 0, "prune_tile": 512, "prune": True}
+    yield {**best, "prune_clusters": 0, "prune_tile": 0, "prune": False}
 
 
 def time_config(x_dev, n: int, d: int, k: int, cfg: dict, outs, reps: int = 3) -> float:
@@ -69,6 +72,7 @@ def time_config(x_dev, n: int, d: int, k: int, cfg: dict, outs, reps: int = 3) -
     p.precision = _lib.PRECISION[cfg["precision"]]
     p.col_splits = cfg["col_splits"]
     p.prune_clusters = cfg.get("prune_clusters", 0)
+    p.prune_tile = cfg.get("prune_tile", 0)
     p.no_prune = 0 if cfg.get("prune", True) else 1
     st = _lib.KnnStats()
     times = []
@@ -88,7 +92,8 @@ def choose(results: list, tie: float = 0.03) -> dict:
     fastest = min(r["ms"] for r in results)
     near = [r for r in results if r["ms"] <= (1.0 + tie) * fastest]
     return min(near, key=lambda r: (r["margin"], r["precision"] != "fp16x3", r["col_splits"] != 0, r["col_splits"],
-                                    not r.get("prune", True), r.get("prune_clusters", 0) != 0))
+                                    not r.get("prune", True), r.get("prune_clusters", 0) != 0,
+                                    r.get("prune_tile", 0) != 0))
 
 
 def tune_shape(n: int, d: int, k: int, *, reps: int = 3, seed: int = 0) -> dict:
@@ -139,7 +144,7 @@ class Tuner:
             db = int(key.split(":")[0][1:])
             self.table[key] = tune_shape(max(nb, k + 2), db, k)
             self.save()
-        return {"prune_clusters": 0, "prune": True, **self.table[key]["best"]}
+        return {"prune_clusters": 0, "prune_tile": 0, "prune": True, **self.table[key]["best"]}
 
     def plan_batch(self, shapes, k: int, fill_blocks: int = 2048):
         """Per-sample (margin, col_splits) and the batch precision: the precision chosen for the
@@ -162,4 +167,5 @@ def knn_kwargs(best: dict) -> dict:
     """Keyword arguments of ``_costal._knn_exact`` for a kept configuration."""
     matmul = {"fp16x3": "mfma-f16x3", "fp32": "mfma-f32", "fp16": "mfma-f16"}[best["precision"]]
     return {"matmul": matmul, "margin": best["margin"], "col_splits": best["col_splits"],
-            "prune_clusters": best.get("prune_clusters", 0), "prune": best.get("prune", True)}
+            "prune_clusters": best.get("prune_clusters", 0), "prune_tile": best.get("prune_tile", 0),
+            "prune": best.get("prune", True)}

@@ -73,10 +73,11 @@ typedef struct costal_knn_params {
   uint64_t col_block;    /* candidate columns per LDS tile hint; 0 = tuned (speed only) */
   uint32_t col_splits;   /* split the candidate range in this many parts; 0 = tuned (speed only) */
   /* KnnParams::prune, inverted so that zero is the default (on): 0 = the fused scan works on the
-   * rows in a locality order (spherical k-means on a sample, rows sorted by nearest center) and
-   * skips every candidate tile whose cosine bound (from the cones of the query block and the tile)
-   * is below the admission threshold of every list of the block; nonzero = every candidate tile
-   * is scanned in row order.  Speed only: the result is the same bit for bit. */
+   * rows in a locality order (spherical k-means on a sample, rows sorted by nearest center; for
+   * Tanimoto selection also by norm) and skips every candidate tile whose score bound (from the
+   * cones, and norm ranges, of the query block and the tile) is below the admission threshold of
+   * every list of the block; nonzero = every candidate tile is scanned in row order.  Speed only:
+   * the result is the same bit for bit. */
   uint32_t no_prune;
   /* Optional per-row admission thresholds (f32, n entries; host memory for costal_knn_exact,
    * device memory for costal_knn_exact_device): candidates whose coarse similarity does not exceed
@@ -96,6 +97,9 @@ typedef struct costal_knn_params {
    * candidates rescan with 1 - exp(model(row) + mlp_offset2) - margin when mlp_offset2 is above
    * mlp_offset (the rows short again go to the exact row scan); otherwise without a threshold. */
   float mlp_offset2;
+  /* Rows of a candidate tile of the locality order (the unit the pruned scan skips): 256 or 512;
+   * 0 = 256, or 512 when 256 would make more than 4096 tiles.  Speed only. */
+  uint32_t prune_tile;
 } costal_knn_params;
 
 /* KnnStats (knn.rs:344-357) plus GPU-side counters */

@@ -7,6 +7,7 @@ must behave the same here; they run on the host before the device is touched.
 
 import os
 import re
+import sys
 
 import numpy as np
 import pytest
@@ -159,6 +160,22 @@ def test_lcg_matches_rust_generator():
         assert g.next_f32() == np.float32(s >> 40) / np.float32(1 << 24)
 
 
+def test_one_hip_runtime_whichever_loads_first():
+    """The library loaded before PyTorch must not bring a second HIP runtime into the process
+    (PyTorch would then find no GPU): a fresh process loads the library, then imports torch."""
+    import subprocess
+
+    code = ("from costal_amd import _lib; _lib.load(); import torch; "
+            "m = open('/proc/self/maps').read().splitlines(); "
+            "print(sorted({l.split()[-1] for l in m if 'libamdhip64' in l}))")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    import ast
+
+    paths = ast.literal_eval(out.stdout.strip().splitlines()[-1])
+    assert len(paths) == 1, paths
+
+
 def test_tuner_choice_is_stable_under_timing_noise():
     """Configurations within 3% of the fastest are ties, settled by the shortest lists, then split
     FP16, then tuned splits, so one noisy timing run does not flip the plan."""
@@ -186,9 +203,10 @@ def test_tuner_grid_covers_the_order_knobs(tmp_path):
     first = list(tuner.candidates(10))
     assert all(c["prune"] and c["prune_clusters"] == 0 for c in first)
     assert {c["precision"] for c in first} == {"fp16x3", "fp32"}
-    best = {"precision": "fp16x3", "margin": 5, "col_splits": 0, "prune_clusters": 0, "prune": True}
+    best = {"precision": "fp16x3", "margin": 5, "col_splits": 0, "prune_clusters": 0, "prune_tile": 0, "prune": True}
     second = list(tuner.order_candidates(best))
-    assert {(c["prune"], c["prune_clusters"]) for c in second} == {(True, 32), (True, 128), (False, 0)}
+    assert {(c["prune"], c["prune_clusters"], c["prune_tile"]) for c in second} == {
+        (True, 32, 0), (True, 128, 0), (True, 0, 512), (False, 0, 0)}
     assert all(c["margin"] == 5 and c["precision"] == "fp16x3" for c in second)
     res = [{**best, "ms": 1.0}] + [{**c, "ms": 0.99} for c in second]
     assert tuner.choose(res) == res[0]
@@ -197,9 +215,9 @@ def test_tuner_grid_covers_the_order_knobs(tmp_path):
     tn = tuner.Tuner(str(tmp_path / "tune.json"))
     tn.table[tuner.bucket(5000, 40, 10)] = {"best": {"precision": "fp32", "margin": 10, "col_splits": 1}}
     got = tn.get(5000, 40, 10)
-    assert got["prune"] and got["prune_clusters"] == 0
+    assert got["prune"] and got["prune_clusters"] == 0 and got["prune_tile"] == 0
     assert tuner.knn_kwargs(got) == {"matmul": "mfma-f32", "margin": 10, "col_splits": 1, "prune_clusters": 0,
-                                     "prune": True}
+                                     "prune_tile": 0, "prune": True}
 
 
 def test_batch_plan_drops_splits_when_samples_fill_the_gpu(tmp_path):

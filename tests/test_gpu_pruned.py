@@ -72,6 +72,23 @@ def test_center_count_never_changes_output(clusters, cases):
     assert_same(got, want, f"{clusters} centers")
 
 
+@pytest.mark.parametrize("tile", [256, 512])
+@pytest.mark.parametrize("name", ["rare_20000x15", "cytometry_12000x39"])
+def test_candidate_tile_rows_never_change_output(name, tile, cases):
+    """Candidate tiles of 512 rows: one LDS tile (J <= 2) or four (J > 4) per candidate tile."""
+    x, k, want = cases[name]
+    got, st = ext._knn_exact(x, k, prune_tile=tile)
+    assert_same(got, want, f"{name} tile {tile}")
+    assert st["blocks_total"] == -(-len(x) // 128) * -(-len(x) // tile)
+
+
+def test_candidate_tile_rows_are_checked():
+    from costal_amd import _lib
+
+    with pytest.raises(_lib.CostalError, match="prune_tile=300: need 0, 256 or 512"):
+        ext._knn_exact(synth.clustered(600, 8, 1), 5, prune_tile=300)
+
+
 @pytest.mark.parametrize("matmul", BACKENDS)
 def test_zero_rows_duplicates_and_integer_ties(matmul, oracle):
     """Zero rows (cosine 0 with everything; sorted after every center), exact duplicates of other

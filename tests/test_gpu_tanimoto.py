@@ -118,6 +118,12 @@ def test_center_count_never_changes_output(clusters, oracle):
     check(x, 10, oracle, f"{clusters} centers", prune_clusters=clusters)
 
 
+@pytest.mark.parametrize("name", ["rare_20000x15", "cytometry_12000x39"])
+def test_candidate_tiles_of_512_rows(name, oracle):
+    x = {"rare_20000x15": synth.rare_396460x15(n=20_000), "cytometry_12000x39": synth.cytometry_86864x39(n=12_000)}[name]
+    check(scaled(x, 12), 10, oracle, name + " tiles of 512", prune_tile=512)
+
+
 def test_duplicates_at_other_scales(oracle):
     """Exact duplicates and copies scaled by 2 and 0.5 (cosine 1, Tanimoto 0.8 with the original)."""
     rng = np.random.default_rng(3)

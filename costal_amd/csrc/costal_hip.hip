@@ -1045,13 +1045,15 @@ __global__ void rescan_thresholds_kernel(const uint32_t* __restrict__ rows, cons
   th[rows[q]] = t;
 }
 
-// Second-chance thresholds of the rows left short by the learned ones: th = 1 - e - margin with
-// e = exp(u + offset), so e * exp(offset2 - offset) = exp(u + offset2).  Any threshold keeps the
-// result exact (a row short again is recomputed by the row scan); this one only has to be low.
-__global__ void second_chance_kernel(const uint32_t* __restrict__ rows, uint32_t nrows, const float* __restrict__ th,
-                                     float margin, float factor, float* __restrict__ th2) {
+// Second-chance thresholds of the rows left short by the learned ones (lo = -inf; the others keep
+// theirs): th = 1 - e - margin with e = exp(u + offset), so e * exp(offset2 - offset) =
+// exp(u + offset2).  Any threshold keeps the result exact (a row short again is rescanned without
+// one); this one only has to be low.
+__global__ void second_chance_kernel(const uint32_t* __restrict__ rows, const double* __restrict__ lo, uint32_t nrows,
+                                     const float* __restrict__ th, float margin, float factor,
+                                     float* __restrict__ th2) {
   const uint32_t q = bid_x() * bdim_x() + tid_x();
-  if (q >= nrows) return;
+  if (q >= nrows || lo[q] > -INFINITY) return;
   const uint32_t r = rows[q];
   th2[r] = 1.0f - (1.0f - th[r] - margin) * factor - margin;
 }
@@ -1070,25 +1072,6 @@ __global__ void split_failures_kernel(const uint32_t* __restrict__ rows, const d
     bounded_lo[p] = lo[q];
   } else {
     unbounded[atomicAdd(nu, 1u)] = rows[q];
-  }
-}
-
-// out = the rows with a finite threshold, then the others (one workgroup; the order within each
-// kind is not kept).
-__global__ __launch_bounds__(1024) void partition_rows_kernel(const uint32_t* __restrict__ rows, uint32_t nrows,
-                                                              const float* __restrict__ th, uint32_t* __restrict__ out) {
-  __shared__ uint32_t s_cnt[3];
-  if (tid_x() < 3) s_cnt[tid_x()] = 0;
-  __syncthreads();
-  for (uint32_t q = tid_x(); q < nrows; q += bdim_x())
-    if (th[rows[q]] > -INFINITY) atomicAdd(&s_cnt[0], 1u);
-  __syncthreads();
-  const uint32_t with = s_cnt[0];
-  for (uint32_t q = tid_x(); q < nrows; q += bdim_x()) {
-    const uint32_t r = rows[q];
-    const bool has = th[r] > -INFINITY;
-    const uint32_t pos = has ? atomicAdd(&s_cnt[1], 1u) : with + atomicAdd(&s_cnt[2], 1u);
-    out[pos] = r;
   }
 }
 
@@ -1789,30 +1772,31 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     HIP_TRY(hipEventRecord(D.evk[4], s));
     const uint32_t grid_fb = std::min<uint32_t>(n, 4u * (uint32_t)std::max(D.cus, 1));
     // Rows that failed the first pass (admission thresholds above their k-th similarity, or lists
-    // too short for their ties) are rescanned without thresholds and with longer lists, as one
-    // batched scan over the row list (wide candidate splits keep the few rows' scan parallel); only
-    // rows that fail again (exact ties) go to the full-row kernel.  (Learned thresholds lowered by
-    // the model's offset once more were measured slower: the rows the model mispredicts are the
-    // ones that fail again, and the full-row kernel is slow.)
+    // too short for their ties) are rescanned with longer lists as one batched scan over the row
+    // list (wide candidate splits keep the few rows' scan parallel); only rows that fail again
+    // (exact ties) go to the full-row kernel.
     uint32_t nfb_short[2] = {0, 0};  // rows that failed, and those of them with fewer than k candidates
     HIP_TRY(hipMemcpyAsync(nfb_short, counters + 1, sizeof nfb_short, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    const uint32_t nfb = nfb_short[0], nwith = nfb_short[0] - nfb_short[1];
+    const uint32_t nfb = nfb_short[0], nshort = nfb_short[1];
     if (nfb > 0) {
       CoarseFn fn2 = coarse_select(P.prec, P.J, list_template(kp2), P.tani);
       if (!fn2) return fail(COSTAL_E_UNSUPPORTED, "no coarse kernel for d=%u", d);
-      // A row that failed only the certificate has its k-th exact score s_k among the first
-      // pass's candidates (fb_lo = s_k - eps32), and its true k-th is at least that: its rescan
-      // admits only scores above fb_lo - eps_list, which every true neighbor's list score exceeds,
-      // with lists of kp2 for its near-ties.  A row with fewer than k candidates (its admission
-      // threshold was too high) rescans without a threshold and with the first pass's lists.
+      // One rescan of every failed row, with lists of kp2 for near-ties.  A row that failed only
+      // the certificate has its k-th exact score s_k among the first pass's candidates (fb_lo =
+      // s_k - eps32), and its true k-th is at least that: it admits only scores above fb_lo -
+      // eps_list, which every true neighbor's list score exceeds.  A row with fewer than k
+      // candidates (its admission threshold was too high) rescans with a second, lower learned
+      // threshold when the model has one (so that the rescan still prunes), else with none.  The
+      // thresholds are indexed by row, so both kinds share the launch.
       rescan_thresholds_kernel<<<(nfb + 255) / 256, 256, 0, s>>>(fb_rows, fb_lo, nfb, eps_list, th_rescan);
       HIP_TRY(hipGetLastError());
-      // rows with a threshold first, then the rest, as two rescans (fb2_rows is free until then)
-      if (nwith > 0 && nwith < nfb) {
-        partition_rows_kernel<<<1, 1024, 0, s>>>(fb_rows, nfb, th_rescan, fb2_rows);
+      const bool chance = nshort > 0 && learned && p.mlp_offset2 != 0.0f && p.mlp_offset2 > p.mlp_offset;
+      if (chance) {
+        const float margin = (float)(2.0 * eps_list);
+        const float factor = (float)std::exp((double)p.mlp_offset2 - (double)p.mlp_offset);
+        second_chance_kernel<<<(nfb + 255) / 256, 256, 0, s>>>(fb_rows, fb_lo, nfb, th_ws, margin, factor, th_rescan);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(fb_rows, fb2_rows, (size_t)nfb * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
       }
       // max_splits, and list entries per row for the rescoring (2 * splits * kpx at most max_entries)
       auto rescan = [&](CoarseFn f, uint32_t kpx, uint32_t* rows, uint32_t nr, const float* th, uint32_t* outr,
@@ -1847,41 +1831,25 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
         }
         return scan_pass(f, kpx, rows, nr, th, ns, tiles_per * P.TC, 0, outr, outl, outc, outs, pr);
       };
-      if (nwith > 0 && (rc = rescan(fn2, kp2, fb_rows, nwith, th_rescan, fb2_rows, fb2_lo, counters + 3, counters + 4,
-                                    32, RESCORE_CMAX)))
+      if ((rc = rescan(fn2, kp2, fb_rows, nfb, th_rescan, fb2_rows, fb2_lo, counters + 3, counters + 4, 32,
+                       RESCORE_CMAX)))
         return rc;
-      if (nfb > nwith) {
-        // short rows: a second, lower learned threshold when the model has one (so that the rescan
-        // prunes at once), else none
-        const float* th_short = nullptr;
-        if (learned && p.mlp_offset2 != 0.0f && p.mlp_offset2 > p.mlp_offset) {
-          const float margin = (float)(2.0 * eps_list);
-          const float factor = (float)std::exp((double)p.mlp_offset2 - (double)p.mlp_offset);
-          second_chance_kernel<<<(nfb - nwith + 255) / 256, 256, 0, s>>>(fb_rows + nwith, nfb - nwith, th_ws, margin,
-                                                                         factor, th_rescan);
+      if (chance) {
+        // Rows the second chance left short are few, and the latency of their scan is what
+        // counts: a threshold-free rescan with up to 96 splits (and at most 1024 list entries per
+        // row, so that the rescoring of the few rows stays quick), not the row scan (which would
+        // stream every candidate of such a row through one workgroup).
+        uint32_t c34[2] = {0, 0};
+        HIP_TRY(hipMemcpyAsync(c34, counters + 3, sizeof c34, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (c34[1] > 0) {
+          split_failures_kernel<<<(c34[0] + 255) / 256, 256, 0, s>>>(fb2_rows, fb2_lo, c34[0], fb_rows, fb_lo,
+                                                                    counters + 5, fb3_rows, counters + 6);
           HIP_TRY(hipGetLastError());
-          th_short = th_rescan;
-        }
-        if ((rc = rescan(fn, P.kp, fb_rows + nwith, nfb - nwith, th_short, fb2_rows, fb2_lo, counters + 3,
-                         counters + 4, 32, RESCORE_CMAX)))
-          return rc;
-        if (th_short) {
-          // Rows the second chance left short are few, and the latency of their scan is what
-          // counts: a threshold-free rescan with up to 96 splits (and at most 1024 list entries per
-          // row, so that the rescoring of the few rows stays quick), not the row scan (which would
-          // stream every candidate of such a row through one workgroup).
-          uint32_t c34[2] = {0, 0};
-          HIP_TRY(hipMemcpyAsync(c34, counters + 3, sizeof c34, hipMemcpyDeviceToHost, s));
-          HIP_TRY(hipStreamSynchronize(s));
-          if (c34[1] > 0) {
-            split_failures_kernel<<<(c34[0] + 255) / 256, 256, 0, s>>>(fb2_rows, fb2_lo, c34[0], fb_rows, fb_lo,
-                                                                      counters + 5, fb3_rows, counters + 6);
-            HIP_TRY(hipGetLastError());
-            if ((rc = rescan(fn, P.kp, fb3_rows, c34[1], nullptr, fb2_rows, fb2_lo, counters + 7, counters + 12,
-                             96, 1024)))
-              return rc;
-            third = true;
-          }
+          if ((rc = rescan(fn, P.kp, fb3_rows, c34[1], nullptr, fb2_rows, fb2_lo, counters + 7, counters + 12,
+                           96, 1024)))
+            return rc;
+          third = true;
         }
       }
     }

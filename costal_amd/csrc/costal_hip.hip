@@ -517,14 +517,37 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
     }
     __syncthreads();
   } else {
-    int P = 2;
-    while ((uint32_t)P < filled) P <<= 1;
-    for (int e = (int)filled + tid_x(); e < P; e += 64) {
-      s_sc[e] = -INFINITY;
-      s_id[e] = NONE;
+    // More entries than threads (rescans with many splits): only the k best are needed, in order,
+    // so k rounds of a wave-wide arg-max under the same order (each thread takes the best entry
+    // it holds, the wave the best of those, which then leaves the pool) instead of a full sort
+    // (k <= 64: round r's winner stays with thread r).  A bitonic sort of 1,024 entries by one
+    // wave took 133 us per row.
+    const uint32_t t = tid_x();
+    const uint32_t rounds = k < filled ? k : filled;
+    double win_sc = -INFINITY;
+    uint32_t win_id = NONE;
+    for (uint32_t r = 0; r < rounds; ++r) {
+      double bs = -INFINITY;
+      uint32_t bi = NONE, be = NONE;
+      for (uint32_t e = t; e < filled; e += 64) {
+        const uint32_t id = s_id[e];
+        const double sc = s_sc[e];
+        if (id != NONE && (bi == NONE || ranks_before(sc, id, bs, bi))) bs = sc, bi = id, be = e;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double os = __shfl_xor(bs, off);
+        const uint32_t oi = (uint32_t)__shfl_xor((int)bi, off), oe = (uint32_t)__shfl_xor((int)be, off);
+        if (oi != NONE && (bi == NONE || ranks_before(os, oi, bs, bi))) bs = os, bi = oi, be = oe;
+      }
+      if (t == r) win_sc = bs, win_id = bi;
+      if (t == 0) s_id[be] = NONE;
+      __syncthreads();
+    }
+    if (t < rounds) {
+      s_sc[t] = win_sc;
+      s_id[t] = win_id;
     }
     __syncthreads();
-    bitonic_lds<64>(s_sc, nullptr, s_id, P);
   }
 
   bool ok;

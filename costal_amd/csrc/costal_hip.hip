@@ -1209,6 +1209,10 @@ struct Device {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evk[8] = {};
   hipEvent_t evcopy = nullptr;  // a device-to-host copy done (the stream may run on behind it)
+  // a second stream (default priority) for work that overlaps the call's own: the learned
+  // thresholds run there beside the locality order; fork and join events
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 // One state per device, selected by the device current at the call (a process may drive several
@@ -1244,6 +1248,9 @@ int device_init(Device& D) {
   HIP_TRY(hipEventCreate(&D.ev0));
   HIP_TRY(hipEventCreate(&D.ev1));
   HIP_TRY(hipEventCreateWithFlags(&D.evcopy, hipEventDisableTiming));
+  HIP_TRY(hipStreamCreateWithFlags(&D.side, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&D.ev_fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&D.ev_join, hipEventDisableTiming));
   for (auto& e : D.evk) HIP_TRY(hipEventCreate(&e));
   // the device's own address of the static scratch (the device is current here)
   D.scratch_err = hipGetSymbolAddress(&D.scratch, HIP_SYMBOL(scratch));
@@ -1653,6 +1660,24 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   double* norms = norms_out ? norms_out : norms_ws;
 
   HIP_TRY(hipEventRecord(D.ev0, s));
+  // The learned thresholds only read x: they run on the side stream from the start (behind
+  // whatever the caller queued on s before the call), beside the prep and the locality order, and
+  // join s before their first reader (the bound sort, or the scan).
+  const bool learned_early = P.fused && learned;
+  if (learned_early) {
+    HIP_TRY(hipEventRecord(D.ev_fork, s));
+    HIP_TRY(hipStreamWaitEvent(D.side, D.ev_fork, 0));
+    // lowered by twice the list error bound, so that a row whose k-th similarity the model bounds
+    // correctly also clears the certificate
+    const double eps_l = P.prec == PREC_F16X3 ? fp16x3_eps(d) : fp32_eps(d);
+    rc = costal_mlp_predict_device(x, n, d, p.mlp_params, p.mlp_bf16, p.mlp_offset, (float)(2.0 * eps_l), 0, th_ws,
+                                   D.side);
+    HIP_TRY(hipEventRecord(D.ev_join, D.side));
+    if (rc) {
+      (void)hipStreamSynchronize(D.side);
+      return fail(rc, "threshold model: %s", costal_mlp_last_error());
+    }
+  }
   counters_init_kernel<<<1, 256, 0, s>>>(counters, COUNTER_WORDS, 1);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(D.evk[0], s));
@@ -1673,21 +1698,10 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   uint32_t head[18] = {};
   HIP_TRY(hipMemcpyAsync(head, counters, (P.fused && P.tani ? 18 : 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipEventRecord(D.evcopy, s));
-  // The learned thresholds go behind the copy, so that the device works on them while the host
-  // waits for the non-finite check (the model only reads x; its output is unused if x is bad).
-  const bool learned_early = P.fused && learned;
-  if (learned_early) {
-    // lowered by twice the list error bound, so that a row whose k-th similarity the model bounds
-    // correctly also clears the certificate
-    const double eps_l = P.prec == PREC_F16X3 ? fp16x3_eps(d) : fp32_eps(d);
-    rc = costal_mlp_predict_device(x, n, d, p.mlp_params, p.mlp_bf16, p.mlp_offset, (float)(2.0 * eps_l), 0, th_ws,
-                                   s);
-    if (rc) return fail(rc, "threshold model: %s", costal_mlp_last_error());
-  }
   HIP_TRY(hipEventSynchronize(D.evcopy));
   const uint32_t bad = head[0];
   if (bad != NONE) {
-    if (learned_early) HIP_TRY(hipStreamSynchronize(s));  // nothing still reads x when we return
+    if (learned_early) HIP_TRY(hipStreamSynchronize(D.side));  // nothing still reads x when we return
     if (st) st->nonfinite_row = bad;
     return fail(COSTAL_E_NON_FINITE, "row %u contains NaN or inf", bad);
   }
@@ -1710,9 +1724,12 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     HIP_TRY(hipEventRecord(D.evk[6], s));
     HIP_TRY(costal::order_rows(s, xs32, P.order, ob, norms, norm_lo, norm_hi));
     HIP_TRY(costal::order_permute(s, xs32, P.order, ob, P.n_pad, P.dp16, split16, xs32w, xs16));
-    HIP_TRY(costal::order_bounds(s, xs32, P.order, ob, norms));
+    // the bound sort leaves out the tiles below every first-pass threshold of a block
+    if (learned_early) HIP_TRY(hipStreamWaitEvent(s, D.ev_join, 0));
+    HIP_TRY(costal::order_bounds(s, xs32, P.order, ob, norms, learned_early ? th_ws : p.row_threshold));
     HIP_TRY(hipEventRecord(D.evk[7], s));
   }
+  if (learned_early && !P.prune) HIP_TRY(hipStreamWaitEvent(s, D.ev_join, 0));  // the scan reads them
   if (P.fused && P.tani) {
     tani_norms_kernel<<<P.n_pad / (P.TC / 2), 64, 0, s>>>(norms, P.prune ? ob.perm : nullptr, n, P.TC / 2, tani_scale,
                                                          tnorm, trange);

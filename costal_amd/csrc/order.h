@@ -115,9 +115,11 @@ hipError_t order_rows(hipStream_t s, const float* xs32, const OrderShape& o, Ord
 hipError_t order_permute(hipStream_t s, const float* xs32, const OrderShape& o, const OrderBufs& b, uint32_t n_pad,
                          uint32_t dp16, int split16, float* xs32w, _Float16* xs16w);
 // Cones (and for Tanimoto norm ranges, from norms) of the query blocks and candidate tiles, the
-// bound matrix and its per-block order.
+// bound matrix and its per-block order.  th0: the first pass's admission thresholds by original
+// row (device memory; null: none): a block's tiles whose bound is below all of its rows'
+// thresholds, which its scan never reaches, are left unsorted after the others.
 hipError_t order_bounds(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b,
-                        const double* norms = nullptr);
+                        const double* norms = nullptr, const float* th0 = nullptr);
 // A rescan's row list (original rows, nr <= RESCAN_PRUNE_ROWS): sorted in place into working
 // order (so that its blocks of qb rows are groups of similar rows), their cones, and every block's
 // tiles in bound order in rseq / rsb.  Needs order_bounds first (the tile cones).

@@ -444,17 +444,43 @@ __device__ __forceinline__ double tani_bound(double c, double amin, double amax,
 // tile first): seq[b*npt + r] = tile of rank r, sb[b*npt + r] = its bound, an f32 rounded up.
 // 1024 threads; a value is (bound key << 32) | ~tile, so that one comparison orders both.  With
 // norm ranges (qnr, tnr), the bound is on Tanimoto scores (tani_bound of the cosine bound).
+// With the first pass's admission thresholds (th0 by original row, the block's rows perm[b*qb ..]),
+// only the tiles whose bound reaches the block's lowest threshold are sorted; the others, which
+// the scan can never reach (its stop test compares the bound with lists' thresholds, which start
+// at these and only rise), follow them in tile order.
 __global__ __launch_bounds__(1024) void bound_sort_kernel(const double* __restrict__ qcen, const double* __restrict__ qang,
                                                           const double* __restrict__ tcen, const double* __restrict__ tang,
                                                           uint32_t dp, uint32_t npt, uint32_t p2, uint32_t* __restrict__ seq,
                                                           float* __restrict__ sb, const double* __restrict__ qnr,
-                                                          const double* __restrict__ tnr) {
+                                                          const double* __restrict__ tnr, const float* __restrict__ th0,
+                                                          const uint32_t* __restrict__ perm, uint32_t qb, uint32_t n) {
   constexpr int EMAX = PRUNE_MAX_TILES / 1024;
   __shared__ uint64_t s_buf[PRUNE_MAX_TILES];
   __shared__ double s_q[64];
-  const uint32_t b = bid_x(), t0 = tid_x();
+  __shared__ float s_cut[16];
+  __shared__ uint32_t s_wq[16];
+  const uint32_t b = bid_x(), t0 = tid_x(), wv = t0 >> 6, lane = t0 & 63;
   if (t0 < 64) s_q[t0] = t0 < dp ? qcen[(size_t)b * dp + t0] : 0.0;
+  // the block's lowest threshold (a NaN threshold admits everything: -inf); -inf without any
+  float cut = -INFINITY;
+  if (th0) {
+    float m = INFINITY;
+    for (uint32_t r = t0; r < qb; r += 1024) {
+      const uint32_t w = b * qb + r;
+      if (w < n) {
+        const float t = th0[perm[w]];
+        m = fminf(m, (__float_as_uint(t) & 0x7fffffffu) > 0x7f800000u ? -INFINITY : t);
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off));
+    if (lane == 0) s_cut[wv] = m;
+  }
   __syncthreads();
+  if (th0) {
+    cut = s_cut[0];
+    for (uint32_t v = 1; v < 16; ++v) cut = fminf(cut, s_cut[v]);
+  }
   const double aq = qang[3 * b], cq = qang[3 * b + 1], sq = qang[3 * b + 2];
   const int E = (int)(p2 > 1024 ? p2 / 1024 : 1);
   uint64_t v[EMAX];
@@ -483,11 +509,70 @@ __global__ __launch_bounds__(1024) void bound_sort_kernel(const double* __restri
       v[s] = ((uint64_t)float_key(key) << 32) | (uint32_t)~t;
     }
   }
-  block_sort_desc<EMAX>(v, E, s_buf);
+  if (!th0) {
+    // no thresholds: every tile sorted, straight from the registers
+    block_sort_desc<EMAX>(v, E, s_buf);
+#pragma unroll
+    for (int s = 0; s < EMAX; ++s) {
+      const uint32_t r = (uint32_t)s * 1024 + t0;
+      if (s < E && r < npt) {
+        seq[(size_t)b * npt + r] = ~(uint32_t)v[s];
+        sb[(size_t)b * npt + r] = key_float((uint32_t)(v[s] >> 32));
+      }
+    }
+    return;
+  }
+  // Compaction in tile order: the tiles whose bound reaches the cut to s_buf[0 .. count), the
+  // others after them in the output (ranks by wave ballots and per-wave counts; a tile's rank
+  // among the others is its index less the qualifying tiles before it).
+  constexpr uint32_t NO_RANK = 0xffffffffu;
+  uint32_t base = 0;
+  uint32_t orank[EMAX];
+#pragma unroll
+  for (int s = 0; s < EMAX; ++s) {
+    orank[s] = NO_RANK;
+    if (s < E) {
+      const uint32_t t = (uint32_t)s * 1024 + t0;
+      const bool real = t < npt;
+      const bool q = real && key_float((uint32_t)(v[s] >> 32)) >= cut;
+      const uint64_t bal = __builtin_amdgcn_ballot_w64(q);
+      const uint32_t below = (uint32_t)__builtin_popcountll(bal & (lane ? (~0ull >> (64 - lane)) : 0ull));
+      if (lane == 0) s_wq[wv] = (uint32_t)__builtin_popcountll(bal);
+      __syncthreads();
+      uint32_t before = base, total = base;
+      for (uint32_t w = 0; w < 16; ++w) {
+        before += w < wv ? s_wq[w] : 0u;
+        total += s_wq[w];
+      }
+      if (q) s_buf[before + below] = v[s];
+      else if (real) orank[s] = t - (before + below);
+      __syncthreads();
+      base = total;
+    }
+  }
+  const uint32_t count = base;
+#pragma unroll
+  for (int s = 0; s < EMAX; ++s) {
+    if (orank[s] != NO_RANK) {
+      const uint32_t r = count + orank[s];
+      seq[(size_t)b * npt + r] = ~(uint32_t)v[s];
+      sb[(size_t)b * npt + r] = key_float((uint32_t)(v[s] >> 32));
+    }
+  }
+  // sort the qualifying ones (the network over the next power of two of them, 1024 at least)
+  int E2 = 1;
+  while ((uint32_t)E2 * 1024 < count) E2 <<= 1;
 #pragma unroll
   for (int s = 0; s < EMAX; ++s) {
     const uint32_t r = (uint32_t)s * 1024 + t0;
-    if (s < E && r < npt) {
+    v[s] = (s < E2 && r < count) ? s_buf[r] : 0ull;
+  }
+  __syncthreads();
+  block_sort_desc<EMAX>(v, E2, s_buf);
+#pragma unroll
+  for (int s = 0; s < EMAX; ++s) {
+    const uint32_t r = (uint32_t)s * 1024 + t0;
+    if (s < E2 && r < count) {
       seq[(size_t)b * npt + r] = ~(uint32_t)v[s];
       sb[(size_t)b * npt + r] = key_float((uint32_t)(v[s] >> 32));
     }
@@ -913,7 +998,8 @@ hipError_t order_permute(hipStream_t s, const float* xs32, const OrderShape& o, 
   return hipGetLastError();
 }
 
-hipError_t order_bounds(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b, const double* norms) {
+hipError_t order_bounds(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b, const double* norms,
+                        const float* th0) {
   const double* nv = o.tani ? norms : nullptr;
   return by_dpm(o.dp, [&](auto dpm) -> hipError_t {
     constexpr int DPM = decltype(dpm)::value;
@@ -922,7 +1008,7 @@ hipError_t order_bounds(hipStream_t s, const float* xs32, const OrderShape& o, O
     uint32_t p2 = 1;
     while (p2 < o.npt) p2 <<= 1;
     bound_sort_kernel<<<o.nblk, 1024, 0, s>>>(b.qcen, b.qang, b.tcen, b.tang, o.dp, o.npt, p2, b.seq, b.sb,
-                                              nv ? b.qnr : nullptr, nv ? b.tnr : nullptr);
+                                              nv ? b.qnr : nullptr, nv ? b.tnr : nullptr, th0, b.perm, o.qb, o.n);
     return hipGetLastError();
   });
 }
@@ -941,7 +1027,7 @@ hipError_t order_rescan(hipStream_t s, const float* xs32, const OrderShape& o, O
     uint32_t q2 = 1;
     while (q2 < o.npt) q2 <<= 1;
     bound_sort_kernel<<<nblk, 1024, 0, s>>>(b.rcen, b.rang, b.tcen, b.tang, o.dp, o.npt, q2, b.rseq, b.rsb,
-                                            nv ? b.rnr : nullptr, nv ? b.tnr : nullptr);
+                                            nv ? b.rnr : nullptr, nv ? b.tnr : nullptr, nullptr, nullptr, o.qb, nr);
     return hipGetLastError();
   });
 }

@@ -13,7 +13,7 @@ from costal_amd import _lib, synth
 pytestmark = pytest.mark.gpu
 
 
-def device_call(fn_name, x, k, matmul="fp16x3", threshold=None):
+def device_call(fn_name, x, k, matmul="fp16x3", threshold=None, select="cosine"):
     import torch
 
     L = _lib.load()
@@ -28,6 +28,7 @@ def device_call(fn_name, x, k, matmul="fp16x3", threshold=None):
     p.k = k
     p.margin = -1
     p.precision = _lib.PRECISION[matmul]
+    p.select = _lib.SELECT[select]
     th = None
     if threshold is not None:
         th = torch.from_numpy(threshold).to(dev)
@@ -59,6 +60,21 @@ def test_noalloc_equals_device_entry_point_twice(matmul):
         np.testing.assert_array_equal(b[2], a[2])
         np.testing.assert_allclose(b[3], a[3], rtol=1e-12, atol=0)
         assert b[4].fallback_rows == a[4].fallback_rows and b[4].candidates == a[4].candidates
+
+
+def test_noalloc_tanimoto_selection_twice():
+    """Tanimoto selection (its norm tables also live in the static scratch), two calls, the second
+    smaller."""
+    for n in (40_000, 3_000):
+        x = synth.cytometry_86864x39(n=n)
+        x = np.ascontiguousarray(x * np.random.default_rng(n).uniform(0.5, 3.0, size=(n, 1)).astype(np.float32))
+        print(f"noalloc tanimoto n={n}", flush=True)
+        a = device_call("costal_knn_exact_device", x, 10, select="tanimoto")
+        b = device_call("costal_knn_exact_device_noalloc", x, 10, select="tanimoto")
+        for u, v in zip(a[:3], b[:3]):
+            np.testing.assert_array_equal(v, u)
+        np.testing.assert_allclose(b[3], a[3], rtol=1e-12, atol=0)
+        assert b[4].path == a[4].path == 0
 
 
 def test_noalloc_refuses_what_does_not_fit():

@@ -1871,13 +1871,19 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
         }
         return scan_pass(f, kpx, rows, nr, th, ns, tiles_per * P.TC, 0, outr, outl, outc, outs, pr);
       };
-      if ((rc = rescan(fn2, kp2, fb_rows, nfb, th_rescan, fb2_rows, fb2_lo, counters + 3, counters + 4, 32,
-                       RESCORE_CMAX)))
+      // Lists of kp2 are for rows with near-ties (396,460 x 15: with lists of 10 instead of 15, three
+      // rows went on to the row scan, 1.3 ms more).  They cost nothing where they fit the first
+      // pass's list capacity; otherwise, where short rows are most of the failures (learned
+      // thresholds), the first pass's lists serve (10^6 x 40, k = 30: 5,038 of 5,063 rows short,
+      // 10.6 ms with lists of 45, whose kernel runs one wave per SIMD).
+      const bool long_lists = list_template(kp2) == P.KP || 2 * nshort < nfb;
+      if ((rc = rescan(long_lists ? fn2 : fn, long_lists ? kp2 : P.kp, fb_rows, nfb, th_rescan, fb2_rows, fb2_lo,
+                       counters + 3, counters + 4, 32, RESCORE_CMAX)))
         return rc;
       if (chance) {
         // Rows the second chance left short are few, and the latency of their scan is what
-        // counts: a threshold-free rescan with up to 96 splits (and at most 1024 list entries per
-        // row, so that the rescoring of the few rows stays quick), not the row scan (which would
+        // counts: a threshold-free rescan with up to 96 splits (up to 2048 list entries per row:
+        // the rescoring takes the top k of many entries quickly), not the row scan (which would
         // stream every candidate of such a row through one workgroup).
         uint32_t c34[2] = {0, 0};
         HIP_TRY(hipMemcpyAsync(c34, counters + 3, sizeof c34, hipMemcpyDeviceToHost, s));
@@ -1887,7 +1893,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
                                                                     counters + 5, fb3_rows, counters + 6);
           HIP_TRY(hipGetLastError());
           if ((rc = rescan(fn, P.kp, fb3_rows, c34[1], nullptr, fb2_rows, fb2_lo, counters + 7, counters + 12,
-                           96, 1024)))
+                           96, RESCORE_CMAX)))
             return rc;
           third = true;
         }

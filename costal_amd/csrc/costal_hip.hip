@@ -5,16 +5,17 @@
 //   prep_kernel      row norms in f64 with a sequential sum (knn.rs:289-296, 315-323), the
 //                    L2-normalized f32 working copy (knn.rs:420-436) and its FP16 copy, and the
 //                    first non-finite row (knn.rs:325-342).
-//   coarse_kernel    the fused blocked coarse selection (knn.rs:603-660, 759-875): MFMA similarity
-//                    tiles of 32 candidates x 32 queries, staged through LDS, never written to HBM;
-//                    each query keeps its best K' = min(k+margin, n-1) candidates in LDS.
-//                    FP32 form: v_mfma_f32_32x32x2f32 on the f32 working copy.
-//                    FP16 form: v_mfma_f32_32x32x16_f16 on the FP16 copy; a candidate that clears
-//                    the list threshold minus the FP16 error bound is re-scored in f32 before it
-//                    is admitted, so the lists hold f32-accurate scores either way.
+//   order.hip        the locality order of the rows and each query block's candidate tiles in
+//                    decreasing bound order (the pruned scan; Tanimoto: norm ranges as well).
+//   coarse_kernel    (scan.h) the fused blocked coarse selection (knn.rs:603-660, 759-875): MFMA
+//                    similarity tiles of 32 candidates x 32 queries, staged through LDS, never
+//                    written to HBM; each lane keeps a sorted list of its best candidates in
+//                    registers (two lists per query), FP32, FP16 (re-scored in f32 on admission)
+//                    or split-FP16 tiles; cosine or Tanimoto selection.
 //   rescore_kernel   finalize_row (knn.rs:688-720): f64 rescoring over the original rows,
 //                    ranking by (score desc, index asc), the epsilon certificate; rows that fail it
-//                    are queued for the full-row path.  Fuses the Tanimoto weight (weight.rs:28-37).
+//                    are rescanned, then queued for the full-row path.  Fuses the Tanimoto weight
+//                    (weight.rs:28-37).
 //   rowscan_kernel   gather_row (knn.rs:723-743) and the brute force (knn.rs:884-924): one workgroup
 //                    streams every candidate of a row, keeps an exact running top-k in LDS.
 //   reweight_kernel  weight::reweight (weight.rs:41-60).

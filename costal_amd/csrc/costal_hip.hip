@@ -1691,12 +1691,10 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     HIP_TRY(hipEventRecord(D.evk[6], s));
     HIP_TRY(costal::order_rows(s, xs32, P.order, ob, norms, norm_lo, norm_hi));
     HIP_TRY(costal::order_permute(s, xs32, P.order, ob, P.n_pad, P.dp16, split16, xs32w, xs16));
-    // the bound sort leaves out the tiles below every first-pass threshold of a block
-    if (learned_early) HIP_TRY(hipStreamWaitEvent(s, D.ev_join, 0));
-    HIP_TRY(costal::order_bounds(s, xs32, P.order, ob, norms, learned_early ? th_ws : p.row_threshold));
+    HIP_TRY(costal::order_bounds(s, xs32, P.order, ob, norms));
     HIP_TRY(hipEventRecord(D.evk[7], s));
   }
-  if (learned_early && !P.prune) HIP_TRY(hipStreamWaitEvent(s, D.ev_join, 0));  // the scan reads them
+  if (learned_early) HIP_TRY(hipStreamWaitEvent(s, D.ev_join, 0));  // the scan reads them
   if (P.fused && P.tani) {
     tani_norms_kernel<<<P.n_pad / (P.TC / 2), 64, 0, s>>>(norms, P.prune ? ob.perm : nullptr, n, P.TC / 2, tani_scale,
                                                          tnorm, trange);

@@ -9,7 +9,8 @@
 // normalized mean of its rows and the largest angle from it to any of them.  For a query block Q
 // and a candidate tile T with cone angles aQ, aT and center angle th, no pair (q, c) has a cosine
 // above cos(max(0, th - aQ - aT)) (spherical triangle inequality).  The scan visits each block's
-// tiles in decreasing bound order and stops at the first bound below every query's admission
+// tiles in decreasing bound order (up to bound buckets, each tile recording its bucket's largest
+// bound) and stops at the first recorded bound below every query's admission
 // threshold: the candidates left out score below every list floor, so the certificate of the
 // rescoring (knn.rs:704-714) holds unchanged and the result is the brute force bit for bit.
 //
@@ -28,7 +29,8 @@ namespace costal {
 // Rows of a candidate tile of the bound matrix (a whole number of the scan's LDS tiles): 256, or
 // 512 where 256-row tiles would be more than the bound sort holds.
 constexpr uint32_t PRUNE_TILE = 256;
-// Largest tile count per query block the bound sort handles (bitonic sort in LDS).
+// Largest tile count per query block the bound order handles (four tiles per thread of one
+// 1024-thread workgroup).
 constexpr uint32_t PRUNE_MAX_TILES = 4096;
 // Rows of a rescan (rows that failed the first certificate) that get their own cones and bounds.
 constexpr uint32_t RESCAN_PRUNE_ROWS = 8192;
@@ -63,7 +65,7 @@ struct OrderBufs {
   double* tang;      // candidate tiles x 3
   double* qnr;       // Tanimoto: query blocks x 2: smallest and largest raw norm
   double* tnr;       // Tanimoto: candidate tiles x 2
-  uint32_t* seq;     // query blocks x tiles: tile ids in decreasing bound order
+  uint32_t* seq;     // query blocks x tiles: tile ids in decreasing (bucketed) bound order
   float* sb;         // query blocks x tiles: the bounds in that order
   // the same for the blocks of a rescan's row list (RESCAN_PRUNE_ROWS / qb blocks at most)
   double* rcen;
@@ -115,11 +117,9 @@ hipError_t order_rows(hipStream_t s, const float* xs32, const OrderShape& o, Ord
 hipError_t order_permute(hipStream_t s, const float* xs32, const OrderShape& o, const OrderBufs& b, uint32_t n_pad,
                          uint32_t dp16, int split16, float* xs32w, _Float16* xs16w);
 // Cones (and for Tanimoto norm ranges, from norms) of the query blocks and candidate tiles, the
-// bound matrix and its per-block order.  th0: the first pass's admission thresholds by original
-// row (device memory; null: none): a block's tiles whose bound is below all of its rows'
-// thresholds, which its scan never reaches, are left unsorted after the others.
+// bound matrix and its per-block order (by bound buckets; see bound_sort_kernel).
 hipError_t order_bounds(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b,
-                        const double* norms = nullptr, const float* th0 = nullptr);
+                        const double* norms = nullptr);
 // A rescan's row list (original rows, nr <= RESCAN_PRUNE_ROWS): sorted in place into working
 // order (so that its blocks of qb rows are groups of similar rows), their cones, and every block's
 // tiles in bound order in rseq / rsb.  Needs order_bounds first (the tile cones).

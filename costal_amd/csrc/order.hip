@@ -440,53 +440,47 @@ __device__ __forceinline__ double tani_bound(double c, double amin, double amax,
   return cc / (smin - cc);
 }
 
-// Bound of every candidate tile for query block bid_x(), sorted in decreasing order (ties: lower
-// tile first): seq[b*npt + r] = tile of rank r, sb[b*npt + r] = its bound, an f32 rounded up.
-// 1024 threads; a value is (bound key << 32) | ~tile, so that one comparison orders both.  With
-// norm ranges (qnr, tnr), the bound is on Tanimoto scores (tani_bound of the cosine bound).
-// With the first pass's admission thresholds (th0 by original row, the block's rows perm[b*qb ..]),
-// only the tiles whose bound reaches the block's lowest threshold are sorted; the others, which
-// the scan can never reach (its stop test compares the bound with lists' thresholds, which start
-// at these and only rise), follow them in tile order.
+// Tiles in decreasing bound order, up to buckets: every candidate tile's bound for query block
+// bid_x() (an f32 rounded up) goes to one of NBUCKET buckets of the distance 1 + 2^-18 - bound, 16
+// per octave (the f32 exponent and top four mantissa bits of that distance, which grow with it),
+// and the tiles are placed by a counting pass, bucket by bucket from the highest bounds, in tile
+// order within a bucket (ranks by wave ballots: deterministic).  Every tile's recorded bound is
+// the largest bound in its bucket, so the recorded bounds never increase along the order and each
+// is an upper bound of its tile's and every later tile's: the scan's stop at the first recorded
+// bound below every list's threshold skips only tiles that cannot qualify (a few tiles of the
+// bucket holding a threshold are scanned without need).  1024 threads; a sorting network over
+// the same tiles took 0.25 ms on 396,460 rows.  With norm ranges (qnr, tnr), the bounds are on
+// Tanimoto scores (tani_bound of the cosine bound).
+constexpr uint32_t NBUCKET = 512;
+constexpr uint32_t BUCKET_BITS = 9;
+
+__device__ __forceinline__ uint32_t bound_bucket(float key) {
+  const float u = (float)(1.0 + 0x1p-18 - (double)key);  // 2.6e-6 .. 2.0001 for bounds in [-1, 1 + 1e-6]
+  const int q = (int)(__float_as_uint(u) >> 19) - (108 << 4);
+  return (uint32_t)(q < 0 ? 0 : q >= (int)NBUCKET ? (int)NBUCKET - 1 : q);
+}
+
 __global__ __launch_bounds__(1024) void bound_sort_kernel(const double* __restrict__ qcen, const double* __restrict__ qang,
                                                           const double* __restrict__ tcen, const double* __restrict__ tang,
-                                                          uint32_t dp, uint32_t npt, uint32_t p2, uint32_t* __restrict__ seq,
+                                                          uint32_t dp, uint32_t npt, uint32_t* __restrict__ seq,
                                                           float* __restrict__ sb, const double* __restrict__ qnr,
-                                                          const double* __restrict__ tnr, const float* __restrict__ th0,
-                                                          const uint32_t* __restrict__ perm, uint32_t qb, uint32_t n) {
+                                                          const double* __restrict__ tnr) {
   constexpr int EMAX = PRUNE_MAX_TILES / 1024;
-  __shared__ uint64_t s_buf[PRUNE_MAX_TILES];
   __shared__ double s_q[64];
-  __shared__ float s_cut[16];
-  __shared__ uint32_t s_wq[16];
+  __shared__ uint32_t s_cnt[NBUCKET], s_max[NBUCKET], s_run[NBUCKET];
+  __shared__ uint32_t s_wc[16][NBUCKET];  // per slot: tiles of each bucket in each wave
   const uint32_t b = bid_x(), t0 = tid_x(), wv = t0 >> 6, lane = t0 & 63;
   if (t0 < 64) s_q[t0] = t0 < dp ? qcen[(size_t)b * dp + t0] : 0.0;
-  // the block's lowest threshold (a NaN threshold admits everything: -inf); -inf without any
-  float cut = -INFINITY;
-  if (th0) {
-    float m = INFINITY;
-    for (uint32_t r = t0; r < qb; r += 1024) {
-      const uint32_t w = b * qb + r;
-      if (w < n) {
-        const float t = th0[perm[w]];
-        m = fminf(m, (__float_as_uint(t) & 0x7fffffffu) > 0x7f800000u ? -INFINITY : t);
-      }
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fminf(m, __shfl_xor(m, off));
-    if (lane == 0) s_cut[wv] = m;
-  }
+  for (uint32_t e = t0; e < NBUCKET; e += 1024) s_cnt[e] = 0, s_max[e] = 0, s_run[e] = 0;
+  for (uint32_t e = t0; e < 16 * NBUCKET; e += 1024) (&s_wc[0][0])[e] = 0;
   __syncthreads();
-  if (th0) {
-    cut = s_cut[0];
-    for (uint32_t v = 1; v < 16; ++v) cut = fminf(cut, s_cut[v]);
-  }
   const double aq = qang[3 * b], cq = qang[3 * b + 1], sq = qang[3 * b + 2];
-  const int E = (int)(p2 > 1024 ? p2 / 1024 : 1);
-  uint64_t v[EMAX];
+  const int E = (int)((npt + 1023) / 1024);
+  uint32_t key[EMAX], bk[EMAX];
 #pragma unroll
   for (int s = 0; s < EMAX; ++s) {
-    v[s] = 0;  // padding: below every real value, sorted last
+    key[s] = 0;
+    bk[s] = NBUCKET;  // no tile
     const uint32_t t = (uint32_t)s * 1024 + t0;
     if (s < E && t < npt) {
       const double* tc = tcen + (size_t)t * dp;
@@ -504,78 +498,65 @@ __global__ __launch_bounds__(1024) void bound_sort_kernel(const double* __restri
       }
       if (qnr) bound = tani_bound(bound, qnr[2 * b], qnr[2 * b + 1], tnr[2 * t], tnr[2 * t + 1]);
       bound += BOUND_MARGIN;
-      float key = (float)bound;
-      if ((double)key < bound) key = nextafterf(key, INFINITY);
-      v[s] = ((uint64_t)float_key(key) << 32) | (uint32_t)~t;
+      float kf = (float)bound;
+      if ((double)kf < bound) kf = nextafterf(kf, INFINITY);
+      key[s] = float_key(kf);
+      bk[s] = bound_bucket(kf);
+      atomicAdd(&s_cnt[bk[s]], 1u);
+      atomicMax(&s_max[bk[s]], key[s]);
     }
-  }
-  if (!th0) {
-    // no thresholds: every tile sorted, straight from the registers
-    block_sort_desc<EMAX>(v, E, s_buf);
-#pragma unroll
-    for (int s = 0; s < EMAX; ++s) {
-      const uint32_t r = (uint32_t)s * 1024 + t0;
-      if (s < E && r < npt) {
-        seq[(size_t)b * npt + r] = ~(uint32_t)v[s];
-        sb[(size_t)b * npt + r] = key_float((uint32_t)(v[s] >> 32));
-      }
-    }
-    return;
-  }
-  // Compaction in tile order: the tiles whose bound reaches the cut to s_buf[0 .. count), the
-  // others after them in the output (ranks by wave ballots and per-wave counts; a tile's rank
-  // among the others is its index less the qualifying tiles before it).
-  constexpr uint32_t NO_RANK = 0xffffffffu;
-  uint32_t base = 0;
-  uint32_t orank[EMAX];
-#pragma unroll
-  for (int s = 0; s < EMAX; ++s) {
-    orank[s] = NO_RANK;
-    if (s < E) {
-      const uint32_t t = (uint32_t)s * 1024 + t0;
-      const bool real = t < npt;
-      const bool q = real && key_float((uint32_t)(v[s] >> 32)) >= cut;
-      const uint64_t bal = __builtin_amdgcn_ballot_w64(q);
-      const uint32_t below = (uint32_t)__builtin_popcountll(bal & (lane ? (~0ull >> (64 - lane)) : 0ull));
-      if (lane == 0) s_wq[wv] = (uint32_t)__builtin_popcountll(bal);
-      __syncthreads();
-      uint32_t before = base, total = base;
-      for (uint32_t w = 0; w < 16; ++w) {
-        before += w < wv ? s_wq[w] : 0u;
-        total += s_wq[w];
-      }
-      if (q) s_buf[before + below] = v[s];
-      else if (real) orank[s] = t - (before + below);
-      __syncthreads();
-      base = total;
-    }
-  }
-  const uint32_t count = base;
-#pragma unroll
-  for (int s = 0; s < EMAX; ++s) {
-    if (orank[s] != NO_RANK) {
-      const uint32_t r = count + orank[s];
-      seq[(size_t)b * npt + r] = ~(uint32_t)v[s];
-      sb[(size_t)b * npt + r] = key_float((uint32_t)(v[s] >> 32));
-    }
-  }
-  // sort the qualifying ones (the network over the next power of two of them, 1024 at least)
-  int E2 = 1;
-  while ((uint32_t)E2 * 1024 < count) E2 <<= 1;
-#pragma unroll
-  for (int s = 0; s < EMAX; ++s) {
-    const uint32_t r = (uint32_t)s * 1024 + t0;
-    v[s] = (s < E2 && r < count) ? s_buf[r] : 0ull;
   }
   __syncthreads();
-  block_sort_desc<EMAX>(v, E2, s_buf);
+  // first place of every bucket: exclusive prefix sum of the counts (threads < NBUCKET, one each)
+  {
+    const uint32_t c = t0 < NBUCKET ? s_cnt[t0] : 0u;
+    uint32_t x = c;
 #pragma unroll
-  for (int s = 0; s < EMAX; ++s) {
-    const uint32_t r = (uint32_t)s * 1024 + t0;
-    if (s < E2 && r < count) {
-      seq[(size_t)b * npt + r] = ~(uint32_t)v[s];
-      sb[(size_t)b * npt + r] = key_float((uint32_t)(v[s] >> 32));
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+      if ((int)lane >= off) x += y;
     }
+    if (lane == 63 && t0 < NBUCKET) s_wc[0][wv] = x;  // wave totals (s_wc is clear, reused here)
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wv && w < NBUCKET / 64; ++w) before += s_wc[0][w];
+    __syncthreads();
+    if (t0 < NBUCKET) s_run[t0] = before + x - c;
+    if (t0 < NBUCKET / 64) s_wc[0][t0] = 0;
+    __syncthreads();
+  }
+  // the tiles of each slot in turn: a tile's place = its bucket's next free place + the tiles of
+  // its bucket in earlier waves of the slot + those in earlier lanes of its wave
+  for (int s = 0; s < E; ++s) {
+    uint32_t bkt = 0;
+#pragma unroll
+    for (int z = 0; z < EMAX; ++z)
+      if (z == s) bkt = bk[z];
+    const bool real = bkt < NBUCKET;
+    uint64_t same = __builtin_amdgcn_ballot_w64(real);
+#pragma unroll
+    for (uint32_t bit = 0; bit < BUCKET_BITS; ++bit) {
+      const bool on = (bkt >> bit) & 1;
+      const uint64_t bal = __builtin_amdgcn_ballot_w64(on);
+      same &= on ? bal : ~bal;
+    }
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const uint32_t r_in = (uint32_t)__builtin_popcountll(same & below);
+    if (real && r_in == 0) s_wc[wv][bkt] = (uint32_t)__builtin_popcountll(same);
+    __syncthreads();
+    if (real) {
+      uint32_t pos = s_run[bkt] + r_in;
+      for (uint32_t w = 0; w < wv; ++w) pos += s_wc[w][bkt];
+      seq[(size_t)b * npt + pos] = (uint32_t)s * 1024 + t0;
+      sb[(size_t)b * npt + pos] = key_float(s_max[bkt]);
+    }
+    __syncthreads();
+    for (uint32_t e = t0; e < NBUCKET; e += 1024) {
+      uint32_t add = 0;
+      for (uint32_t w = 0; w < 16; ++w) add += s_wc[w][e], s_wc[w][e] = 0;
+      s_run[e] += add;
+    }
+    __syncthreads();
   }
 }
 
@@ -998,17 +979,14 @@ hipError_t order_permute(hipStream_t s, const float* xs32, const OrderShape& o, 
   return hipGetLastError();
 }
 
-hipError_t order_bounds(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b, const double* norms,
-                        const float* th0) {
+hipError_t order_bounds(hipStream_t s, const float* xs32, const OrderShape& o, OrderBufs& b, const double* norms) {
   const double* nv = o.tani ? norms : nullptr;
   return by_dpm(o.dp, [&](auto dpm) -> hipError_t {
     constexpr int DPM = decltype(dpm)::value;
     cone_kernel<DPM><<<o.nblk, 64, 0, s>>>(xs32, o.dp, b.perm, o.n, o.qb, b.qcen, b.qang, nv, b.qnr);
     cone_kernel<DPM><<<o.npt, 64, 0, s>>>(xs32, o.dp, b.perm, o.n, o.pt, b.tcen, b.tang, nv, b.tnr);
-    uint32_t p2 = 1;
-    while (p2 < o.npt) p2 <<= 1;
-    bound_sort_kernel<<<o.nblk, 1024, 0, s>>>(b.qcen, b.qang, b.tcen, b.tang, o.dp, o.npt, p2, b.seq, b.sb,
-                                              nv ? b.qnr : nullptr, nv ? b.tnr : nullptr, th0, b.perm, o.qb, o.n);
+    bound_sort_kernel<<<o.nblk, 1024, 0, s>>>(b.qcen, b.qang, b.tcen, b.tang, o.dp, o.npt, b.seq, b.sb,
+                                              nv ? b.qnr : nullptr, nv ? b.tnr : nullptr);
     return hipGetLastError();
   });
 }
@@ -1024,10 +1002,8 @@ hipError_t order_rescan(hipStream_t s, const float* xs32, const OrderShape& o, O
   return by_dpm(o.dp, [&](auto dpm) -> hipError_t {
     constexpr int DPM = decltype(dpm)::value;
     cone_kernel<DPM><<<nblk, 64, 0, s>>>(xs32, o.dp, rows, nr, o.qb, b.rcen, b.rang, nv, b.rnr);
-    uint32_t q2 = 1;
-    while (q2 < o.npt) q2 <<= 1;
-    bound_sort_kernel<<<nblk, 1024, 0, s>>>(b.rcen, b.rang, b.tcen, b.tang, o.dp, o.npt, q2, b.rseq, b.rsb,
-                                            nv ? b.rnr : nullptr, nv ? b.tnr : nullptr, nullptr, nullptr, o.qb, nr);
+    bound_sort_kernel<<<nblk, 1024, 0, s>>>(b.rcen, b.rang, b.tcen, b.tang, o.dp, o.npt, b.rseq, b.rsb,
+                                            nv ? b.rnr : nullptr, nv ? b.tnr : nullptr);
     return hipGetLastError();
   });
 }

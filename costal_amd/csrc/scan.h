@@ -40,7 +40,8 @@ struct ScanArgs {
   const uint32_t* perm;          // working row -> original row (null: the identity)
   const uint32_t* iperm;         // original row -> working row (for a row list)
   const uint32_t* seq;           // query block b: tiles seq[b*npt ..] in decreasing bound order
-  const float* sb;               // their bounds (upper bounds of every query-candidate score)
+  const float* sb;               // their recorded bounds (non-increasing; each bounds every query-
+                                 // candidate score of its tile and of the tiles after it)
   uint32_t npt;                  // candidate tiles
   uint32_t pt;                   // rows of a candidate tile (a multiple of the LDS tile)
   unsigned long long* scanned;   // candidate tiles scanned (summed over blocks and splits)

@@ -109,6 +109,23 @@ def test_zero_rows_duplicates_and_integer_ties(matmul, oracle):
 
 
 @pytest.mark.parametrize("matmul", ["mfma-f32", "mfma-f16x3"])
+def test_signed_centered_mixture(matmul, oracle):
+    """Signed, centered data: clusters in opposite directions give tile pairs with negative cosine
+    bounds (the bound order's buckets run from 1 down to -1)."""
+    rng = np.random.default_rng(17)
+    centers = rng.normal(0.0, 1.0, size=(12, 16))
+    lab = rng.integers(0, 12, size=16_000)
+    x = np.ascontiguousarray(np.concatenate([centers, -centers])[np.concatenate([lab[:8000], lab[8000:] + 12])]
+                             + rng.normal(0.0, 0.3, size=(16_000, 16)), dtype=np.float32)
+    x -= x.mean(axis=0, dtype=np.float64).astype(np.float32)
+    want = oracle.knn_bruteforce(x, 10)
+    got, st = ext._knn_exact(x, 10, matmul=matmul)
+    print(f"  skipped {st['blocks_skipped']} of {st['blocks_total']}", flush=True)
+    assert_same(got, want, f"signed {matmul}")
+    assert st["blocks_skipped"] > 0
+
+
+@pytest.mark.parametrize("matmul", ["mfma-f32", "mfma-f16x3"])
 def test_pruned_scan_with_admission_thresholds(matmul, cases):
     """Thresholds start the pruning before any list is full: good ones (just below each row's k-th
     cosine), and bad ones (above it on a third of the rows: those rows rescan)."""

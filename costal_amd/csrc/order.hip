@@ -26,6 +26,19 @@ constexpr double CONE_MARGIN = 1e-6;
 // Added to every bound: the f64 evaluation error of the bound and of the rescored cosine.
 constexpr double BOUND_MARGIN = 1e-6;
 
+__device__ __forceinline__ double wave_sum(double x) {
+  // fixed butterfly: the same order on every run
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  return x;
+}
+
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x = fmin(x, __shfl_xor(x, off));
+  return x;
+}
+
 // One row of dp floats into DPM registers (zero beyond dp).
 template <int DPM>
 __device__ __forceinline__ void load_row(const float* __restrict__ row, uint32_t dp, float (&v)[DPM]) {
@@ -124,21 +137,18 @@ __global__ void km_clear_kernel(long long* __restrict__ sums, uint32_t* __restri
 
 // New centers: the normalized means of their rows (a center with no rows keeps its place); the
 // sums and counts are cleared for the next pass.
-__global__ void km_update_kernel(uint32_t dp, uint32_t C, long long* __restrict__ sums, uint32_t* __restrict__ counts,
-                                 float* __restrict__ centers) {
-  const uint32_t c = bid_x() * bdim_x() + tid_x();
-  if (c >= C) return;
+// One wave per center, a lane per feature (dp <= 64): the new center is the normalized sum (kept
+// when the center drew no row), and the sums and count are cleared for the next pass.  (A thread
+// per center walking its features took 7-12 us per pass.)
+__global__ __launch_bounds__(64) void km_update_kernel(uint32_t dp, long long* __restrict__ sums,
+                                                       uint32_t* __restrict__ counts, float* __restrict__ centers) {
+  const uint32_t c = bid_x(), f = tid_x();
   long long* s = sums + (size_t)c * dp;
-  if (counts[c] > 0) {
-    double nn = 0.0;
-    for (uint32_t f = 0; f < dp; ++f) nn += (double)s[f] * (double)s[f];
-    if (nn > 0.0) {
-      const double inv = 1.0 / sqrt(nn);
-      for (uint32_t f = 0; f < dp; ++f) centers[(size_t)c * dp + f] = (float)((double)s[f] * inv);
-    }
-  }
-  for (uint32_t f = 0; f < dp; ++f) s[f] = 0;
-  counts[c] = 0;
+  const double v = f < dp ? (double)s[f] : 0.0;
+  const double nn = wave_sum(v * v);
+  if (counts[c] > 0 && nn > 0.0 && f < dp) centers[(size_t)c * dp + f] = (float)(v * (1.0 / sqrt(nn)));
+  if (f < dp) s[f] = 0;
+  if (f == 0) counts[c] = 0;
 }
 
 // Every row's center (zero rows: center C, sorted last), its rank among the rows of its 256-row
@@ -244,19 +254,6 @@ __global__ void permute_kernel(const float* __restrict__ xs32, const uint32_t* _
       if (split16) *reinterpret_cast<f16x8*>(xs16w + (size_t)w * s16 + dp16 + t) = lo;
     }
   }
-}
-
-__device__ __forceinline__ double wave_sum(double x) {
-  // fixed butterfly: the same order on every run
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-  return x;
-}
-
-__device__ __forceinline__ double wave_min(double x) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) x = fmin(x, __shfl_xor(x, off));
-  return x;
 }
 
 // Cone of positions [g*G, min(n, g*G + G)) of a row list (rows[w]: the original row at position w,
@@ -927,7 +924,7 @@ hipError_t order_rows(hipStream_t s, const float* xs32, const OrderShape& o, Ord
     for (uint32_t it = 0; it < o.iters; ++it) {
       km_step_kernel<DPM><<<(o.sample + 255) / 256, 256, lds_step, s>>>(xs32, o.dp, o.C, o.sample, stride, b.centers,
                                                                           b.sums, b.counts);
-      km_update_kernel<<<(o.C + 63) / 64, 64, 0, s>>>(o.dp, o.C, b.sums, b.counts, b.centers);
+      km_update_kernel<<<o.C, 64, 0, s>>>(o.dp, b.sums, b.counts, b.centers);
     }
     const size_t lds_assign = (size_t)o.C * DPM * 4 + (o.C + 1) * 4;
     km_assign_kernel<DPM><<<nb, 256, lds_assign, s>>>(xs32, o.dp, o.n, o.C, b.centers, b.assign, b.rank, b.hist,

@@ -1807,12 +1807,12 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
       auto rescan = [&](CoarseFn f, uint32_t kpx, uint32_t* rows, uint32_t nr, const float* th, uint32_t* outr,
                         double* outl, uint32_t* outc, uint32_t* outs, uint32_t max_splits,
                         uint32_t max_entries) -> int {
-        // about one workgroup per CU: the rows are few, so splits are what fills the GPU, but
-        // every split fills its lists from scratch, so more than one per CU was slower (three per
-        // CU in the locality order: 24 M -> 37 M inserts on 396,460 x 15, 1.3 -> 1.4 ms)
+        // about two workgroups per CU: the rows are few, so splits are what fills the GPU, but
+        // every split fills its lists from scratch (10^6 x 40, k = 30, learned: one per CU 8.3 ms
+        // for the fallback, two 7.1 ms; 396,460 x 15: 1.00 -> 0.95 ms)
         const bool pruned = P.prune && nr <= costal::RESCAN_PRUNE_ROWS;
         const uint32_t blocks = (nr + P.QB - 1) / P.QB;
-        uint32_t ns = ((uint32_t)std::max(D.cus, 1) + blocks - 1) / blocks;
+        uint32_t ns = (2u * (uint32_t)std::max(D.cus, 1) + blocks - 1) / blocks;
         ns = std::min<uint32_t>(ns, max_splits);
         ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, ntile / 2));
         ns = std::min<uint32_t>(ns, std::max<uint32_t>(1, max_entries / (2 * kpx)));

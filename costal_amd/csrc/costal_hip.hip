@@ -1296,7 +1296,9 @@ inline uint32_t round_up(uint32_t v, uint32_t m) { return (v + m - 1) / m * m; }
 // The fused path handles d <= 64 (FP32) / 128 (FP16) and K' <= 64; everything else runs the
 // exact row scan.
 constexpr uint32_t FUSED_KP_MAX = 64;
-constexpr uint32_t RESCORE_CMAX = 2048;  // list entries one row's rescoring holds (largest rescore_kernel)
+constexpr uint32_t RESCORE_CMAX = 2048;  // list entries one row's rescoring holds (passes over many rows)
+// ... and for the few rows of the last rescan (48 KB of LDS per row: a rescoring workgroup each)
+constexpr uint32_t RESCORE_CMAX_FEW = 4096;
 constexpr uint32_t ROWSCAN_K_MAX = 2048;
 
 struct Plan {
@@ -1726,7 +1728,8 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     else if (slots <= 256) RESCORE(256);
     else if (slots <= 512) RESCORE(512);
     else if (slots <= 1024) RESCORE(1024);
-    else RESCORE(2048);
+    else if (slots <= 2048) RESCORE(2048);
+    else RESCORE(4096);
 #undef RESCORE
     HIP_TRY(hipGetLastError());
     return 0;
@@ -1847,7 +1850,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
         return rc;
       if (chance) {
         // Rows the second chance left short are few, and the latency of their scan is what
-        // counts: a threshold-free rescan with up to 96 splits (up to 2048 list entries per row:
+        // counts: a threshold-free rescan with up to 96 splits (up to 4096 list entries per row:
         // the rescoring takes the top k of many entries quickly), not the row scan (which would
         // stream every candidate of such a row through one workgroup).
         uint32_t c34[2] = {0, 0};
@@ -1858,7 +1861,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
                                                                     counters + 5, fb3_rows, counters + 6);
           HIP_TRY(hipGetLastError());
           if ((rc = rescan(fn, P.kp, fb3_rows, c34[1], nullptr, fb2_rows, fb2_lo, counters + 7, counters + 12,
-                           96, RESCORE_CMAX)))
+                           96, RESCORE_CMAX_FEW)))
             return rc;
           third = true;
         }

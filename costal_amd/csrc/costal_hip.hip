@@ -1962,8 +1962,13 @@ hipStream_t pick_stream(Device& D, void* stream) { return stream ? (hipStream_t)
 
 // the shortest decimal that reads back as x (NaN, inf and -inf spelled as Rust prints them)
 std::string shortest(double x) {
-  if (std::isnan(x)) return "NaN";
-  if (std::isinf(x)) return x > 0 ? "inf" : "-inf";
+  // NaN and inf by their bits (the device pass of this file is built without NaN semantics)
+  uint64_t bits;
+  memcpy(&bits, &x, sizeof bits);
+  if ((bits & 0x7ff0000000000000ull) == 0x7ff0000000000000ull) {
+    if (bits & 0x000fffffffffffffull) return "NaN";
+    return (bits >> 63) ? "-inf" : "inf";
+  }
   char buf[64];
   for (int p = 1; p <= 17; ++p) {
     snprintf(buf, sizeof buf, "%.*g", p, x);

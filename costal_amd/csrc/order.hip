@@ -473,10 +473,10 @@ __global__ __launch_bounds__(1024) void bound_sort_kernel(const double* __restri
   __syncthreads();
   const double aq = qang[3 * b], cq = qang[3 * b + 1], sq = qang[3 * b + 2];
   const int E = (int)((npt + 1023) / 1024);
-  uint32_t key[EMAX], bk[EMAX];
-#pragma unroll
-  for (int s = 0; s < EMAX; ++s) {
-    key[s] = 0;
+  uint32_t bk[EMAX];
+  // the bucket of tile s * 1024 + t0 (a lambda applied to each slot by hand: a pragma-unrolled loop
+  // around the feature loop is not unrolled by the compiler)
+  auto place = [&](int s) __attribute__((always_inline)) {
     bk[s] = NBUCKET;  // no tile
     const uint32_t t = (uint32_t)s * 1024 + t0;
     if (s < E && t < npt) {
@@ -497,12 +497,16 @@ __global__ __launch_bounds__(1024) void bound_sort_kernel(const double* __restri
       bound += BOUND_MARGIN;
       float kf = (float)bound;
       if ((double)kf < bound) kf = nextafterf(kf, INFINITY);
-      key[s] = float_key(kf);
       bk[s] = bound_bucket(kf);
       atomicAdd(&s_cnt[bk[s]], 1u);
-      atomicMax(&s_max[bk[s]], key[s]);
+      atomicMax(&s_max[bk[s]], float_key(kf));
     }
-  }
+  };
+  place(0);
+  if constexpr (EMAX > 1) place(1);
+  if constexpr (EMAX > 2) place(2);
+  if constexpr (EMAX > 3) place(3);
+  static_assert(EMAX <= 4, "four tiles per thread at most");
   __syncthreads();
   // first place of every bucket: exclusive prefix sum of the counts (threads < NBUCKET, one each)
   {

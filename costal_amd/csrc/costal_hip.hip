@@ -621,8 +621,12 @@ __device__ void rescore_half(double* __restrict__ s_sc, uint32_t* __restrict__ s
   }
 }
 
+// The rows of a row list, or every row: in the locality order when perm is given (consecutive
+// workgroups then rescore similar rows, whose candidates overlap, so their rows stay in L2: 10^6 x
+// 40, k = 30: 3.54 -> 2.78 ms), lists by row.
 __global__ __launch_bounds__(64) void rescore_pair_kernel(
-    const float* __restrict__ x, uint32_t d, const double* __restrict__ norms, uint32_t k, int kp,
+    const float* __restrict__ x, const uint32_t* __restrict__ perm, uint32_t d, const double* __restrict__ norms,
+    uint32_t k, int kp,
     uint32_t nsplit, const uint32_t* __restrict__ rowlist, uint32_t nq, const uint32_t* __restrict__ cand_idx,
     const float* __restrict__ cand_sc, const uint32_t* __restrict__ cand_cnt, const float* __restrict__ cand_floor,
     int all_admitted, double eps_list, double eps32, uint32_t select, uint32_t* __restrict__ idx,
@@ -634,15 +638,17 @@ __global__ __launch_bounds__(64) void rescore_pair_kernel(
   const uint32_t half = tid_x() >> 5, lane = tid_x() & 31;
   const uint32_t qq = bid_x() * 2 + half;
   const bool valid = qq < nq;
-  const uint32_t i = valid ? (rowlist ? rowlist[qq] : qq) : 0;
-  rescore_half(s_sc[half], s_id[half], &s_fill[half], valid, lane, x, d, norms, k, kp, nsplit, qq, i, cand_idx,
+  const uint32_t i = valid ? (rowlist ? rowlist[qq] : perm ? perm[qq] : qq) : 0;
+  rescore_half(s_sc[half], s_id[half], &s_fill[half], valid, lane, x, d, norms, k, kp, nsplit, rowlist ? qq : i, i,
+               cand_idx,
                cand_sc, cand_cnt, cand_floor, all_admitted, eps_list, eps32, select, idx, cos_out, wts, i, fb_rows,
                fb_lo, fb_count, short_count, err_max);
 }
 
 template <int CMAX>
 __global__ __launch_bounds__(64) void rescore_kernel(
-    const float* __restrict__ x, uint32_t d, const double* __restrict__ norms, uint32_t n,
+    const float* __restrict__ x, const uint32_t* __restrict__ perm, uint32_t d, const double* __restrict__ norms,
+    uint32_t n,
     uint32_t k, int kp, uint32_t nsplit, const uint32_t* __restrict__ rowlist, uint32_t nq,
     const uint32_t* __restrict__ cand_idx, const float* __restrict__ cand_sc, const uint32_t* __restrict__ cand_cnt,
     const float* __restrict__ cand_floor, int all_admitted, double eps_list, double eps32, uint32_t select,
@@ -652,8 +658,9 @@ __global__ __launch_bounds__(64) void rescore_kernel(
   __shared__ double s_sc[CMAX];
   __shared__ uint32_t s_id[CMAX];
   for (uint32_t qq = bid_x(); qq < nq; qq += gdim_x()) {
-    const uint32_t i = rowlist ? rowlist[qq] : qq;
-    rescore_row<CMAX>(s_sc, s_id, x, d, norms, k, kp, nsplit, qq, i, cand_idx, cand_sc, cand_cnt, cand_floor,
+    const uint32_t i = rowlist ? rowlist[qq] : perm ? perm[qq] : qq;
+    rescore_row<CMAX>(s_sc, s_id, x, d, norms, k, kp, nsplit, rowlist ? qq : i, i, cand_idx, cand_sc, cand_cnt,
+                      cand_floor,
                       all_admitted, eps_list, eps32, select, idx, cos_out, wts, i, fb_rows, fb_lo, fb_count,
                       short_count, err_max);
     __syncthreads();  // the LDS arrays are refilled by the next row
@@ -1716,11 +1723,13 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     const uint32_t nlists = 2 * nsplit;
     const uint32_t slots = nlists * kp;
 #define RESCORE(CM)                                                                                    \
-  rescore_kernel<CM><<<nq, 64, 0, s>>>(x, d, norms, n, k, (int)kp, nlists, rows, nq, cand_idx, cand_sc,       \
+  rescore_kernel<CM><<<nq, 64, 0, s>>>(x, P.prune ? ob.perm : nullptr, d, norms, n, k, (int)kp, nlists, rows, nq, \
+                                       cand_idx, cand_sc,                                                     \
                                        cand_cnt, cand_floor, all_admitted, eps_list, eps_row, sel, idx, cos,  \
                                        wts, fbr, fbl, fbc, shc, err_max)
     if (slots <= 32) {
-      rescore_pair_kernel<<<(nq + 1) / 2, 64, 0, s>>>(x, d, norms, k, (int)kp, nlists, rows, nq, cand_idx, cand_sc,
+      rescore_pair_kernel<<<(nq + 1) / 2, 64, 0, s>>>(x, P.prune ? ob.perm : nullptr, d, norms, k, (int)kp, nlists, rows,
+                                                      nq, cand_idx, cand_sc,
                                                       cand_cnt, cand_floor, all_admitted, eps_list, eps_row, sel, idx,
                                                       cos, wts, fbr, fbl, fbc, shc, err_max);
     } else if (slots <= 64) RESCORE(64);

@@ -46,6 +46,7 @@
 #include <functional>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/costal_hip.h"
@@ -72,6 +73,33 @@ __device__ __forceinline__ double dot64(const float* __restrict__ a, const float
 #pragma unroll 8
   for (uint32_t t = 0; t < d; ++t) s += (double)a[t] * (double)b[t];
   return s;
+}
+
+// dot64 over rows of rs floats.  V4: rs is a multiple of 4, the rows are 16-byte aligned and hold
+// zeros past their features; they are read 16 bytes at a time (a gathered row then costs one
+// vector load per 4 features instead of one per feature, and never straddles a cache line more
+// than the row's own length needs).  The sum starts at +0.0 and so is never -0.0 (x + -x rounds
+// to +0.0), and the padding adds +0.0 products, which leave it unchanged: bit-identical to dot64.
+// Two 16-byte words of each row in flight: the rescoring kernels then keep to 64 VGPRs (eight
+// waves per SIMD; four words took 95 and made 10^6 x 40 rescoring 5% slower, one word 34%).
+template <bool V4>
+__device__ __forceinline__ double dot64r(const float* __restrict__ a, const float* __restrict__ b, uint32_t rs) {
+  if constexpr (V4) {
+    const f32x4* __restrict__ a4 = reinterpret_cast<const f32x4*>(a);
+    const f32x4* __restrict__ b4 = reinterpret_cast<const f32x4*>(b);
+    double s = 0.0;
+#pragma unroll 2
+    for (uint32_t c = 0; c < rs / 4; ++c) {
+      const f32x4 u = a4[c], v = b4[c];
+      s += (double)u[0] * (double)v[0];
+      s += (double)u[1] * (double)v[1];
+      s += (double)u[2] * (double)v[2];
+      s += (double)u[3] * (double)v[3];
+    }
+    return s;
+  } else {
+    return dot64(a, b, rs);
+  }
 }
 
 // knn.rs:300-312
@@ -110,8 +138,9 @@ __device__ __forceinline__ bool ranks_before(double sa, uint32_t ia, double sb, 
 // 64-feature chunks (the next chunk in flight while the current one is summed), and thread r sums
 // row r's squares in feature order (f64, index order: bit-identical norms), flagging the first
 // row with a NaN or inf (the test reads the bits as integers); the normalized copies are written
-// straight from global, coalesced: the f32 copy f32(f64(x)/norm), and the FP16 copy, or the split
-// one (hi = fp16(v), lo = fp16(v - hi), v - hi exact in f32; hi then lo halves per row).  Rows in
+// straight from global, coalesced: the f32 copy f32(f64(x)/norm), the FP16 copy, or the split
+// one (hi = fp16(v), lo = fp16(v - hi), v - hi exact in f32; hi then lo halves per row), and when
+// xr is given the rows themselves padded with zeros to dp32 floats (what the rescoring reads).  Rows in
 // [n, n_pad) are zeros, so that whole tiles can be read without bounds checks.  (One thread per
 // row reading its own row took 1.3 ms for 10^6 rows of 40 features.)
 constexpr int PREP_ROWS = 32;
@@ -120,7 +149,8 @@ constexpr int PREP_CF = 64;
 __global__ __launch_bounds__(256) void prep_kernel(const float* __restrict__ x, uint32_t n, uint32_t d,
                                                         uint32_t n_pad, uint32_t dp32, uint32_t dp16, int split16,
                                                         double* __restrict__ norms, float* __restrict__ xs32,
-                                                        _Float16* __restrict__ xs16, uint32_t* __restrict__ bad_row) {
+                                                        _Float16* __restrict__ xs16, float* __restrict__ xr,
+                                                        uint32_t* __restrict__ bad_row) {
   constexpr int PER = PREP_ROWS * PREP_CF / 256;  // elements per thread per chunk
   __shared__ uint32_t tile[2][PREP_ROWS][PREP_CF + 1];  // the rows' bits (the non-finite test reads bits)
   __shared__ double nrm_s[PREP_ROWS];
@@ -176,8 +206,10 @@ __global__ __launch_bounds__(256) void prep_kernel(const float* __restrict__ x, 
     const uint32_t r = e / dp32, f = e % dp32;
     if (r0 + r >= n_pad) break;
     const double nr_ = nrm_s[r];
-    xs32[(size_t)(r0 + r) * dp32 + f] =
-        (r < nr && f < d && nr_ > 0.0) ? (float)((double)__uint_as_float(xb[(size_t)(r0 + r) * d + f]) / nr_) : 0.0f;
+    const bool in = r < nr && f < d;
+    const float v = in ? __uint_as_float(xb[(size_t)(r0 + r) * d + f]) : 0.0f;
+    xs32[(size_t)(r0 + r) * dp32 + f] = (in && nr_ > 0.0) ? (float)((double)v / nr_) : 0.0f;
+    if (xr) xr[(size_t)(r0 + r) * dp32 + f] = v;
   }
   if (xs16) {
     // the FP16 copy (split: hi then lo halves), one feature per thread
@@ -426,8 +458,9 @@ __device__ __forceinline__ void wave_err_max(double err, unsigned long long* __r
 // err_max collects the largest |list score - f64 score| over the row's admitted candidates: the
 // measured counterpart of the error bound eps_list that the certificate relies on.  Scores are
 // cosines, or with Tanimoto selection Tanimoto scores (ranked by them; the cosines of the k kept
-// are computed again for the output).
-template <int CMAX>
+// are computed again for the output).  Row j of the original rows starts at x + j * d; with V4, d
+// is the stride of rows padded as dot64r<true> reads them.
+template <int CMAX, bool V4>
 __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_id, const float* __restrict__ x,
                             uint32_t d, const double* __restrict__ norms, uint32_t k, int kp, uint32_t nsplit,
                             uint32_t qq, uint32_t i, const uint32_t* __restrict__ cand_idx,
@@ -460,7 +493,7 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
     if (r < cand_cnt[slot]) {
       const uint32_t j = cand_idx[slot * kp + r];
       const uint32_t pos = atomicAdd(&s_fill, 1u);
-      const double c = score64(dot64(xi, x + (size_t)j * d, d), ni, norms[j], tani);
+      const double c = score64(dot64r<V4>(xi, x + (size_t)j * d, d), ni, norms[j], tani);
       err = fmax(err, fabs((double)cand_sc[slot * kp + r] - c));
       s_sc[pos] = c;
       s_id[pos] = j;
@@ -531,7 +564,7 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
   if (ok) {
     for (uint32_t t = tid_x(); t < k; t += 64) {
       const uint32_t j = s_id[t];
-      const double c = tani ? cos64(dot64(xi, x + (size_t)j * d, d), ni, norms[j]) : s_sc[t];
+      const double c = tani ? cos64(dot64r<V4>(xi, x + (size_t)j * d, d), ni, norms[j]) : s_sc[t];
       idx[(size_t)i * k + t] = j;
       cos_out[(size_t)i * k + t] = c;
       if (wts) wts[(size_t)i * k + t] = tani_from_cos(ni, norms[j], c);
@@ -548,6 +581,7 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
 // the same gather, f64 rescoring, rank placement, certificate and outputs as rescore_row, with
 // twice the rows in flight (the work is latency-bound).  Both halves call this together with their
 // own row (valid = false for a missing one), so every thread meets every barrier.
+template <bool V4>
 __device__ void rescore_half(double* __restrict__ s_sc, uint32_t* __restrict__ s_id, uint32_t* __restrict__ s_fill,
                              bool valid, uint32_t lane, const float* __restrict__ x, uint32_t d,
                              const double* __restrict__ norms, uint32_t k, int kp, uint32_t nsplit, uint32_t qq,
@@ -578,7 +612,7 @@ __device__ void rescore_half(double* __restrict__ s_sc, uint32_t* __restrict__ s
     if (r < cand_cnt[slot]) {
       const uint32_t j = cand_idx[slot * kp + r];
       const uint32_t pos = atomicAdd(s_fill, 1u);
-      const double c = score64(dot64(x + (size_t)i * d, x + (size_t)j * d, d), ni, norms[j], tani);
+      const double c = score64(dot64r<V4>(x + (size_t)i * d, x + (size_t)j * d, d), ni, norms[j], tani);
       err = fmax(err, fabs((double)cand_sc[slot * kp + r] - c));
       s_sc[pos] = c;
       s_id[pos] = j;
@@ -608,7 +642,7 @@ __device__ void rescore_half(double* __restrict__ s_sc, uint32_t* __restrict__ s
   if (ok) {
     for (uint32_t t = lane; t < k; t += 32) {
       const uint32_t j = s_id[t];
-      const double c = tani ? cos64(dot64(x + (size_t)i * d, x + (size_t)j * d, d), ni, norms[j]) : s_sc[t];
+      const double c = tani ? cos64(dot64r<V4>(x + (size_t)i * d, x + (size_t)j * d, d), ni, norms[j]) : s_sc[t];
       idx[(size_t)i * k + t] = j;
       cos_out[(size_t)i * k + t] = c;
       if (wts) wts[(size_t)i * k + t] = tani_from_cos(ni, norms[j], c);
@@ -623,7 +657,9 @@ __device__ void rescore_half(double* __restrict__ s_sc, uint32_t* __restrict__ s
 
 // The rows of a row list, or every row: in the locality order when perm is given (consecutive
 // workgroups then rescore similar rows, whose candidates overlap, so their rows stay in L2: 10^6 x
-// 40, k = 30: 3.54 -> 2.78 ms), lists by row.
+// 40, k = 30: 3.54 -> 2.78 ms), lists by row.  With V4, x holds rows of d floats as dot64r<true>
+// reads them (the input itself, or its padded copy).
+template <bool V4>
 __global__ __launch_bounds__(64) void rescore_pair_kernel(
     const float* __restrict__ x, const uint32_t* __restrict__ perm, uint32_t d, const double* __restrict__ norms,
     uint32_t k, int kp,
@@ -639,13 +675,14 @@ __global__ __launch_bounds__(64) void rescore_pair_kernel(
   const uint32_t qq = bid_x() * 2 + half;
   const bool valid = qq < nq;
   const uint32_t i = valid ? (rowlist ? rowlist[qq] : perm ? perm[qq] : qq) : 0;
-  rescore_half(s_sc[half], s_id[half], &s_fill[half], valid, lane, x, d, norms, k, kp, nsplit, rowlist ? qq : i, i,
+  rescore_half<V4>(s_sc[half], s_id[half], &s_fill[half], valid, lane, x, d, norms, k, kp, nsplit, rowlist ? qq : i,
+                   i,
                cand_idx,
                cand_sc, cand_cnt, cand_floor, all_admitted, eps_list, eps32, select, idx, cos_out, wts, i, fb_rows,
                fb_lo, fb_count, short_count, err_max);
 }
 
-template <int CMAX>
+template <int CMAX, bool V4>
 __global__ __launch_bounds__(64) void rescore_kernel(
     const float* __restrict__ x, const uint32_t* __restrict__ perm, uint32_t d, const double* __restrict__ norms,
     uint32_t n,
@@ -659,7 +696,7 @@ __global__ __launch_bounds__(64) void rescore_kernel(
   __shared__ uint32_t s_id[CMAX];
   for (uint32_t qq = bid_x(); qq < nq; qq += gdim_x()) {
     const uint32_t i = rowlist ? rowlist[qq] : perm ? perm[qq] : qq;
-    rescore_row<CMAX>(s_sc, s_id, x, d, norms, k, kp, nsplit, rowlist ? qq : i, i, cand_idx, cand_sc, cand_cnt,
+    rescore_row<CMAX, V4>(s_sc, s_id, x, d, norms, k, kp, nsplit, rowlist ? qq : i, i, cand_idx, cand_sc, cand_cnt,
                       cand_floor,
                       all_admitted, eps_list, eps32, select, idx, cos_out, wts, i, fb_rows, fb_lo, fb_count,
                       short_count, err_max);
@@ -911,7 +948,7 @@ __global__ __launch_bounds__(64) void rescore_batch_kernel(
   const uint32_t i = g - S.row_base;
   const uint32_t nlists = 2 * S.nsplit;
   const int all_admitted = (S.nsplit == 1 && (uint32_t)S.kp >= S.n - 1) ? 1 : 0;
-  rescore_row<CMAX>(s_sc, s_id, S.x, S.d, S.norms, k, S.kp, nlists, i, i, cand_idx + S.entry_base,
+  rescore_row<CMAX, false>(s_sc, s_id, S.x, S.d, S.norms, k, S.kp, nlists, i, i, cand_idx + S.entry_base,
                     cand_sc + S.entry_base, cand_cnt + S.slot_base, cand_floor + S.slot_base, all_admitted, eps_list,
                     eps32, COSTAL_SELECT_COSINE, S.idx, S.cos, S.wts, g, fb_rows, fb_lo, fb_count, short_count,
                     err_max);
@@ -935,7 +972,8 @@ __global__ __launch_bounds__(64) void rescore_batch_pair_kernel(
   const BatchSample& S = samples[valid ? find_sample(row_first, nsamples, g) : 0];
   const uint32_t i = valid ? g - S.row_base : 0;
   const int all_admitted = (S.nsplit == 1 && (uint32_t)S.kp >= S.n - 1) ? 1 : 0;
-  rescore_half(s_sc[half], s_id[half], &s_fill[half], valid, lane, S.x, S.d, S.norms, k, S.kp, 2 * S.nsplit, i, i,
+  rescore_half<false>(s_sc[half], s_id[half], &s_fill[half], valid, lane, S.x, S.d, S.norms, k, S.kp, 2 * S.nsplit,
+                      i, i,
                cand_idx + S.entry_base, cand_sc + S.entry_base, cand_cnt + S.slot_base, cand_floor + S.slot_base,
                all_admitted, eps_list, eps32, COSTAL_SELECT_COSINE, S.idx, S.cos, S.wts, g, fb_rows, fb_lo, fb_count,
                short_count, err_max);
@@ -1547,6 +1585,11 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
       std::max<uint64_t>(P.kp, std::min<uint64_t>((uint64_t)k + (k + 1) / 2, FUSED_KP_MAX)), n - 1);
   size_t nq_slots = 0, nq_entries = 0, score_floats = 0;
   const bool learned = P.fused && p.mlp_params && !p.row_threshold;
+  // The rescoring gathers candidate rows 16 bytes at a time: from the input itself when its rows
+  // are whole 16-byte words, else from a copy padded to dp32 floats that prep writes (left out,
+  // and the rows read one float at a time, when a caller's fixed scratch cannot hold it).
+  const bool x_v4 = d % 4 == 0 && (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+  bool xr_copy = (P.fused || P.wide) && !x_v4;
   auto workspace_bytes = [&]() -> size_t {
     // two candidate lists per query and split (one per lane of the query's lane pair)
     nq_slots = (P.fused || P.wide) ? (size_t)n * P.nsplit * 2 : 0;
@@ -1577,9 +1620,14 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
       c.take<float>(P.n_pad);
       c.take<float2>(P.n_pad / (P.TC / 2));
     }
+    if (xr_copy) c.take<float>((size_t)P.n_pad * P.dp32);
     return c.off + 256;
   };
   size_t bytes = workspace_bytes();
+  if (fixed && bytes > fixed->cap && xr_copy) {
+    xr_copy = false;  // rows read one float at a time
+    bytes = workspace_bytes();
+  }
   if (fixed && bytes > fixed->cap && P.prune) {
     P.prune = false;  // the bound matrix does not fit: scan every tile
     bytes = workspace_bytes();
@@ -1636,7 +1684,12 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   float* scores = P.wide ? c.take<float>(score_floats) : nullptr;
   float* tnorm = (P.fused && P.tani) ? c.take<float>(P.n_pad) : nullptr;
   float2* trange = (P.fused && P.tani) ? c.take<float2>(P.n_pad / (P.TC / 2)) : nullptr;
+  float* xr = xr_copy ? c.take<float>((size_t)P.n_pad * P.dp32) : nullptr;
   double* norms = norms_out ? norms_out : norms_ws;
+  // the rows the rescoring reads, and their stride in floats
+  const bool rows_v4 = x_v4 || xr_copy;
+  const float* xrows = xr_copy ? xr : x;
+  const uint32_t rstride = xr_copy ? P.dp32 : d;
 
   HIP_TRY(hipEventRecord(D.ev0, s));
   // The learned thresholds only read x: they run on the side stream from the start (behind
@@ -1662,7 +1715,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   HIP_TRY(hipEventRecord(D.evk[0], s));
   prep_kernel<<<(P.n_pad + PREP_ROWS - 1) / PREP_ROWS, 256, 0, s>>>(x, n, d, P.n_pad, P.dp32, P.dp16, split16,
                                                                          norms, xs32, P.prune ? nullptr : xs16,
-                                                                         counters);
+                                                                         xr, counters);
   HIP_TRY(hipGetLastError());
   // Tanimoto: the norm extremes come back with the non-finite check (counters[14..17])
   if (P.fused && P.tani) {
@@ -1722,24 +1775,29 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
                      double eps_list, uint32_t* fbr, double* fbl, uint32_t* fbc, uint32_t* shc) -> int {
     const uint32_t nlists = 2 * nsplit;
     const uint32_t slots = nlists * kp;
-#define RESCORE(CM)                                                                                    \
-  rescore_kernel<CM><<<nq, 64, 0, s>>>(x, P.prune ? ob.perm : nullptr, d, norms, n, k, (int)kp, nlists, rows, nq, \
-                                       cand_idx, cand_sc,                                                     \
-                                       cand_cnt, cand_floor, all_admitted, eps_list, eps_row, sel, idx, cos,  \
-                                       wts, fbr, fbl, fbc, shc, err_max)
-    if (slots <= 32) {
-      rescore_pair_kernel<<<(nq + 1) / 2, 64, 0, s>>>(x, P.prune ? ob.perm : nullptr, d, norms, k, (int)kp, nlists, rows,
-                                                      nq, cand_idx, cand_sc,
-                                                      cand_cnt, cand_floor, all_admitted, eps_list, eps_row, sel, idx,
-                                                      cos, wts, fbr, fbl, fbc, shc, err_max);
-    } else if (slots <= 64) RESCORE(64);
-    else if (slots <= 128) RESCORE(128);
-    else if (slots <= 256) RESCORE(256);
-    else if (slots <= 512) RESCORE(512);
-    else if (slots <= 1024) RESCORE(1024);
-    else if (slots <= 2048) RESCORE(2048);
-    else RESCORE(4096);
+    // rows_v4: the rows are read 16 bytes at a time (xrows, rstride as dot64r<true> reads them)
+    auto launch = [&](auto v4) {
+      constexpr bool V4 = decltype(v4)::value;
+#define RESCORE(CM)                                                                                              \
+  rescore_kernel<CM, V4><<<nq, 64, 0, s>>>(xrows, P.prune ? ob.perm : nullptr, rstride, norms, n, k, (int)kp, nlists, \
+                                           rows, nq, cand_idx, cand_sc, cand_cnt, cand_floor, all_admitted,      \
+                                           eps_list, eps_row, sel, idx, cos, wts, fbr, fbl, fbc, shc, err_max)
+      if (slots <= 32) {
+        rescore_pair_kernel<V4><<<(nq + 1) / 2, 64, 0, s>>>(xrows, P.prune ? ob.perm : nullptr, rstride, norms, k,
+                                                            (int)kp, nlists, rows, nq, cand_idx, cand_sc, cand_cnt,
+                                                            cand_floor, all_admitted, eps_list, eps_row, sel, idx, cos,
+                                                            wts, fbr, fbl, fbc, shc, err_max);
+      } else if (slots <= 64) RESCORE(64);
+      else if (slots <= 128) RESCORE(128);
+      else if (slots <= 256) RESCORE(256);
+      else if (slots <= 512) RESCORE(512);
+      else if (slots <= 1024) RESCORE(1024);
+      else if (slots <= 2048) RESCORE(2048);
+      else RESCORE(4096);
 #undef RESCORE
+    };
+    if (rows_v4) launch(std::true_type{});
+    else launch(std::false_type{});
     HIP_TRY(hipGetLastError());
     return 0;
   };

@@ -112,6 +112,46 @@ def test_independent_numpy_oracle(matmul):
         assert set(idx[i]) == set(order[i, :12])
 
 
+@pytest.mark.parametrize("d", [15, 32])
+def test_device_rows_at_any_float_offset(d, oracle):
+    """The device entry point on the same rows starting on a 16-byte boundary and 4 bytes past one.
+    The rescoring reads rows 16 bytes at a time: rows of 32 features straight from an aligned input,
+    from a padded copy otherwise (15 features, or a misaligned start); the output is the host
+    entry point's, bit for bit, which equals the brute force's."""
+    import ctypes as C
+
+    import torch
+
+    from costal_amd import _lib
+
+    x = np.ascontiguousarray(synth.cytometry_86864x39(n=4000)[:, :d])
+    n, k = x.shape[0], 10
+    want, _ = ext._knn_exact(x, k)
+    assert_same(want, oracle.knn_bruteforce(x, k), f"d={d} host entry point")
+    L = _lib.load()
+    dev = torch.device("cuda:0")
+    flat = torch.empty(n * d + 4, dtype=torch.float32, device=dev)
+    for off in (0, 1):
+        v = flat[off:off + n * d]
+        v.copy_(torch.from_numpy(x.ravel()))
+        assert (v.data_ptr() % 16 == 0) == (off == 0)
+        idx = torch.empty((n, k), dtype=torch.int32, device=dev)
+        cos = torch.empty((n, k), dtype=torch.float64, device=dev)
+        norms = torch.empty(n, dtype=torch.float64, device=dev)
+        p = _lib.KnnParams()
+        p.k = k
+        p.margin = -1
+        p.precision = _lib.PRECISION["fp16x3"]
+        st = _lib.KnnStats()
+        torch.cuda.synchronize()
+        _lib.check(L.costal_knn_exact_device(C.c_void_p(v.data_ptr()), n, d, C.byref(p), C.c_void_p(idx.data_ptr()),
+                                             C.c_void_p(cos.data_ptr()), C.c_void_p(norms.data_ptr()), None,
+                                             C.byref(st), C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        torch.cuda.synchronize()
+        assert_same((idx.cpu().numpy().astype(np.uint32), cos.cpu().numpy(), norms.cpu().numpy()), want,
+                    f"d={d} offset {4 * off} bytes")
+
+
 @pytest.mark.parametrize("matmul", BACKENDS)
 @pytest.mark.parametrize("splits", [0, 1, 3, 8])
 def test_output_independent_of_splits(splits, matmul, oracle):

@@ -82,3 +82,29 @@ def test_noalloc_refuses_what_does_not_fit():
     with pytest.raises(_lib.CostalError, match="bytes of device scratch") as e:
         device_call("costal_knn_exact_device_noalloc", x, 30)
     assert e.value.code == _lib.E_UNSUPPORTED
+
+
+def test_noalloc_largest_fit_reads_rows_unpadded():
+    """The largest row count the static scratch accepts (found by bisection; a refused call
+    launches nothing).  There the call has given up, in this order, the padded copy of the rows
+    that the rescoring reads 16 bytes at a time (it then reads the input one float at a time),
+    the pruned scan (no tile pair skipped) and candidate splits; its output still equals the
+    allocating entry point's bit for bit."""
+    x_all = synth.rare_396460x15(n=2_000_000, sample_seed=7)
+    lo, hi = 60_000, 2_000_000  # lo fits, hi does not (test_noalloc_refuses_what_does_not_fit)
+    while hi - lo > 4096:
+        mid = (lo + hi) // 2
+        try:
+            device_call("costal_knn_exact_device_noalloc", np.ascontiguousarray(x_all[:mid]), 10)
+            lo = mid
+        except _lib.CostalError as e:
+            assert e.code == _lib.E_UNSUPPORTED
+            hi = mid
+    x = np.ascontiguousarray(x_all[:lo])
+    print(f"noalloc largest fit n={lo}", flush=True)
+    a = device_call("costal_knn_exact_device", x, 10)
+    b = device_call("costal_knn_exact_device_noalloc", x, 10)
+    assert b[4].blocks_skipped == 0 and a[4].blocks_skipped > 0
+    for u, v in zip(a[:3], b[:3]):
+        np.testing.assert_array_equal(v, u)
+    np.testing.assert_allclose(b[3], a[3], rtol=1e-12, atol=0)

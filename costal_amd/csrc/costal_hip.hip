@@ -1223,7 +1223,9 @@ struct Device {
   // call start/end and per-kernel brackets: prep, coarse, rescore, fallback
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t evk[8] = {};
-  hipEvent_t evcopy = nullptr;  // a device-to-host copy done (the stream may run on behind it)
+  // pinned host memory for the small readbacks the host branches on (counters), made by the
+  // allocating entry points; null until then, and for a caller that never allocates
+  void* hbuf = nullptr;
   // a second stream (default priority) for work that overlaps the call's own: the learned
   // thresholds run there beside the locality order; fork and join events
   hipStream_t side = nullptr;
@@ -1262,7 +1264,6 @@ int device_init(Device& D) {
   HIP_TRY(hipStreamCreateWithFlags(&D.stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&D.ev0));
   HIP_TRY(hipEventCreate(&D.ev1));
-  HIP_TRY(hipEventCreateWithFlags(&D.evcopy, hipEventDisableTiming));
   HIP_TRY(hipStreamCreateWithFlags(&D.side, hipStreamNonBlocking));
   HIP_TRY(hipEventCreateWithFlags(&D.ev_fork, hipEventDisableTiming));
   HIP_TRY(hipEventCreateWithFlags(&D.ev_join, hipEventDisableTiming));
@@ -1277,7 +1278,28 @@ int device_init(Device& D) {
   return 0;
 }
 
+constexpr size_t HBUF_BYTES = 4096;
+
+// Copies bytes at src (device) to dst (host) once the work queued on s before it is done, and
+// waits for it.  Through the device's pinned buffer when it has one: a copy into pageable memory
+// is staged by the runtime, which adds to each of the call's host round trips.
+int readback(Device& D, void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (D.hbuf && bytes <= HBUF_BYTES) {
+    HIP_TRY(hipMemcpyAsync(D.hbuf, src, bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    memcpy(dst, D.hbuf, bytes);
+  } else {
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+  }
+  return 0;
+}
+
 int ws_reserve(Device& D, size_t bytes) {
+  if (!D.hbuf && hipHostMalloc(&D.hbuf, HBUF_BYTES, hipHostMallocDefault) != hipSuccess) {
+    D.hbuf = nullptr;  // readbacks go to pageable memory
+    (void)hipGetLastError();
+  }
   if (bytes <= D.ws_cap) return 0;
   if (D.ws) {
     HIP_TRY(hipStreamSynchronize(D.stream));
@@ -1725,9 +1747,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   }
   HIP_TRY(hipEventRecord(D.evk[1], s));
   uint32_t head[18] = {};
-  HIP_TRY(hipMemcpyAsync(head, counters, (P.fused && P.tani ? 18 : 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipEventRecord(D.evcopy, s));
-  HIP_TRY(hipEventSynchronize(D.evcopy));
+  if ((rc = readback(D, head, counters, (P.fused && P.tani ? 18 : 1) * sizeof(uint32_t), s))) return rc;
   const uint32_t bad = head[0];
   if (bad != NONE) {
     if (learned_early) HIP_TRY(hipStreamSynchronize(D.side));  // nothing still reads x when we return
@@ -1851,8 +1871,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     // list (wide candidate splits keep the few rows' scan parallel); only rows that fail again
     // (exact ties) go to the full-row kernel.
     uint32_t nfb_short[2] = {0, 0};  // rows that failed, and those of them with fewer than k candidates
-    HIP_TRY(hipMemcpyAsync(nfb_short, counters + 1, sizeof nfb_short, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    if ((rc = readback(D, nfb_short, counters + 1, sizeof nfb_short, s))) return rc;
     const uint32_t nfb = nfb_short[0], nshort = nfb_short[1];
     if (nfb > 0) {
       CoarseFn fn2 = coarse_select(P.prec, P.J, list_template(kp2), P.tani);
@@ -1921,8 +1940,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
         // the rescoring takes the top k of many entries quickly), not the row scan (which would
         // stream every candidate of such a row through one workgroup).
         uint32_t c34[2] = {0, 0};
-        HIP_TRY(hipMemcpyAsync(c34, counters + 3, sizeof c34, hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
+        if ((rc = readback(D, c34, counters + 3, sizeof c34, s))) return rc;
         if (c34[1] > 0) {
           split_failures_kernel<<<(c34[0] + 255) / 256, 256, 0, s>>>(fb2_rows, fb2_lo, c34[0], fb_rows, fb_lo,
                                                                     counters + 5, fb3_rows, counters + 6);
@@ -1985,8 +2003,7 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
   }
   HIP_TRY(hipEventRecord(D.ev1, s));
   uint32_t cnt[COUNTER_WORDS] = {};
-  HIP_TRY(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  if ((rc = readback(D, cnt, counters, sizeof cnt, s))) return rc;
   if (st) {
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, D.ev0, D.ev1) != hipSuccess) ms = 0.0f;
@@ -2322,8 +2339,7 @@ int knn_batched_locked(Device& D, const costal_sample* smp, uint32_t S, const co
   }
   HIP_TRY(hipEventRecord(D.ev1, s));
   uint32_t cnt[COUNTER_WORDS] = {};
-  HIP_TRY(hipMemcpyAsync(cnt, counters, sizeof cnt, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  if (int rb = readback(D, cnt, counters, sizeof cnt, s)) return rb;
   if (cnt[0] != NONE || cnt[1] != NONE) {
     if (st) {
       memset(st, 0, sizeof *st);

@@ -17,6 +17,8 @@
 
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "../../include/costal_hip.h"
 
 namespace mlp {
@@ -121,6 +123,124 @@ __global__ __launch_bounds__(256) void predict_kernel(const float* __restrict__ 
   load_features(x + (size_t)r * d, d, xn);
   const float y = forward_row<BF16>(S, xn, h1, h2);
   out[r] = raw ? y - offset : 1.0f - expf(y + offset) - margin;
+}
+
+// BF16 inference on the matrix cores (v_mfma_f32_32x32x16_bf16): one wave per 32 rows at a time.
+// The layers are computed transposed, hidden units x rows: H1^T = W1 Xn^T, H2^T = W2 H1^T.  In a
+// 32x32 accumulator, lane l holds row l % 32 and units 8 (v / 4) + 4 (l / 32) + v % 4 of the tile
+// (v = 0..15), so values 8c'..8c'+7 of the two lane halves together cover 16 units: layer 1's
+// accumulators, after the bias, ReLU and BF16 rounding, are layer 2's B operand as they stand,
+// with each K chunk in that permuted unit order, and W2's A operand is loaded in the same order.
+// Inputs, weights and activations are rounded to BF16 as in forward_row<true>; their products are
+// exact in f32 and accumulate in f32 in the matrix core's order (forward_row's fmaf chain agrees to
+// f32 rounding).  A thread per row on the vector ALUs (predict_kernel) took 198 us for 396,460
+// rows of 15 features, beside the kNN call's first kernels, which it slowed.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// the hidden unit of accumulator value v of lane half h, in 32-unit tile T
+__device__ __forceinline__ int acc_unit(int T, int v, int h) { return 32 * T + 8 * (v >> 2) + 4 * h + (v & 3); }
+
+// eight BF16-rounded f32 values as BF16 (element m in the low half of dword m / 2 when m is even)
+__device__ __forceinline__ bf16x8 pack_bf16(const float (&v)[8]) {
+  uint32_t u[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) u[q] = (__float_as_uint(v[2 * q]) >> 16) | (__float_as_uint(v[2 * q + 1]) & 0xffff0000u);
+  return __builtin_bit_cast(bf16x8, make_uint4(u[0], u[1], u[2], u[3]));
+}
+
+// KC: 16-feature chunks of the first layer (d <= 16 KC)
+template <int KC>
+__global__ __launch_bounds__(256) void predict_mfma_kernel(const float* __restrict__ x, uint32_t n, uint32_t d,
+                                                           const float* __restrict__ P, float offset, float margin,
+                                                           int raw, float* __restrict__ out) {
+  __shared__ float s_b1[H], s_b2[H], s_w3[H];
+  const float* W1 = P;
+  const float* b1 = W1 + H * d;
+  const float* W2 = b1 + H;
+  const float* b2 = W2 + H * H;
+  const float* w3 = b2 + H;
+  const float b3 = w3[H];
+  for (int e = tid_x(); e < H; e += bdim_x()) {
+    s_b1[e] = b1[e];
+    s_b2[e] = b2[e];
+    s_w3[e] = bf16r(w3[e]);
+  }
+  __syncthreads();
+  const int lane = tid_x() & 63, i = lane & 31, h = lane >> 5;
+  // the weights in this lane's A-operand layout: row i of each 32-row tile, K elements 8h..8h+7
+  bf16x8 w1op[2][KC], w2op[2][4];
+#pragma unroll
+  for (int T = 0; T < 2; ++T) {
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+      float v[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const uint32_t f = 16 * c + 8 * h + m;
+        v[m] = f < d ? bf16r(W1[(size_t)(32 * T + i) * d + f]) : 0.0f;
+      }
+      w1op[T][c] = pack_bf16(v);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      float v[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = bf16r(W2[(32 * T + i) * H + acc_unit(c >> 1, 8 * (c & 1) + m, h)]);
+      w2op[T][c] = pack_bf16(v);
+    }
+  }
+  const uint32_t nwaves = gdim_x() * (bdim_x() >> 6);
+  for (uint32_t r0 = (bid_x() * (bdim_x() >> 6) + (tid_x() >> 6)) * 32; r0 < n; r0 += nwaves * 32) {
+    // this lane's row (both halves hold it): f64 sequential norm, features f32(x / norm) in BF16
+    const uint32_t r = r0 + i < n ? r0 + i : n - 1;
+    const float* row = x + (size_t)r * d;
+    double s = 0.0;
+    for (uint32_t t = 0; t < d; ++t) s += (double)row[t] * (double)row[t];
+    const double nrm = sqrt(s);
+    bf16x8 xop[KC];
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+      float v[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const uint32_t f = 16 * c + 8 * h + m;
+        v[m] = (f < d && nrm > 0.0) ? bf16r((float)((double)row[f] / nrm)) : 0.0f;
+      }
+      xop[c] = pack_bf16(v);
+    }
+    // layer 1, accumulators starting at the bias
+    f32x16 a1[2];
+#pragma unroll
+    for (int T = 0; T < 2; ++T) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) a1[T][v] = s_b1[acc_unit(T, v, h)];
+#pragma unroll
+      for (int c = 0; c < KC; ++c) a1[T] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1op[T][c], xop[c], a1[T], 0, 0, 0);
+    }
+    bf16x8 hop[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      float v[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = bf16r(fmaxf(a1[c >> 1][8 * (c & 1) + m], 0.0f));
+      hop[c] = pack_bf16(v);
+    }
+    // layer 2, then the output unit over this lane's 32 hidden units and its partner lane's
+    float y = 0.0f;
+#pragma unroll
+    for (int T = 0; T < 2; ++T) {
+      f32x16 a2;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) a2[v] = s_b2[acc_unit(T, v, h)];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) a2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2op[T][c], hop[c], a2, 0, 0, 0);
+#pragma unroll
+      for (int v = 0; v < 16; ++v) y = fmaf(s_w3[acc_unit(T, v, h)], bf16r(fmaxf(a2[v], 0.0f)), y);
+    }
+    y = b3 + (y + __shfl_xor(y, 32));
+    if (h == 0 && r0 + i < n) out[r0 + i] = raw ? y - offset : 1.0f - expf(y + offset) - margin;
+  }
 }
 
 // Training forward + backward of the activations for one batch; per row it stores
@@ -285,11 +405,22 @@ int costal_mlp_predict_device(const float* x, uint64_t n, uint64_t d, const floa
   if (n == 0) return 0;
   if (n > 0xffffffffull) return mlp::fail(COSTAL_E_TOO_MANY_ROWS, "too many rows");
   hipStream_t s = (hipStream_t)stream;
-  const unsigned grid = (unsigned)((n + 255) / 256);
-  if (bf16)
-    mlp::predict_kernel<true><<<grid, 256, 0, s>>>(x, (uint32_t)n, (uint32_t)d, params, offset, margin, (int)raw, out);
-  else
+  if (bf16) {
+    // four waves of 32-row batches per workgroup; the waves loop when there are more batches
+    // than 1,024 workgroups hold (each wave loads the weights once)
+    const unsigned grid = (unsigned)std::min<uint64_t>((n + 127) / 128, 1024);
+    const uint32_t kc = (uint32_t)((d + 15) / 16);
+#define PREDICT(KC) \
+  mlp::predict_mfma_kernel<KC><<<grid, 256, 0, s>>>(x, (uint32_t)n, (uint32_t)d, params, offset, margin, (int)raw, out)
+    if (kc == 1) PREDICT(1);
+    else if (kc == 2) PREDICT(2);
+    else if (kc == 3) PREDICT(3);
+    else PREDICT(4);
+#undef PREDICT
+  } else {
+    const unsigned grid = (unsigned)((n + 255) / 256);
     mlp::predict_kernel<false><<<grid, 256, 0, s>>>(x, (uint32_t)n, (uint32_t)d, params, offset, margin, (int)raw, out);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return mlp::fail(COSTAL_E_DEVICE, "predict launch failed: ", hipGetErrorString(e));
   return 0;

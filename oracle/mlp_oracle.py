@@ -5,6 +5,8 @@ costal_amd/csrc/mlp.hip (tests/ import it; the product path never does).  This p
 counterpart in the reference (it is the admission-threshold model of the fused scan), so it is
 pinned by mathematics instead: its gradients are checked against central finite differences on
 small instances (tests/test_mlp_oracle.py), which a wrong transpose, sign or missing ReLU mask fails.
+forward_bf16 (the BF16 inference's roundings) is pinned by round-to-nearest-even on halfway values
+and by networks whose every value is exact in bfloat16, where it must equal forward() bit for bit.
 
 Parameter layout (flat): W1[64][d], b1[64], W2[64][64], b2[64], w3[64], b3.
 Features: x / |x| with the f64 sequential norm, rounded to float32 (zero rows -> zeros).
@@ -53,6 +55,29 @@ def forward(p, x, d):
     h2 = np.maximum(z2, 0.0)
     y = h2 @ w3 + b3
     return y, (xn, z1, h1, z2, h2)
+
+
+def bf16_round(a) -> np.ndarray:
+    """Round-to-nearest-even to bfloat16, kept in float32 (the low 16 bits cleared)."""
+    u = np.asarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def forward_bf16(p, x, d):
+    """forward() as the BF16 inference rounds it: features, W1, W2, w3 and both hidden layers'
+    activations (after the ReLU) rounded to bfloat16; biases in float32; each layer's sum taken in
+    float64 and rounded to float32 before the ReLU (the GPU accumulates in float32, in its own
+    order, so the two agree to float32 rounding of the sums)."""
+    W1, b1, W2, b2, w3, b3 = unpack(p, d)
+    def r(a):
+        return bf16_round(a).astype(np.float64)
+    def f32(a):
+        return np.asarray(a, dtype=np.float32).astype(np.float64)
+    xn = r(features(x))
+    h1 = r(np.maximum(f32(xn @ r(W1).T + b1), 0.0))
+    h2 = r(np.maximum(f32(h1 @ r(W2).T + b2), 0.0))
+    return f32(h2 @ r(w3) + b3)
 
 
 def loss_and_grad(p, x, target, d):

@@ -39,6 +39,30 @@ def test_gpu_predict_matches_oracle_forward():
     assert np.abs(y16 - ycpu).max() <= 0.03 * np.abs(ycpu).max()
 
 
+@pytest.mark.parametrize("d", [15, 40, 64])
+def test_gpu_bf16_predict_matches_oracle_roundings(d):
+    """The matrix-core BF16 inference (one to four 16-feature chunks in the first layer) against
+    the oracle's restatement of the same roundings: the GPU sums in float32 in its own order, so a
+    hidden value now and then rounds to the other bfloat16 neighbour; any slip in the operand
+    layouts would be an error of the output's own size."""
+    rng = np.random.default_rng(d)
+    x = rng.normal(size=(3000, d)).astype(np.float32)
+    x[7] = 0.0  # a zero row
+    x[:1000] = np.abs(x[:1000])
+    p = learned.init_params(d, seed=d)
+    o = M.H * d  # nonzero biases, so that every bias reaches its unit
+    p[o:o + M.H] = rng.normal(0, 0.2, M.H).astype(np.float32)
+    p[o + M.H + M.H * M.H:o + 2 * M.H + M.H * M.H] = rng.normal(0, 0.2, M.H).astype(np.float32)
+    y16 = learned.predict_raw(p, x, bf16=True)
+    want = M.forward_bf16(p, x, d)
+    err = np.abs(y16 - want)
+    scale = np.abs(want).max()
+    assert scale > 0.1
+    assert np.median(err) <= 2e-6 * scale
+    assert err.max() <= 2e-2 * scale
+    assert (err <= 1e-5 * scale).mean() >= 0.9
+
+
 def test_training_reduces_loss():
     x = synth.learned_draw(20000, sample_seed=7)
     rng = np.random.default_rng(0)

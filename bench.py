@@ -50,13 +50,13 @@ PRECISION_OF = {"fp32": "fp32", "fp16": "fp16", "fp16x3": "fp16x3", "fp16x3+lear
 # factors from a 1 GiB device copy under the same counters; rocprofv3 --pmc runs of this script,
 # mean over the first passes of the run (profiles/, one summary per kernel and workload).
 MEASURED_TRAFFIC = {
-    ("rare_396460x15", "fp16x3+learned"): 8441.3e6,
-    ("learned_1000000x40", "fp16x3+learned"): 95837.0e6,
-    ("cytometry_86864x39", "fp16x3"): 1840.7e6,
-    ("counts_3605x2000", "fp32"): 1014.7e6 + 57.3e6,
-    ("batched_64", "fp16x3"): 2830.9e6,
-    ("rare_396460x15", "tanimoto-select"): 10563.1e6,
-    ("cytometry_86864x39", "tanimoto-select"): 1861.0e6,
+    ("rare_396460x15", "fp16x3+learned"): 8455.9e6,
+    ("learned_1000000x40", "fp16x3+learned"): 95774.3e6,
+    ("cytometry_86864x39", "fp16x3"): 1847.9e6,
+    ("counts_3605x2000", "fp32"): 1015.0e6 + 57.3e6,
+    ("batched_64", "fp16x3"): 2839.3e6,
+    ("rare_396460x15", "tanimoto-select"): 10571.8e6,
+    ("cytometry_86864x39", "tanimoto-select"): 1859.4e6,
 }
 WORKLOADS = {
     # name: (k, forms, train the threshold model on this many other cells)

@@ -454,13 +454,14 @@ __device__ __forceinline__ void wave_err_max(double err, unsigned long long* __r
     atomicMax(err_max, bits);
 }
 
-// One query's rescoring (64 threads).  Failed rows are recorded as fb_tag in fb_rows.
+// One query's rescoring (NT threads, a multiple of 64: all of them gather and rescore the entries,
+// the first wave ranks them).  Failed rows are recorded as fb_tag in fb_rows.
 // err_max collects the largest |list score - f64 score| over the row's admitted candidates: the
 // measured counterpart of the error bound eps_list that the certificate relies on.  Scores are
 // cosines, or with Tanimoto selection Tanimoto scores (ranked by them; the cosines of the k kept
 // are computed again for the output).  Row j of the original rows starts at x + j * d; with V4, d
 // is the stride of rows padded as dot64r<true> reads them.
-template <int CMAX, bool V4>
+template <int CMAX, bool V4, int NT = 64>
 __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_id, const float* __restrict__ x,
                             uint32_t d, const double* __restrict__ norms, uint32_t k, int kp, uint32_t nsplit,
                             uint32_t qq, uint32_t i, const uint32_t* __restrict__ cand_idx,
@@ -487,7 +488,7 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
   if (tid_x() == 0) s_fill = 0;
   __syncthreads();
   double err = 0.0;  // the largest |list score - f64 cosine| seen by this thread
-  for (int e = tid_x(); e < tot_slots; e += 64) {
+  for (int e = tid_x(); e < tot_slots; e += NT) {
     const uint32_t s = (uint32_t)e / (uint32_t)kp, r = (uint32_t)e % (uint32_t)kp;
     const size_t slot = (size_t)qq * nsplit + s;
     if (r < cand_cnt[slot]) {
@@ -524,15 +525,19 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
     // so k rounds of a wave-wide arg-max under the same order (each thread takes the best entry
     // it holds, the wave the best of those, which then leaves the pool) instead of a full sort
     // (k <= 64: round r's winner stays with thread r).  A bitonic sort of 1,024 entries by one
-    // wave took 133 us per row.
+    // wave took 133 us per row.  With several waves each takes the best of its threads', and the
+    // waves' bests meet in LDS (the order is total, so every thread picks the same winner).
     const uint32_t t = tid_x();
     const uint32_t rounds = k < filled ? k : filled;
+    constexpr int NWR = NT / 64;
+    __shared__ double s_wbs[NWR];
+    __shared__ uint32_t s_wbi[NWR], s_wbe[NWR];
     double win_sc = -INFINITY;
     uint32_t win_id = NONE;
     for (uint32_t r = 0; r < rounds; ++r) {
       double bs = -INFINITY;
       uint32_t bi = NONE, be = NONE;
-      for (uint32_t e = t; e < filled; e += 64) {
+      for (uint32_t e = t; e < filled; e += NT) {
         const uint32_t id = s_id[e];
         const double sc = s_sc[e];
         if (id != NONE && (bi == NONE || ranks_before(sc, id, bs, bi))) bs = sc, bi = id, be = e;
@@ -541,6 +546,16 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
         const double os = __shfl_xor(bs, off);
         const uint32_t oi = (uint32_t)__shfl_xor((int)bi, off), oe = (uint32_t)__shfl_xor((int)be, off);
         if (oi != NONE && (bi == NONE || ranks_before(os, oi, bs, bi))) bs = os, bi = oi, be = oe;
+      }
+      if constexpr (NWR > 1) {
+        if ((t & 63) == 0) s_wbs[t >> 6] = bs, s_wbi[t >> 6] = bi, s_wbe[t >> 6] = be;
+        __syncthreads();
+#pragma unroll
+        for (int v = 0; v < NWR; ++v) {
+          const double os = s_wbs[v];
+          const uint32_t oi = s_wbi[v], oe = s_wbe[v];
+          if (oi != NONE && (bi == NONE || ranks_before(os, oi, bs, bi))) bs = os, bi = oi, be = oe;
+        }
       }
       if (t == r) win_sc = bs, win_id = bi;
       if (t == 0) s_id[be] = NONE;
@@ -562,7 +577,7 @@ __device__ void rescore_row(double* __restrict__ s_sc, uint32_t* __restrict__ s_
     ok = s_sc[k - 1] > floor_ + eps_list;
   }
   if (ok) {
-    for (uint32_t t = tid_x(); t < k; t += 64) {
+    for (uint32_t t = tid_x(); t < k; t += NT) {
       const uint32_t j = s_id[t];
       const double c = tani ? cos64(dot64r<V4>(xi, x + (size_t)j * d, d), ni, norms[j]) : s_sc[t];
       idx[(size_t)i * k + t] = j;
@@ -682,8 +697,14 @@ __global__ __launch_bounds__(64) void rescore_pair_kernel(
                fb_lo, fb_count, short_count, err_max);
 }
 
+// Threads per row of rescore_kernel: the few rows of the rescan over up to 96 splits gather and
+// rank their entries with four waves, so that each thread's chain of dependent gathers and LDS
+// reads stays short (two rows of 1,740 entries: 99 us with one wave, 50 us with four); rows of
+// up to 1,024 entries come in thousands and keep one wave each (four: 62 -> 75 us for 2,876 rows).
+constexpr int rescore_threads(int cmax) { return cmax >= 2048 ? 256 : 64; }
+
 template <int CMAX, bool V4>
-__global__ __launch_bounds__(64) void rescore_kernel(
+__global__ __launch_bounds__(rescore_threads(CMAX)) void rescore_kernel(
     const float* __restrict__ x, const uint32_t* __restrict__ perm, uint32_t d, const double* __restrict__ norms,
     uint32_t n,
     uint32_t k, int kp, uint32_t nsplit, const uint32_t* __restrict__ rowlist, uint32_t nq,
@@ -696,10 +717,9 @@ __global__ __launch_bounds__(64) void rescore_kernel(
   __shared__ uint32_t s_id[CMAX];
   for (uint32_t qq = bid_x(); qq < nq; qq += gdim_x()) {
     const uint32_t i = rowlist ? rowlist[qq] : perm ? perm[qq] : qq;
-    rescore_row<CMAX, V4>(s_sc, s_id, x, d, norms, k, kp, nsplit, rowlist ? qq : i, i, cand_idx, cand_sc, cand_cnt,
-                      cand_floor,
-                      all_admitted, eps_list, eps32, select, idx, cos_out, wts, i, fb_rows, fb_lo, fb_count,
-                      short_count, err_max);
+    rescore_row<CMAX, V4, rescore_threads(CMAX)>(s_sc, s_id, x, d, norms, k, kp, nsplit, rowlist ? qq : i, i, cand_idx,
+                                                  cand_sc, cand_cnt, cand_floor, all_admitted, eps_list, eps32, select,
+                                                  idx, cos_out, wts, i, fb_rows, fb_lo, fb_count, short_count, err_max);
     __syncthreads();  // the LDS arrays are refilled by the next row
   }
 }
@@ -1798,10 +1818,10 @@ int knn_device_locked(Device& D, const float* x, uint64_t n64, uint64_t d64, con
     // rows_v4: the rows are read 16 bytes at a time (xrows, rstride as dot64r<true> reads them)
     auto launch = [&](auto v4) {
       constexpr bool V4 = decltype(v4)::value;
-#define RESCORE(CM)                                                                                              \
-  rescore_kernel<CM, V4><<<nq, 64, 0, s>>>(xrows, P.prune ? ob.perm : nullptr, rstride, norms, n, k, (int)kp, nlists, \
-                                           rows, nq, cand_idx, cand_sc, cand_cnt, cand_floor, all_admitted,      \
-                                           eps_list, eps_row, sel, idx, cos, wts, fbr, fbl, fbc, shc, err_max)
+#define RESCORE(CM)                                                                                       \
+  rescore_kernel<CM, V4><<<nq, rescore_threads(CM), 0, s>>>(                                                  \
+      xrows, P.prune ? ob.perm : nullptr, rstride, norms, n, k, (int)kp, nlists, rows, nq, cand_idx, cand_sc, \
+      cand_cnt, cand_floor, all_admitted, eps_list, eps_row, sel, idx, cos, wts, fbr, fbl, fbc, shc, err_max)
       if (slots <= 32) {
         rescore_pair_kernel<V4><<<(nq + 1) / 2, 64, 0, s>>>(xrows, P.prune ? ob.perm : nullptr, rstride, norms, k,
                                                             (int)kp, nlists, rows, nq, cand_idx, cand_sc, cand_cnt,

@@ -300,3 +300,35 @@ def test_non_finite_row_is_reported():
     x[7, 0] = np.inf
     with pytest.raises(ValueError, match="row 3 contains NaN or inf"):
         ext._knn_exact(x, 5)
+
+
+@pytest.mark.parametrize("case", ["nan", "inf", "-inf", "row of inf", "two rows"])
+def test_non_finite_row_on_the_pruned_path(case, oracle):
+    """At a size that takes the pruned scan: the first row with a NaN or inf is the call's error,
+    as a value, and the next call, on clean rows, is exact and pruned."""
+    from costal_amd import _lib
+
+    x = synth.rare_396460x15(n=30000)
+    y = x.copy()
+    first = 20011
+    if case == "nan":
+        y[first, 4] = np.nan
+    elif case == "inf":
+        y[first, 0] = np.inf
+    elif case == "-inf":
+        y[first, 14] = -np.inf
+    elif case == "row of inf":
+        y[first] = np.inf
+    else:
+        first = 517
+        y[first, 3] = np.inf
+        y[29999, 2] = np.nan
+    with pytest.raises(_lib.CostalError, match=f"row {first} contains NaN or inf") as e:
+        ext._knn_exact(y, 10)
+    assert e.value.code == _lib.E_NON_FINITE and e.value.row == first
+    got, st = ext._knn_exact(x, 10)
+    assert st["blocks_skipped"] > 0
+    (widx, wcos, wnorms), _ = oracle.knn_blocked(x, 10, sort_by_norm=False, rows=(19500, 20500))
+    np.testing.assert_array_equal(got[0][19500:20500], widx[19500:20500])
+    np.testing.assert_array_equal(got[1][19500:20500], wcos[19500:20500])
+    np.testing.assert_array_equal(got[2], wnorms)

@@ -687,7 +687,7 @@ __global__ __launch_bounds__(64) void rescore_pair_kernel(
   __shared__ uint32_t s_id[2][32];
   __shared__ uint32_t s_fill[2];
   const uint32_t half = tid_x() >> 5, lane = tid_x() & 31;
-  const uint32_t qq = bid_x() * 2 + half;
+  const uint32_t qq = xcd_place(bid_x(), gdim_x()) * 2 + half;
   const bool valid = qq < nq;
   const uint32_t i = valid ? (rowlist ? rowlist[qq] : perm ? perm[qq] : qq) : 0;
   rescore_half<V4>(s_sc[half], s_id[half], &s_fill[half], valid, lane, x, d, norms, k, kp, nsplit, rowlist ? qq : i,
@@ -715,7 +715,7 @@ __global__ __launch_bounds__(rescore_threads(CMAX)) void rescore_kernel(
     uint32_t* __restrict__ short_count, unsigned long long* __restrict__ err_max) {
   __shared__ double s_sc[CMAX];
   __shared__ uint32_t s_id[CMAX];
-  for (uint32_t qq = bid_x(); qq < nq; qq += gdim_x()) {
+  for (uint32_t qq = xcd_place(bid_x(), gdim_x()); qq < nq; qq += gdim_x()) {
     const uint32_t i = rowlist ? rowlist[qq] : perm ? perm[qq] : qq;
     rescore_row<CMAX, V4, rescore_threads(CMAX)>(s_sc, s_id, x, d, norms, k, kp, nsplit, rowlist ? qq : i, i, cand_idx,
                                                   cand_sc, cand_cnt, cand_floor, all_admitted, eps_list, eps32, select,

@@ -50,7 +50,7 @@ PRECISION_OF = {"fp32": "fp32", "fp16": "fp16", "fp16x3": "fp16x3", "fp16x3+lear
 # factors from a 1 GiB device copy under the same counters; rocprofv3 --pmc runs of this script,
 # mean over the first passes of the run (profiles/, one summary per kernel and workload).
 MEASURED_TRAFFIC = {
-    ("rare_396460x15", "fp16x3+learned"): 8455.9e6,
+    ("rare_396460x15", "fp16x3+learned"): 8461.6e6,
     ("learned_1000000x40", "fp16x3+learned"): 95774.3e6,
     ("cytometry_86864x39", "fp16x3"): 1847.9e6,
     ("counts_3605x2000", "fp32"): 1015.0e6 + 57.3e6,
